@@ -1,0 +1,109 @@
+"""ctypes binding of ``libchx.so`` (C ABI declared in ``include/chx.h``).
+
+The library is loaded from the package directory. There is no fallback: if
+the library is missing or cannot be loaded, importing this module raises
+ImportError, and every device entry point fails loudly.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_int, c_size_t, c_uint8, c_uint32, c_void_p
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("CHX_LIBRARY", os.path.join(_HERE, "libchx.so"))
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"HIP library not built: {LIB_PATH} is missing. Build it with "
+        "`python -c 'import __graft_entry__ as g; g.build()'` or `python -m cuhepy_amd._build`."
+    )
+
+try:
+    lib = ctypes.CDLL(LIB_PATH)
+except OSError as e:  # pragma: no cover - depends on the ROCm runtime
+    raise ImportError(f"cannot load {LIB_PATH}: {e}") from e
+
+u32p = POINTER(c_uint32)
+u8p = POINTER(c_uint8)
+vp = c_void_p
+
+CHX_OK, CHX_EINVAL, CHX_EDEVICE, CHX_ENOMEM, CHX_EUNSUPPORTED = 0, 1, 2, 3, 4
+OP_MODEXP, OP_MULMOD, OP_LOOKUP, OP_RAGGED = 0, 1, 2, 3
+
+# name -> (restype, argtypes); the list is the exported surface of chx.h
+SIGNATURES = {
+    "chx_version": (c_char_p, []),
+    "chx_last_error": (c_int, [c_char_p, c_size_t]),
+    "chx_ctx_create": (c_int, [c_int, POINTER(vp)]),
+    "chx_ctx_destroy": (c_int, [vp]),
+    "chx_ctx_set_stream": (c_int, [vp, vp]),
+    "chx_ctx_get_stream": (vp, [vp]),
+    "chx_ctx_sync": (c_int, [vp]),
+    "chx_ctx_set_tuning": (c_int, [vp, c_int, c_int, c_int, c_int]),
+    "chx_supported_tpi": (c_int, [c_int, POINTER(c_int), c_int]),
+    "chx_ctx_scratch_bytes": (c_size_t, [vp]),
+    "chx_mod_create": (c_int, [vp, u32p, c_int, POINTER(vp)]),
+    "chx_mod_destroy": (c_int, [vp]),
+    "chx_mod_limbs": (c_int, [vp]),
+    "chx_mulmod": (c_int, [vp, vp, u32p, u32p, u32p, c_size_t]),
+    "chx_mulmod_dev": (c_int, [vp, vp, vp, vp, vp, c_size_t, c_size_t]),
+    "chx_mulmod_bcast_dev": (c_int, [vp, vp, vp, u32p, vp, c_size_t, c_size_t]),
+    "chx_modexp": (c_int, [vp, vp, u32p, c_size_t, u32p, c_int, u32p]),
+    "chx_modexp_dev": (c_int, [vp, vp, vp, vp, c_size_t, c_size_t, u32p, c_int]),
+    "chx_paillier_key_create": (c_int, [vp, u32p, c_int, u32p, u32p, u32p, u32p, c_int, u32p, u32p, u32p,
+                                        POINTER(vp)]),
+    "chx_paillier_key_destroy": (c_int, [vp]),
+    "chx_paillier_width": (c_int, [vp]),
+    "chx_paillier_encrypt": (c_int, [vp, vp, u32p, u32p, u32p, c_size_t]),
+    "chx_paillier_encrypt_dev": (c_int, [vp, vp, vp, vp, vp, c_size_t, c_size_t]),
+    "chx_paillier_decrypt": (c_int, [vp, vp, u32p, u32p, c_size_t]),
+    "chx_paillier_decrypt_dev": (c_int, [vp, vp, vp, vp, c_size_t, c_size_t]),
+    "chx_paillier_decrypt_crt": (c_int, [vp, vp, u32p, u32p, c_size_t]),
+    "chx_paillier_decrypt_crt_dev": (c_int, [vp, vp, vp, vp, c_size_t, c_size_t]),
+    "chx_paillier_decode_hamming": (c_int, [vp, vp, u32p, u32p, u32p, c_size_t]),
+    "chx_lookup_key_create": (c_int, [vp, u32p, c_int, u32p, c_int, u32p, u32p, c_int, u32p, c_int,
+                                      POINTER(vp)]),
+    "chx_lookup_key_destroy": (c_int, [vp]),
+    "chx_lookup_width": (c_int, [vp]),
+    "chx_lookup_encrypt": (c_int, [vp, vp, u8p, u8p, c_int, u32p, c_size_t]),
+    "chx_lookup_encrypt_dev": (c_int, [vp, vp, vp, vp, c_int, vp, c_size_t, c_size_t]),
+    "chx_lookup_decrypt": (c_int, [vp, vp, u32p, u32p, c_size_t]),
+    "chx_lookup_decode_hamming": (c_int, [vp, vp, u32p, u32p, u32p, c_size_t]),
+    "chx_aos_to_il_dev": (c_int, [vp, vp, vp, c_size_t, c_int, c_size_t]),
+    "chx_il_to_aos_dev": (c_int, [vp, vp, vp, c_size_t, c_int, c_size_t]),
+    "chx_host_schedule": (c_int, [u32p, c_int, c_int, u32p, c_int]),
+    "chx_host_mont_consts": (c_int, [u32p, c_int, u32p, u32p]),
+    "chx_probe_mad": (c_int, [vp, c_int, c_int, POINTER(c_double), POINTER(c_double)]),
+    "chx_ctx_set_timing": (c_int, [vp, c_int]),
+    "chx_ctx_kernel_time": (c_int, [vp, c_char_p, POINTER(c_double), POINTER(c_int)]),
+    "chx_ctx_reset_timing": (c_int, [vp]),
+}
+
+for _name, (_res, _args) in SIGNATURES.items():
+    _f = getattr(lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+
+def last_error() -> str:
+    buf = ctypes.create_string_buffer(1024)
+    lib.chx_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+class ChxError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"chx error {code}: {message}")
+        self.code = code
+
+
+def check(rc: int) -> None:
+    """Map a C-ABI status to a Python exception (ValueError for bad input)."""
+    if rc == CHX_OK:
+        return
+    msg = last_error()
+    if rc == CHX_EINVAL:
+        raise ValueError(msg)
+    raise ChxError(rc, msg)

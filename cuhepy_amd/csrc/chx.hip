@@ -1,0 +1,1140 @@
+// C-ABI implementation of the gfx950 Paillier library (declared in include/chx.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/chx.h"
+#include "host_bn.hpp"
+#include "kernels.hpp"
+
+using namespace chx;
+using chx::host::BN;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_last_error;
+
+namespace {
+
+struct ChxError : std::runtime_error {
+  int code;
+  ChxError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] void fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  throw ChxError(code, buf);
+}
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) fail(CHX_EDEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return CHX_OK;
+  } catch (const ChxError& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::invalid_argument& e) {
+    g_last_error = e.what();
+    return CHX_EINVAL;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return CHX_ENOMEM;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return CHX_EDEVICE;
+  }
+}
+
+size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
+
+// ---------------------------------------------------------------------------
+// device buffers
+// ---------------------------------------------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void ensure(size_t need) {
+    if (need <= bytes) return;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    size_t sz = std::max<size_t>(need, 1 << 20);
+    hipError_t e = hipMalloc(&p, sz);
+    if (e != hipSuccess) fail(CHX_ENOMEM, "hipMalloc(%zu): %s", sz, hipGetErrorString(e));
+    bytes = sz;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T> T* as() { return reinterpret_cast<T*>(p); }
+};
+
+}  // namespace
+
+enum Slot { S_IN0, S_IN1, S_OUT, S_AOS, S_TAB, S_TMP, S_AUX, S_NSLOT };
+
+struct TimedLaunch {
+  std::string name;
+  hipEvent_t start, stop;
+};
+
+struct chx_ctx {
+  int device = 0;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  DevBuf slot[S_NSLOT];
+  int tune_tpi[4][3] = {};  // [op][limbs 32/64/128]
+  int tune_window[4][3] = {};
+  bool timing = false;
+  std::vector<TimedLaunch> pending;
+  std::map<std::string, std::pair<double, int>> stats;
+};
+
+struct chx_mod {
+  chx_ctx* ctx;
+  host::MontConsts hc;
+  ModDev* d = nullptr;
+};
+
+struct ExpSched {
+  std::vector<uint32_t> h;
+  uint32_t* d = nullptr;
+  int window = 0;
+  int bits = 0;
+};
+
+struct chx_pkey {
+  chx_ctx* ctx;
+  int W = 0;  // limbs of n^2
+  int H = 0;  // W / 2
+  bool crt = false;
+  host::MontConsts n2, n;
+  PaillierDev* d = nullptr;
+  BN e_n, e_phi, e_pm1, e_qm1;
+  std::map<std::pair<int, int>, ExpSched> scheds;  // (which, window) -> schedule
+};
+
+struct chx_lkey {
+  chx_ctx* ctx;
+  int W = 0;
+  int chunks = 0;
+  int noise = 0;
+  host::MontConsts n2, n;
+  ModDev* d_n2 = nullptr;
+  ModDev* d_n = nullptr;
+  uint32_t* d_gar = nullptr;   // g_a_inv * R mod n
+  uint32_t* d_gtab = nullptr;  // Montgomery-form g table, AoS
+  uint32_t* d_ntab = nullptr;  // Montgomery-form noise table, AoS
+  BN e_a;
+  std::map<int, ExpSched> scheds;
+};
+
+namespace {
+
+int lidx(int limbs) { return limbs == 32 ? 0 : limbs == 64 ? 1 : limbs == 128 ? 2 : -1; }
+
+// lanes-per-bignum values instantiated for each width
+const std::vector<int>& built_tpi(int limbs) {
+  static const std::vector<int> t32 = {1, 4};
+  static const std::vector<int> t64 = {4, 16};
+  static const std::vector<int> t128 = {8, 32};
+  static const std::vector<int> none = {};
+  return limbs == 32 ? t32 : limbs == 64 ? t64 : limbs == 128 ? t128 : none;
+}
+
+// Number of lanes that keeps every SIMD of the chip busy with 2 waves.
+size_t target_lanes(int device) {
+  static std::map<int, size_t> cache;
+  auto it = cache.find(device);
+  if (it != cache.end()) return it->second;
+  hipDeviceProp_t prop;
+  size_t v = 256 * 4 * 64 * 2;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+    v = (size_t)prop.multiProcessorCount * 4 * 64 * 2;
+  cache[device] = v;
+  return v;
+}
+
+// Default lanes per bignum: the smallest built TPI whose lanes fill the chip,
+// else the largest built one (small batches want many lanes per bignum).
+int pick_tpi(chx_ctx* ctx, int op, int limbs, size_t count) {
+  const int li = lidx(limbs);
+  if (li < 0) fail(CHX_EUNSUPPORTED, "width of %d limbs is not built", limbs);
+  const auto& opts = built_tpi(limbs);
+  const int forced = ctx->tune_tpi[op][li];
+  if (forced) {
+    if (std::find(opts.begin(), opts.end(), forced) == opts.end())
+      fail(CHX_EUNSUPPORTED, "tpi=%d is not built for %d limbs", forced, limbs);
+    return forced;
+  }
+  const size_t want = target_lanes(ctx->device);
+  for (int t : opts)
+    if (count * (size_t)t >= want) return t;
+  return opts.back();
+}
+
+int pick_window(chx_ctx* ctx, int op, int limbs, int ebits) {
+  const int li = lidx(limbs);
+  const int forced = li >= 0 ? ctx->tune_window[op][li] : 0;
+  if (forced) return forced;
+  return host::default_window(ebits);
+}
+
+void record_start(chx_ctx* ctx, const char* name) {
+  if (!ctx->timing) return;
+  TimedLaunch t;
+  t.name = name;
+  hip_check(hipEventCreate(&t.start), "hipEventCreate");
+  hip_check(hipEventCreate(&t.stop), "hipEventCreate");
+  hip_check(hipEventRecord(t.start, ctx->stream), "hipEventRecord");
+  ctx->pending.push_back(t);
+}
+
+void record_stop(chx_ctx* ctx) {
+  if (!ctx->timing || ctx->pending.empty()) return;
+  hip_check(hipEventRecord(ctx->pending.back().stop, ctx->stream), "hipEventRecord");
+}
+
+void flush_timing(chx_ctx* ctx) {
+  for (auto& t : ctx->pending) {
+    hip_check(hipEventSynchronize(t.stop), "hipEventSynchronize");
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, t.start, t.stop), "hipEventElapsedTime");
+    auto& s = ctx->stats[t.name];
+    s.first += ms;
+    s.second += 1;
+    (void)hipEventDestroy(t.start);
+    (void)hipEventDestroy(t.stop);
+  }
+  ctx->pending.clear();
+}
+
+void check_launch(const char* what) { hip_check(hipGetLastError(), what); }
+
+size_t lds_bytes(int L, int tpi) { return (size_t)WAVES_PER_BLOCK * (L / 4) * (64 / tpi) * 16; }
+
+dim3 grid_for(size_t count, int tpi, int rows = 1) {
+  const size_t per_block = BLOCK_THREADS / tpi;
+  return dim3((unsigned)((count + per_block - 1) / per_block), rows, 1);
+}
+
+// ---------------------------------------------------------------------------
+// dispatch over (L, TPI)
+// ---------------------------------------------------------------------------
+#define CHX_DISPATCH(L_, TPI_, CALL)                                                           \
+  do {                                                                                         \
+    const int l__ = (L_), t__ = (TPI_);                                                        \
+    if (l__ == 32 && t__ == 1) { constexpr int L = 32, TPI = 1; CALL; }                        \
+    else if (l__ == 32 && t__ == 4) { constexpr int L = 32, TPI = 4; CALL; }                   \
+    else if (l__ == 64 && t__ == 4) { constexpr int L = 64, TPI = 4; CALL; }                   \
+    else if (l__ == 64 && t__ == 16) { constexpr int L = 64, TPI = 16; CALL; }                 \
+    else if (l__ == 128 && t__ == 8) { constexpr int L = 128, TPI = 8; CALL; }                 \
+    else if (l__ == 128 && t__ == 32) { constexpr int L = 128, TPI = 32; CALL; }               \
+    else fail(CHX_EUNSUPPORTED, "no kernel built for %d limbs with %d lanes per bignum", l__, t__); \
+  } while (0)
+
+// CRT halves: H in {32, 64}
+#define CHX_DISPATCH_H(H_, TPI_, CALL)                                                         \
+  do {                                                                                         \
+    const int h__ = (H_), t__ = (TPI_);                                                        \
+    if (h__ == 32 && t__ == 1) { constexpr int H = 32, TPI = 1; CALL; }                        \
+    else if (h__ == 32 && t__ == 4) { constexpr int H = 32, TPI = 4; CALL; }                   \
+    else if (h__ == 64 && t__ == 4) { constexpr int H = 64, TPI = 4; CALL; }                   \
+    else if (h__ == 64 && t__ == 16) { constexpr int H = 64, TPI = 16; CALL; }                 \
+    else fail(CHX_EUNSUPPORTED, "no CRT kernel built for %d-limb halves with %d lanes", h__, t__); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// host <-> device helpers
+// ---------------------------------------------------------------------------
+void fill_moddev(ModDev& md, const host::MontConsts& c) {
+  memset(&md, 0, sizeof md);
+  md.L = (u32)c.L;
+  md.n0inv = c.n0inv;
+  md.bits = (u32)c.bits;
+  for (int i = 0; i < c.L; ++i) {
+    md.N[i] = c.N[i];
+    md.R1[i] = c.R1[i];
+    md.R2[i] = c.R2[i];
+    md.R3[i] = c.R3[i];
+  }
+}
+
+void fill_val(ValDev& v, const BN& x) {
+  memset(&v, 0, sizeof v);
+  for (size_t i = 0; i < x.size() && i < (size_t)LMAX; ++i) v.v[i] = x[i];
+}
+
+template <class T>
+T* dev_upload(const T* host, size_t n, hipStream_t s) {
+  T* d = nullptr;
+  hipError_t e = hipMalloc(&d, sizeof(T) * std::max<size_t>(n, 1));
+  if (e != hipSuccess) fail(CHX_ENOMEM, "hipMalloc: %s", hipGetErrorString(e));
+  hip_check(hipMemcpyAsync(d, host, sizeof(T) * n, hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
+  hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  return d;
+}
+
+int pick_width(int bits_needed) {
+  if (bits_needed <= 32 * 32) return 32;
+  if (bits_needed <= 64 * 32) return 64;
+  if (bits_needed <= 128 * 32) return 128;
+  fail(CHX_EUNSUPPORTED, "values of %d bits exceed the widest built width (4096 bits)", bits_needed);
+}
+
+// validate that every item of an AoS batch is below the modulus
+void check_below(const uint32_t* v, size_t count, int limbs, const BN& N, const char* what) {
+  for (size_t i = 0; i < count; ++i) {
+    const uint32_t* x = v + i * limbs;
+    int cmp = 0;
+    for (int j = limbs - 1; j >= 0 && cmp == 0; --j) {
+      const uint32_t nj = j < (int)N.size() ? N[j] : 0u;
+      if (x[j] != nj) cmp = x[j] < nj ? -1 : 1;
+    }
+    if (cmp >= 0) fail(CHX_EINVAL, "%s[%zu] is not below the modulus", what, i);
+  }
+}
+
+void aos_to_il(chx_ctx* ctx, const uint32_t* aos, uint32_t* il, size_t count, int L, size_t stride) {
+  dim3 g((unsigned)((stride + 31) / 32), (unsigned)((L + 31) / 32));
+  hipLaunchKernelGGL(k_aos_to_il, g, dim3(256), 0, ctx->stream, aos, il, (int)count, L, (int)stride);
+  check_launch("k_aos_to_il");
+}
+
+void il_to_aos(chx_ctx* ctx, const uint32_t* il, uint32_t* aos, size_t count, int L, size_t stride) {
+  dim3 g((unsigned)((count + 31) / 32), (unsigned)((L + 31) / 32));
+  hipLaunchKernelGGL(k_il_to_aos, g, dim3(256), 0, ctx->stream, il, aos, (int)count, L, (int)stride);
+  check_launch("k_il_to_aos");
+}
+
+// upload an AoS host batch into an interleaved device slot
+uint32_t* upload_il(chx_ctx* ctx, Slot slot, const uint32_t* host, size_t count, int L, size_t stride) {
+  const size_t bytes = count * L * sizeof(uint32_t);
+  ctx->slot[S_AOS].ensure(bytes);
+  ctx->slot[slot].ensure(stride * L * sizeof(uint32_t));
+  hip_check(hipMemcpyAsync(ctx->slot[S_AOS].p, host, bytes, hipMemcpyHostToDevice, ctx->stream), "H2D");
+  aos_to_il(ctx, ctx->slot[S_AOS].as<uint32_t>(), ctx->slot[slot].as<uint32_t>(), count, L, stride);
+  return ctx->slot[slot].as<uint32_t>();
+}
+
+void download_il(chx_ctx* ctx, const uint32_t* il, uint32_t* host, size_t count, int L, size_t stride) {
+  const size_t bytes = count * L * sizeof(uint32_t);
+  ctx->slot[S_AOS].ensure(bytes);
+  il_to_aos(ctx, il, ctx->slot[S_AOS].as<uint32_t>(), count, L, stride);
+  hip_check(hipMemcpyAsync(host, ctx->slot[S_AOS].p, bytes, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+  hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+}
+
+const ExpSched& get_sched_p(chx_pkey* k, int which, int window) {
+  auto key = std::make_pair(which, window);
+  auto it = k->scheds.find(key);
+  if (it != k->scheds.end()) return it->second;
+  const BN& e = which == 0 ? k->e_n : which == 1 ? k->e_phi : which == 2 ? k->e_pm1 : k->e_qm1;
+  ExpSched s;
+  s.window = window;
+  s.bits = host::bn_bits(e);
+  s.h = host::sliding_schedule(e, window);
+  if (s.h.empty()) fail(CHX_EINVAL, "zero exponent");
+  s.d = dev_upload(s.h.data(), s.h.size(), k->ctx->stream);
+  return k->scheds.emplace(key, std::move(s)).first->second;
+}
+
+int exp_bits_p(const chx_pkey* k, int which) {
+  const BN& e = which == 0 ? k->e_n : which == 1 ? k->e_phi : which == 2 ? k->e_pm1 : k->e_qm1;
+  return host::bn_bits(e);
+}
+
+// --------------------------------------------------------------------------
+// launchers (device-resident, interleaved)
+// --------------------------------------------------------------------------
+void launch_mulmod(chx_ctx* ctx, const ModDev* md, int L, const uint32_t* a, const uint32_t* b, uint32_t* out,
+                   size_t count, size_t stride) {
+  const int tpi = pick_tpi(ctx, CHX_OP_MULMOD, L, count);
+  record_start(ctx, "mulmod");
+  CHX_DISPATCH(L, tpi, {
+    hipLaunchKernelGGL((k_mulmod<L, TPI>), grid_for(count, TPI), dim3(BLOCK_THREADS), lds_bytes(L, TPI),
+                       ctx->stream, md, a, b, out, (int)count, (int)stride);
+  });
+  record_stop(ctx);
+  check_launch("k_mulmod");
+}
+
+void launch_modexp(chx_ctx* ctx, const ModDev* md, int L, const uint32_t* base, uint32_t* out, size_t count,
+                   size_t stride, const ExpSched& s) {
+  const int tpi = pick_tpi(ctx, CHX_OP_MODEXP, L, count);
+  const int tentries = 1 << (s.window - 1);
+  const size_t tstride = round_up(count, BLOCK_THREADS);
+  ctx->slot[S_TAB].ensure((size_t)tentries * L * tstride * sizeof(uint32_t));
+  record_start(ctx, "modexp");
+  CHX_DISPATCH(L, tpi, {
+    hipLaunchKernelGGL((k_modexp<L, TPI>), grid_for(count, TPI), dim3(BLOCK_THREADS), lds_bytes(L, TPI),
+                       ctx->stream, md, base, out, (int)count, (int)stride, s.d, (int)s.h.size(), tentries,
+                       ctx->slot[S_TAB].as<uint32_t>(), (int)tstride);
+  });
+  record_stop(ctx);
+  check_launch("k_modexp");
+}
+
+void launch_decrypt(chx_ctx* ctx, const ModDev* n2, const ModDev* nm, const uint32_t* cR, int L,
+                    const uint32_t* c, uint32_t* out, size_t count, size_t stride, const ExpSched& s,
+                    const uint32_t* mask, uint32_t* pop) {
+  const int tpi = pick_tpi(ctx, CHX_OP_MODEXP, L, count);
+  const int tentries = 1 << (s.window - 1);
+  const size_t tstride = round_up(count, BLOCK_THREADS);
+  ctx->slot[S_TAB].ensure((size_t)tentries * L * tstride * sizeof(uint32_t));
+  record_start(ctx, "decrypt");
+  CHX_DISPATCH(L, tpi, {
+    hipLaunchKernelGGL((k_decrypt<L, TPI>), grid_for(count, TPI), dim3(BLOCK_THREADS), lds_bytes(L, TPI),
+                       ctx->stream, n2, nm, cR, c, out, (int)count, (int)stride, s.d, (int)s.h.size(),
+                       tentries, ctx->slot[S_TAB].as<uint32_t>(), (int)tstride, mask, pop);
+  });
+  record_stop(ctx);
+  check_launch("k_decrypt");
+}
+
+void launch_encrypt(chx_pkey* k, const uint32_t* m, const uint32_t* r, uint32_t* c, size_t count,
+                    size_t stride) {
+  chx_ctx* ctx = k->ctx;
+  const int L = k->W;
+  const int tpi = pick_tpi(ctx, CHX_OP_MODEXP, L, count);
+  const ExpSched& s = get_sched_p(k, 0, pick_window(ctx, CHX_OP_MODEXP, L, exp_bits_p(k, 0)));
+  const int tentries = 1 << (s.window - 1);
+  const size_t tstride = round_up(count, BLOCK_THREADS);
+  ctx->slot[S_TAB].ensure((size_t)tentries * L * tstride * sizeof(uint32_t));
+  record_start(ctx, "encrypt");
+  CHX_DISPATCH(L, tpi, {
+    hipLaunchKernelGGL((k_encrypt<L, TPI>), grid_for(count, TPI), dim3(BLOCK_THREADS), lds_bytes(L, TPI),
+                       ctx->stream, k->d, m, r, c, (int)count, (int)stride, s.d, (int)s.h.size(), tentries,
+                       ctx->slot[S_TAB].as<uint32_t>(), (int)tstride);
+  });
+  record_stop(ctx);
+  check_launch("k_encrypt");
+}
+
+void launch_crt(chx_pkey* k, const uint32_t* c, uint32_t* m, size_t count, size_t stride) {
+  chx_ctx* ctx = k->ctx;
+  if (!k->crt) fail(CHX_EINVAL, "key was created without CRT factors");
+  const int H = k->H;
+  const int tpi = pick_tpi(ctx, CHX_OP_MODEXP, H, 2 * count);
+  const int w = pick_window(ctx, CHX_OP_MODEXP, H, std::max(exp_bits_p(k, 2), exp_bits_p(k, 3)));
+  const ExpSched& sp = get_sched_p(k, 2, w);
+  const ExpSched& sq = get_sched_p(k, 3, w);
+  const int tentries = 1 << (w - 1);
+  const size_t tstride = round_up(count, BLOCK_THREADS);
+  ctx->slot[S_TAB].ensure((size_t)2 * tentries * H * tstride * sizeof(uint32_t));
+  ctx->slot[S_TMP].ensure((size_t)2 * H * stride * sizeof(uint32_t));
+  uint32_t* halves = ctx->slot[S_TMP].as<uint32_t>();
+  record_start(ctx, "crt_half");
+  CHX_DISPATCH_H(H, tpi, {
+    hipLaunchKernelGGL((k_crt_half<H, TPI>), grid_for(count, TPI, 2), dim3(BLOCK_THREADS), lds_bytes(H, TPI),
+                       ctx->stream, k->d, c, halves, (int)count, (int)stride, sp.d, (int)sp.h.size(), sq.d,
+                       (int)sq.h.size(), tentries, ctx->slot[S_TAB].as<uint32_t>(), (int)tstride);
+  });
+  record_stop(ctx);
+  check_launch("k_crt_half");
+  const int tpi2 = pick_tpi(ctx, CHX_OP_MULMOD, H, count);
+  record_start(ctx, "crt_combine");
+  CHX_DISPATCH_H(H, tpi2, {
+    hipLaunchKernelGGL((k_crt_combine<H, TPI>), grid_for(count, TPI), dim3(BLOCK_THREADS), lds_bytes(H, TPI),
+                       ctx->stream, k->d, halves, m, (int)count, (int)stride);
+  });
+  record_stop(ctx);
+  check_launch("k_crt_combine");
+}
+
+const ExpSched& get_sched_l(chx_lkey* k, int window) {
+  auto it = k->scheds.find(window);
+  if (it != k->scheds.end()) return it->second;
+  ExpSched s;
+  s.window = window;
+  s.bits = host::bn_bits(k->e_a);
+  s.h = host::sliding_schedule(k->e_a, window);
+  if (s.h.empty()) fail(CHX_EINVAL, "zero exponent");
+  s.d = dev_upload(s.h.data(), s.h.size(), k->ctx->stream);
+  return k->scheds.emplace(window, std::move(s)).first->second;
+}
+
+void launch_lookup_encrypt(chx_lkey* k, const uint8_t* digits, const uint8_t* nidx, int nmult, uint32_t* c,
+                           size_t count, size_t stride) {
+  chx_ctx* ctx = k->ctx;
+  const int L = k->W;
+  const int tpi = pick_tpi(ctx, CHX_OP_LOOKUP, L, count);
+  record_start(ctx, "lookup_encrypt");
+  CHX_DISPATCH(L, tpi, {
+    hipLaunchKernelGGL((k_lookup_encrypt<L, TPI>), grid_for(count, TPI), dim3(BLOCK_THREADS),
+                       lds_bytes(L, TPI), ctx->stream, k->d_n2, k->d_gtab, k->d_ntab, digits, k->chunks, nidx,
+                       nmult, k->noise, c, (int)count, (int)stride);
+  });
+  record_stop(ctx);
+  check_launch("k_lookup_encrypt");
+}
+
+template <class F>
+int with_ctx(chx_ctx* ctx, F&& f) {
+  if (!ctx) {
+    g_last_error = "null context";
+    return CHX_EINVAL;
+  }
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  return guarded([&] {
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    f();
+  });
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* chx_version(void) { return "cuhepy_amd-chx 0.1 (gfx950)"; }
+
+int chx_last_error(char* buf, size_t size) {
+  if (buf && size) {
+    size_t n = std::min(size - 1, g_last_error.size());
+    memcpy(buf, g_last_error.data(), n);
+    buf[n] = 0;
+  }
+  return (int)g_last_error.size();
+}
+
+int chx_ctx_create(int device, chx_ctx** out) {
+  return guarded([&] {
+    if (!out) fail(CHX_EINVAL, "null out pointer");
+    int n = 0;
+    hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    if (device < 0 || device >= n) fail(CHX_EINVAL, "device %d not present (%d visible)", device, n);
+    hip_check(hipSetDevice(device), "hipSetDevice");
+    chx_ctx* c = new chx_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete c;
+      fail(CHX_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    c->stream = c->own;
+    *out = c;
+  });
+}
+
+int chx_ctx_destroy(chx_ctx* ctx) {
+  if (!ctx) return CHX_OK;
+  int rc = guarded([&] {
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    (void)hipStreamSynchronize(ctx->stream);
+    flush_timing(ctx);
+    for (auto& b : ctx->slot) b.release();
+    if (ctx->own) (void)hipStreamDestroy(ctx->own);
+  });
+  delete ctx;
+  return rc;
+}
+
+int chx_ctx_set_stream(chx_ctx* ctx, void* stream) {
+  return with_ctx(ctx, [&] { ctx->stream = stream ? (hipStream_t)stream : ctx->own; });
+}
+
+void* chx_ctx_get_stream(chx_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int chx_ctx_sync(chx_ctx* ctx) {
+  return with_ctx(ctx, [&] { hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize"); });
+}
+
+int chx_ctx_set_tuning(chx_ctx* ctx, int op, int limbs, int tpi, int window) {
+  return with_ctx(ctx, [&] {
+    const int li = lidx(limbs);
+    if (op < 0 || op > 3 || li < 0) fail(CHX_EINVAL, "bad op %d or limbs %d", op, limbs);
+    if (tpi) {
+      const auto& opts = built_tpi(limbs);
+      if (std::find(opts.begin(), opts.end(), tpi) == opts.end())
+        fail(CHX_EUNSUPPORTED, "tpi=%d is not built for %d limbs", tpi, limbs);
+    }
+    if (window < 0 || window > 7) fail(CHX_EINVAL, "window %d out of range [1, 7]", window);
+    ctx->tune_tpi[op][li] = tpi;
+    ctx->tune_window[op][li] = window;
+  });
+}
+
+int chx_supported_tpi(int limbs, int* tpis, int cap) {
+  const auto& v = built_tpi(limbs);
+  for (int i = 0; i < (int)v.size() && i < cap; ++i) tpis[i] = v[i];
+  return (int)v.size();
+}
+
+size_t chx_ctx_scratch_bytes(chx_ctx* ctx) {
+  if (!ctx) return 0;
+  size_t s = 0;
+  for (auto& b : ctx->slot) s += b.bytes;
+  return s;
+}
+
+int chx_mod_create(chx_ctx* ctx, const uint32_t* n, int limbs, chx_mod** out) {
+  return with_ctx(ctx, [&] {
+    if (!n || !out) fail(CHX_EINVAL, "null argument");
+    if (lidx(limbs) < 0) fail(CHX_EUNSUPPORTED, "limbs must be 32, 64 or 128 (got %d)", limbs);
+    chx_mod* m = new chx_mod();
+    m->ctx = ctx;
+    try {
+      m->hc = host::mont_consts(host::bn_from(n, limbs, limbs));
+      ModDev md;
+      fill_moddev(md, m->hc);
+      m->d = dev_upload(&md, 1, ctx->stream);
+    } catch (...) {
+      delete m;
+      throw;
+    }
+    *out = m;
+  });
+}
+
+int chx_mod_destroy(chx_mod* m) {
+  if (!m) return CHX_OK;
+  int rc = guarded([&] {
+    hip_check(hipSetDevice(m->ctx->device), "hipSetDevice");
+    if (m->d) (void)hipFree(m->d);
+  });
+  delete m;
+  return rc;
+}
+
+int chx_mod_limbs(const chx_mod* m) { return m ? m->hc.L : 0; }
+
+int chx_mulmod(chx_ctx* ctx, const chx_mod* m, const uint32_t* a, const uint32_t* b, uint32_t* out,
+               size_t count) {
+  return with_ctx(ctx, [&] {
+    if (!m || !a || !b || !out) fail(CHX_EINVAL, "null argument");
+    if (count == 0) return;
+    const int L = m->hc.L;
+    const size_t stride = round_up(count, 64);
+    uint32_t* da = upload_il(ctx, S_IN0, a, count, L, stride);
+    uint32_t* db = upload_il(ctx, S_IN1, b, count, L, stride);
+    ctx->slot[S_OUT].ensure(stride * L * 4);
+    launch_mulmod(ctx, m->d, L, da, db, ctx->slot[S_OUT].as<uint32_t>(), count, stride);
+    download_il(ctx, ctx->slot[S_OUT].as<uint32_t>(), out, count, L, stride);
+  });
+}
+
+int chx_mulmod_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* a, const uint32_t* b, uint32_t* out,
+                   size_t count, size_t stride) {
+  return with_ctx(ctx, [&] {
+    if (!m || !a || !b || !out) fail(CHX_EINVAL, "null argument");
+    if (count == 0) return;
+    if (stride < round_up(count, 64)) fail(CHX_EINVAL, "stride %zu below count rounded to 64", stride);
+    launch_mulmod(ctx, m->d, m->hc.L, a, b, out, count, stride);
+  });
+}
+
+int chx_mulmod_bcast_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* x, const uint32_t* y_host,
+                         uint32_t* out, size_t count, size_t stride) {
+  return with_ctx(ctx, [&] {
+    if (!m || !x || !y_host || !out) fail(CHX_EINVAL, "null argument");
+    if (count == 0) return;
+    const int L = m->hc.L;
+    if (stride < round_up(count, 64)) fail(CHX_EINVAL, "stride %zu below count rounded to 64", stride);
+    BN yR = host::to_mont_host(host::bn_mod(host::bn_from(y_host, L, L), m->hc.N), m->hc);
+    ctx->slot[S_AUX].ensure(L * 4);
+    hip_check(hipMemcpyAsync(ctx->slot[S_AUX].p, yR.data(), L * 4, hipMemcpyHostToDevice, ctx->stream), "H2D");
+    const int tpi = pick_tpi(ctx, CHX_OP_MULMOD, L, count);
+    record_start(ctx, "mulmod_bcast");
+    CHX_DISPATCH(L, tpi, {
+      hipLaunchKernelGGL((k_mulmod_bcast<L, TPI>), grid_for(count, TPI), dim3(BLOCK_THREADS), lds_bytes(L, TPI),
+                         ctx->stream, m->d, x, ctx->slot[S_AUX].as<uint32_t>(), out, (int)count, (int)stride);
+    });
+    record_stop(ctx);
+    check_launch("k_mulmod_bcast");
+  });
+}
+
+static ExpSched make_sched(chx_ctx* ctx, const uint32_t* e, int e_limbs, int L, int op) {
+  BN eb = host::bn_from(e, e_limbs, e_limbs);
+  ExpSched s;
+  s.bits = host::bn_bits(eb);
+  s.window = pick_window(ctx, op, L, s.bits);
+  s.h = host::sliding_schedule(eb, s.window);
+  return s;
+}
+
+int chx_modexp(chx_ctx* ctx, const chx_mod* m, const uint32_t* base, size_t count, const uint32_t* e,
+               int e_limbs, uint32_t* out) {
+  return with_ctx(ctx, [&] {
+    if (!m || !base || !out || (!e && e_limbs)) fail(CHX_EINVAL, "null argument");
+    if (count == 0) return;
+    const int L = m->hc.L;
+    check_below(base, count, L, m->hc.N, "base");
+    ExpSched s = make_sched(ctx, e, e_limbs, L, CHX_OP_MODEXP);
+    if (s.h.empty()) {  // e == 0: result 1 (N > 1)
+      std::vector<uint32_t> one((size_t)count * L, 0u);
+      for (size_t i = 0; i < count; ++i) one[i * L] = 1u;
+      memcpy(out, one.data(), one.size() * 4);
+      return;
+    }
+    const size_t stride = round_up(count, 64);
+    uint32_t* db = upload_il(ctx, S_IN0, base, count, L, stride);
+    ctx->slot[S_AUX].ensure(s.h.size() * 4);
+    hip_check(hipMemcpyAsync(ctx->slot[S_AUX].p, s.h.data(), s.h.size() * 4, hipMemcpyHostToDevice, ctx->stream),
+              "H2D");
+    s.d = ctx->slot[S_AUX].as<uint32_t>();
+    ctx->slot[S_OUT].ensure(stride * L * 4);
+    launch_modexp(ctx, m->d, L, db, ctx->slot[S_OUT].as<uint32_t>(), count, stride, s);
+    download_il(ctx, ctx->slot[S_OUT].as<uint32_t>(), out, count, L, stride);
+    s.d = nullptr;
+  });
+}
+
+int chx_modexp_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* base, uint32_t* out, size_t count,
+                   size_t stride, const uint32_t* e, int e_limbs) {
+  return with_ctx(ctx, [&] {
+    if (!m || !base || !out || !e) fail(CHX_EINVAL, "null argument");
+    if (count == 0) return;
+    if (stride < round_up(count, 64)) fail(CHX_EINVAL, "stride %zu below count rounded to 64", stride);
+    const int L = m->hc.L;
+    ExpSched s = make_sched(ctx, e, e_limbs, L, CHX_OP_MODEXP);
+    if (s.h.empty()) fail(CHX_EINVAL, "zero exponent is handled by the host-buffer entry point only");
+    ctx->slot[S_AUX].ensure(s.h.size() * 4);
+    hip_check(hipMemcpyAsync(ctx->slot[S_AUX].p, s.h.data(), s.h.size() * 4, hipMemcpyHostToDevice, ctx->stream),
+              "H2D");
+    s.d = ctx->slot[S_AUX].as<uint32_t>();
+    launch_modexp(ctx, m->d, L, base, out, count, stride, s);
+    s.d = nullptr;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Paillier
+// ---------------------------------------------------------------------------
+int chx_paillier_key_create(chx_ctx* ctx, const uint32_t* n, int n_limbs, const uint32_t* phi,
+                            const uint32_t* inv, const uint32_t* p, const uint32_t* q, int half_limbs,
+                            const uint32_t* hp, const uint32_t* hq, const uint32_t* qinv, chx_pkey** out) {
+  return with_ctx(ctx, [&] {
+    if (!n || !phi || !inv || !out || n_limbs <= 0 || n_limbs > 64) fail(CHX_EINVAL, "bad key arguments");
+    BN nb = host::bn_from(n, n_limbs, n_limbs);
+    const int nbits = host::bn_bits(nb);
+    if (nbits < 8) fail(CHX_EINVAL, "modulus too small");
+    const int W = pick_width(2 * nbits);
+    std::unique_ptr<chx_pkey> k(new chx_pkey());
+    k->ctx = ctx;
+    k->W = W;
+    k->H = W / 2;
+    // n^2
+    BN nW = host::bn_from(n, n_limbs, W);
+    BN n2 = host::bn_sqr_lo(nW);
+    k->n2 = host::mont_consts(n2);
+    k->n = host::mont_consts(nW);
+    BN phiW = host::bn_from(phi, n_limbs, W);
+    BN invW = host::bn_from(inv, n_limbs, W);
+    if (host::bn_cmp(invW, nW) >= 0) fail(CHX_EINVAL, "inv must be below n");
+    k->e_n = nW;
+    k->e_phi = phiW;
+    std::unique_ptr<PaillierDev> pd(new PaillierDev());
+    memset(pd.get(), 0, sizeof(PaillierDev));
+    fill_moddev(pd->n2, k->n2);
+    fill_moddev(pd->n, k->n);
+    // n * R mod n^2: n < n^2, shift n by 32W bits mod n^2
+    fill_val(pd->nR, host::to_mont_host(nW, k->n2));
+    fill_val(pd->muR, host::to_mont_host(invW, k->n));
+    if (p && q && hp && hq && qinv && half_limbs > 0) {
+      const int H = W / 2;
+      if (half_limbs > H) fail(CHX_EINVAL, "factors wider than half the width");
+      if (H != 32 && H != 64) fail(CHX_EUNSUPPORTED, "CRT needs 32- or 64-limb halves (width %d)", W);
+      BN pb = host::bn_from(p, half_limbs, H), qb = host::bn_from(q, half_limbs, H);
+      // p^2, q^2 at H limbs
+      auto sq = [](const BN& x) { return host::bn_sqr_lo(x); };
+      host::MontConsts cp2 = host::mont_consts(sq(pb)), cq2 = host::mont_consts(sq(qb));
+      host::MontConsts cp = host::mont_consts(pb), cq = host::mont_consts(qb);
+      BN nH = host::bn_from(n, n_limbs, H);
+      host::MontConsts cn = host::mont_consts(nH);
+      fill_moddev(pd->p2, cp2);
+      fill_moddev(pd->q2, cq2);
+      fill_moddev(pd->p, cp);
+      fill_moddev(pd->q, cq);
+      fill_moddev(pd->nh, cn);
+      fill_val(pd->hpR, host::to_mont_host(host::bn_from(hp, half_limbs, H), cp));
+      fill_val(pd->hqR, host::to_mont_host(host::bn_from(hq, half_limbs, H), cq));
+      fill_val(pd->qinvR, host::to_mont_host(host::bn_from(qinv, half_limbs, H), cp));
+      fill_val(pd->qRn, host::to_mont_host(qb, cn));
+      BN one(H, 0u);
+      one[0] = 1;
+      k->e_pm1 = pb;
+      host::bn_sub(k->e_pm1, one);
+      k->e_qm1 = qb;
+      host::bn_sub(k->e_qm1, one);
+      k->crt = true;
+    }
+    k->d = dev_upload(pd.get(), 1, ctx->stream);
+    *out = k.release();
+  });
+}
+
+int chx_paillier_key_destroy(chx_pkey* k) {
+  if (!k) return CHX_OK;
+  int rc = guarded([&] {
+    hip_check(hipSetDevice(k->ctx->device), "hipSetDevice");
+    if (k->d) (void)hipFree(k->d);
+    for (auto& s : k->scheds)
+      if (s.second.d) (void)hipFree(s.second.d);
+  });
+  delete k;
+  return rc;
+}
+
+int chx_paillier_width(const chx_pkey* k) { return k ? k->W : 0; }
+
+int chx_paillier_encrypt(chx_ctx* ctx, const chx_pkey* kc, const uint32_t* m, const uint32_t* r, uint32_t* c,
+                         size_t count) {
+  chx_pkey* k = const_cast<chx_pkey*>(kc);
+  return with_ctx(ctx, [&] {
+    if (!k || !m || !r || !c) fail(CHX_EINVAL, "null argument");
+    if (k->ctx != ctx) fail(CHX_EINVAL, "key belongs to another context");
+    if (count == 0) return;
+    const int L = k->W;
+    check_below(m, count, L, k->n.N, "m");
+    check_below(r, count, L, k->n.N, "r");
+    const size_t stride = round_up(count, 64);
+    uint32_t* dm = upload_il(ctx, S_IN0, m, count, L, stride);
+    uint32_t* dr = upload_il(ctx, S_IN1, r, count, L, stride);
+    ctx->slot[S_OUT].ensure(stride * L * 4);
+    launch_encrypt(k, dm, dr, ctx->slot[S_OUT].as<uint32_t>(), count, stride);
+    download_il(ctx, ctx->slot[S_OUT].as<uint32_t>(), c, count, L, stride);
+  });
+}
+
+int chx_paillier_encrypt_dev(chx_ctx* ctx, const chx_pkey* kc, const uint32_t* m, const uint32_t* r,
+                             uint32_t* c, size_t count, size_t stride) {
+  chx_pkey* k = const_cast<chx_pkey*>(kc);
+  return with_ctx(ctx, [&] {
+    if (!k || !m || !r || !c) fail(CHX_EINVAL, "null argument");
+    if (count == 0) return;
+    if (stride < round_up(count, 64)) fail(CHX_EINVAL, "stride %zu below count rounded to 64", stride);
+    launch_encrypt(k, m, r, c, count, stride);
+  });
+}
+
+static void paillier_decrypt_impl(chx_pkey* k, const uint32_t* dc, uint32_t* dm, size_t count, size_t stride,
+                                  const uint32_t* mask, uint32_t* pop) {
+  chx_ctx* ctx = k->ctx;
+  const int L = k->W;
+  const ExpSched& s = get_sched_p(k, 1, pick_window(ctx, CHX_OP_MODEXP, L, exp_bits_p(k, 1)));
+  PaillierDev* d = k->d;
+  launch_decrypt(ctx, &d->n2, &d->n, d->muR.v, L, dc, dm, count, stride, s, mask, pop);
+}
+
+int chx_paillier_decrypt(chx_ctx* ctx, const chx_pkey* kc, const uint32_t* c, uint32_t* m, size_t count) {
+  chx_pkey* k = const_cast<chx_pkey*>(kc);
+  return with_ctx(ctx, [&] {
+    if (!k || !c || !m) fail(CHX_EINVAL, "null argument");
+    if (count == 0) return;
+    const int L = k->W;
+    check_below(c, count, L, k->n2.N, "c");
+    const size_t stride = round_up(count, 64);
+    uint32_t* dc = upload_il(ctx, S_IN0, c, count, L, stride);
+    ctx->slot[S_OUT].ensure(stride * L * 4);
+    paillier_decrypt_impl(k, dc, ctx->slot[S_OUT].as<uint32_t>(), count, stride, nullptr, nullptr);
+    download_il(ctx, ctx->slot[S_OUT].as<uint32_t>(), m, count, L, stride);
+  });
+}
+
+int chx_paillier_decrypt_dev(chx_ctx* ctx, const chx_pkey* kc, const uint32_t* c, uint32_t* m, size_t count,
+                             size_t stride) {
+  chx_pkey* k = const_cast<chx_pkey*>(kc);
+  return with_ctx(ctx, [&] {
+    if (!k || !c || !m) fail(CHX_EINVAL, "null argument");
+    if (count == 0) return;
+    if (stride < round_up(count, 64)) fail(CHX_EINVAL, "stride %zu below count rounded to 64", stride);
+    paillier_decrypt_impl(k, c, m, count, stride, nullptr, nullptr);
+  });
+}
+
+int chx_paillier_decrypt_crt(chx_ctx* ctx, const chx_pkey* kc, const uint32_t* c, uint32_t* m, size_t count) {
+  chx_pkey* k = const_cast<chx_pkey*>(kc);
+  return with_ctx(ctx, [&] {
+    if (!k || !c || !m) fail(CHX_EINVAL, "null argument");
+    if (count == 0) return;
+    const int L = k->W;
+    check_below(c, count, L, k->n2.N, "c");
+    const size_t stride = round_up(count, 64);
+    uint32_t* dc = upload_il(ctx, S_IN0, c, count, L, stride);
+    ctx->slot[S_OUT].ensure(stride * k->H * 4);
+    launch_crt(k, dc, ctx->slot[S_OUT].as<uint32_t>(), count, stride);
+    download_il(ctx, ctx->slot[S_OUT].as<uint32_t>(), m, count, k->H, stride);
+  });
+}
+
+int chx_paillier_decrypt_crt_dev(chx_ctx* ctx, const chx_pkey* kc, const uint32_t* c, uint32_t* m,
+                                 size_t count, size_t stride) {
+  chx_pkey* k = const_cast<chx_pkey*>(kc);
+  return with_ctx(ctx, [&] {
+    if (!k || !c || !m) fail(CHX_EINVAL, "null argument");
+    if (count == 0) return;
+    if (stride < round_up(count, 64)) fail(CHX_EINVAL, "stride %zu below count rounded to 64", stride);
+    launch_crt(k, c, m, count, stride);
+  });
+}
+
+int chx_paillier_decode_hamming(chx_ctx* ctx, const chx_pkey* kc, const uint32_t* c, const uint32_t* mask,
+                                uint32_t* counts, size_t count) {
+  chx_pkey* k = const_cast<chx_pkey*>(kc);
+  return with_ctx(ctx, [&] {
+    if (!k || !c || !mask || !counts) fail(CHX_EINVAL, "null argument");
+    if (count == 0) return;
+    const int L = k->W;
+    check_below(c, count, L, k->n2.N, "c");
+    const size_t stride = round_up(count, 64);
+    uint32_t* dc = upload_il(ctx, S_IN0, c, count, L, stride);
+    ctx->slot[S_IN1].ensure(L * 4);
+    hip_check(hipMemcpyAsync(ctx->slot[S_IN1].p, mask, L * 4, hipMemcpyHostToDevice, ctx->stream), "H2D");
+    ctx->slot[S_OUT].ensure(count * 4);
+    paillier_decrypt_impl(k, dc, nullptr, count, stride, ctx->slot[S_IN1].as<uint32_t>(),
+                          ctx->slot[S_OUT].as<uint32_t>());
+    hip_check(hipMemcpyAsync(counts, ctx->slot[S_OUT].p, count * 4, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+    hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  });
+}
+
+// ---------------------------------------------------------------------------
+// lookup
+// ---------------------------------------------------------------------------
+int chx_lookup_key_create(chx_ctx* ctx, const uint32_t* n, int n_limbs, const uint32_t* a, int a_limbs,
+                          const uint32_t* g_a_inv, const uint32_t* g_table, int chunks,
+                          const uint32_t* noise_table, int noise_entries, chx_lkey** out) {
+  return with_ctx(ctx, [&] {
+    if (!n || !a || !g_a_inv || !g_table || !noise_table || !out) fail(CHX_EINVAL, "null argument");
+    if (n_limbs <= 0 || n_limbs > 64 || chunks <= 0 || noise_entries <= 0 || noise_entries > 256)
+      fail(CHX_EINVAL, "bad sizes");
+    BN nb = host::bn_from(n, n_limbs, n_limbs);
+    const int W = pick_width(2 * host::bn_bits(nb));
+    std::unique_ptr<chx_lkey> k(new chx_lkey());
+    k->ctx = ctx;
+    k->W = W;
+    k->chunks = chunks;
+    k->noise = noise_entries;
+    BN nW = host::bn_from(n, n_limbs, W);
+    BN n2 = host::bn_sqr_lo(nW);
+    k->n2 = host::mont_consts(n2);
+    k->n = host::mont_consts(nW);
+    k->e_a = host::bn_from(a, a_limbs, std::max(a_limbs, 1));
+    ModDev md;
+    fill_moddev(md, k->n2);
+    k->d_n2 = dev_upload(&md, 1, ctx->stream);
+    fill_moddev(md, k->n);
+    k->d_n = dev_upload(&md, 1, ctx->stream);
+    BN gar = host::to_mont_host(host::bn_from(g_a_inv, n_limbs, W), k->n);
+    k->d_gar = dev_upload(gar.data(), W, ctx->stream);
+    // tables to Montgomery form (host shift-and-reduce; once per key)
+    const size_t gentries = (size_t)chunks * 256;
+    std::vector<uint32_t> gm(gentries * W), nm((size_t)noise_entries * W);
+    for (size_t e = 0; e < gentries; ++e) {
+      BN v = host::bn_from(g_table + e * W, W, W);
+      BN vm = host::to_mont_host(v, k->n2);
+      std::copy(vm.begin(), vm.end(), gm.begin() + e * W);
+    }
+    for (int e = 0; e < noise_entries; ++e) {
+      BN v = host::bn_from(noise_table + (size_t)e * W, W, W);
+      BN vm = host::to_mont_host(v, k->n2);
+      std::copy(vm.begin(), vm.end(), nm.begin() + (size_t)e * W);
+    }
+    k->d_gtab = dev_upload(gm.data(), gm.size(), ctx->stream);
+    k->d_ntab = dev_upload(nm.data(), nm.size(), ctx->stream);
+    *out = k.release();
+  });
+}
+
+int chx_lookup_key_destroy(chx_lkey* k) {
+  if (!k) return CHX_OK;
+  int rc = guarded([&] {
+    hip_check(hipSetDevice(k->ctx->device), "hipSetDevice");
+    for (void* p : {(void*)k->d_n2, (void*)k->d_n, (void*)k->d_gar, (void*)k->d_gtab, (void*)k->d_ntab})
+      if (p) (void)hipFree(p);
+    for (auto& s : k->scheds)
+      if (s.second.d) (void)hipFree(s.second.d);
+  });
+  delete k;
+  return rc;
+}
+
+int chx_lookup_width(const chx_lkey* k) { return k ? k->W : 0; }
+
+int chx_lookup_encrypt(chx_ctx* ctx, const chx_lkey* kc, const uint8_t* digits, const uint8_t* noise_idx,
+                       int nmult, uint32_t* c, size_t count) {
+  chx_lkey* k = const_cast<chx_lkey*>(kc);
+  return with_ctx(ctx, [&] {
+    if (!k || !digits || !c || (nmult > 0 && !noise_idx)) fail(CHX_EINVAL, "null argument");
+    if (count == 0) return;
+    if (nmult < 0 || nmult > 64) fail(CHX_EINVAL, "nmult out of range");
+    for (size_t i = 0; i < count * (size_t)nmult; ++i)
+      if (noise_idx[i] >= k->noise) fail(CHX_EINVAL, "noise index %u out of range", noise_idx[i]);
+    const int L = k->W;
+    const size_t stride = round_up(count, 64);
+    const size_t dbytes = count * (size_t)k->chunks, nbytes = count * (size_t)nmult;
+    ctx->slot[S_IN0].ensure(dbytes + 16);
+    ctx->slot[S_IN1].ensure(nbytes + 16);
+    hip_check(hipMemcpyAsync(ctx->slot[S_IN0].p, digits, dbytes, hipMemcpyHostToDevice, ctx->stream), "H2D");
+    if (nbytes)
+      hip_check(hipMemcpyAsync(ctx->slot[S_IN1].p, noise_idx, nbytes, hipMemcpyHostToDevice, ctx->stream), "H2D");
+    ctx->slot[S_OUT].ensure(stride * L * 4);
+    launch_lookup_encrypt(k, ctx->slot[S_IN0].as<uint8_t>(), ctx->slot[S_IN1].as<uint8_t>(), nmult,
+                          ctx->slot[S_OUT].as<uint32_t>(), count, stride);
+    download_il(ctx, ctx->slot[S_OUT].as<uint32_t>(), c, count, L, stride);
+  });
+}
+
+int chx_lookup_encrypt_dev(chx_ctx* ctx, const chx_lkey* kc, const uint8_t* digits, const uint8_t* noise_idx,
+                           int nmult, uint32_t* c, size_t count, size_t stride) {
+  chx_lkey* k = const_cast<chx_lkey*>(kc);
+  return with_ctx(ctx, [&] {
+    if (!k || !digits || !c || (nmult > 0 && !noise_idx)) fail(CHX_EINVAL, "null argument");
+    if (count == 0) return;
+    if (stride < round_up(count, 64)) fail(CHX_EINVAL, "stride %zu below count rounded to 64", stride);
+    launch_lookup_encrypt(k, digits, noise_idx, nmult, c, count, stride);
+  });
+}
+
+static void lookup_decrypt_impl(chx_lkey* k, const uint32_t* dc, uint32_t* dm, size_t count, size_t stride,
+                                const uint32_t* mask, uint32_t* pop) {
+  chx_ctx* ctx = k->ctx;
+  const int L = k->W;
+  const ExpSched& s = get_sched_l(k, pick_window(ctx, CHX_OP_MODEXP, L, host::bn_bits(k->e_a)));
+  launch_decrypt(ctx, k->d_n2, k->d_n, k->d_gar, L, dc, dm, count, stride, s, mask, pop);
+}
+
+int chx_lookup_decrypt(chx_ctx* ctx, const chx_lkey* kc, const uint32_t* c, uint32_t* m, size_t count) {
+  chx_lkey* k = const_cast<chx_lkey*>(kc);
+  return with_ctx(ctx, [&] {
+    if (!k || !c || !m) fail(CHX_EINVAL, "null argument");
+    if (count == 0) return;
+    const int L = k->W;
+    check_below(c, count, L, k->n2.N, "c");
+    const size_t stride = round_up(count, 64);
+    uint32_t* dc = upload_il(ctx, S_IN0, c, count, L, stride);
+    ctx->slot[S_OUT].ensure(stride * L * 4);
+    lookup_decrypt_impl(k, dc, ctx->slot[S_OUT].as<uint32_t>(), count, stride, nullptr, nullptr);
+    download_il(ctx, ctx->slot[S_OUT].as<uint32_t>(), m, count, L, stride);
+  });
+}
+
+int chx_lookup_decode_hamming(chx_ctx* ctx, const chx_lkey* kc, const uint32_t* c, const uint32_t* mask,
+                              uint32_t* counts, size_t count) {
+  chx_lkey* k = const_cast<chx_lkey*>(kc);
+  return with_ctx(ctx, [&] {
+    if (!k || !c || !mask || !counts) fail(CHX_EINVAL, "null argument");
+    if (count == 0) return;
+    const int L = k->W;
+    check_below(c, count, L, k->n2.N, "c");
+    const size_t stride = round_up(count, 64);
+    uint32_t* dc = upload_il(ctx, S_IN0, c, count, L, stride);
+    ctx->slot[S_IN1].ensure(L * 4);
+    hip_check(hipMemcpyAsync(ctx->slot[S_IN1].p, mask, L * 4, hipMemcpyHostToDevice, ctx->stream), "H2D");
+    ctx->slot[S_OUT].ensure(count * 4);
+    lookup_decrypt_impl(k, dc, nullptr, count, stride, ctx->slot[S_IN1].as<uint32_t>(),
+                        ctx->slot[S_OUT].as<uint32_t>());
+    hip_check(hipMemcpyAsync(counts, ctx->slot[S_OUT].p, count * 4, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+    hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  });
+}
+
+// ---------------------------------------------------------------------------
+// layout + measurement
+// ---------------------------------------------------------------------------
+int chx_aos_to_il_dev(chx_ctx* ctx, const uint32_t* aos, uint32_t* il, size_t count, int limbs, size_t stride) {
+  return with_ctx(ctx, [&] {
+    if (!aos || !il || limbs <= 0) fail(CHX_EINVAL, "bad argument");
+    if (stride < count) fail(CHX_EINVAL, "stride below count");
+    if (count) aos_to_il(ctx, aos, il, count, limbs, stride);
+  });
+}
+
+int chx_il_to_aos_dev(chx_ctx* ctx, const uint32_t* il, uint32_t* aos, size_t count, int limbs, size_t stride) {
+  return with_ctx(ctx, [&] {
+    if (!aos || !il || limbs <= 0) fail(CHX_EINVAL, "bad argument");
+    if (stride < count) fail(CHX_EINVAL, "stride below count");
+    if (count) il_to_aos(ctx, il, aos, count, limbs, stride);
+  });
+}
+
+int chx_host_schedule(const uint32_t* e, int e_limbs, int window, uint32_t* out, int cap) {
+  int n = -1;
+  int rc = guarded([&] {
+    if (!e || e_limbs <= 0 || window < 1 || window > 7) fail(CHX_EINVAL, "bad argument");
+    std::vector<uint32_t> s = host::sliding_schedule(host::bn_from(e, e_limbs, e_limbs), window);
+    for (int i = 0; i < (int)s.size() && i < cap; ++i) out[i] = s[i];
+    n = (int)s.size();
+  });
+  return rc == CHX_OK ? n : -rc;
+}
+
+int chx_host_mont_consts(const uint32_t* n, int limbs, uint32_t* n0inv, uint32_t* r2) {
+  return guarded([&] {
+    if (!n || !n0inv || !r2 || limbs <= 0) fail(CHX_EINVAL, "bad argument");
+    host::MontConsts c = host::mont_consts(host::bn_from(n, limbs, limbs));
+    *n0inv = c.n0inv;
+    for (int i = 0; i < limbs; ++i) r2[i] = c.R2[i];
+  });
+}
+
+int chx_probe_mad(chx_ctx* ctx, int blocks, int iters, double* seconds, double* macs) {
+  return with_ctx(ctx, [&] {
+    if (blocks <= 0 || iters <= 0 || !seconds || !macs) fail(CHX_EINVAL, "bad argument");
+    ctx->slot[S_AUX].ensure((size_t)blocks * 256 * 8);
+    hipEvent_t a, b;
+    hip_check(hipEventCreate(&a), "hipEventCreate");
+    hip_check(hipEventCreate(&b), "hipEventCreate");
+    hipLaunchKernelGGL(k_probe_mad, dim3(blocks), dim3(256), 0, ctx->stream, ctx->slot[S_AUX].as<u64>(), 8, 7u);
+    hip_check(hipEventRecord(a, ctx->stream), "hipEventRecord");
+    hipLaunchKernelGGL(k_probe_mad, dim3(blocks), dim3(256), 0, ctx->stream, ctx->slot[S_AUX].as<u64>(), iters, 7u);
+    hip_check(hipEventRecord(b, ctx->stream), "hipEventRecord");
+    check_launch("k_probe_mad");
+    hip_check(hipEventSynchronize(b), "hipEventSynchronize");
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *seconds = ms * 1e-3;
+    *macs = (double)blocks * 256.0 * (double)iters * 16.0;
+  });
+}
+
+int chx_ctx_set_timing(chx_ctx* ctx, int enable) {
+  return with_ctx(ctx, [&] { ctx->timing = enable != 0; });
+}
+
+int chx_ctx_kernel_time(chx_ctx* ctx, const char* name, double* total_ms, int* launches) {
+  return with_ctx(ctx, [&] {
+    if (!name || !total_ms || !launches) fail(CHX_EINVAL, "null argument");
+    flush_timing(ctx);
+    auto it = ctx->stats.find(name);
+    *total_ms = it == ctx->stats.end() ? 0.0 : it->second.first;
+    *launches = it == ctx->stats.end() ? 0 : it->second.second;
+  });
+}
+
+int chx_ctx_reset_timing(chx_ctx* ctx) {
+  return with_ctx(ctx, [&] {
+    flush_timing(ctx);
+    ctx->stats.clear();
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,203 @@
+// Host-side fixed-width bignum helpers: Montgomery constants, conversions to
+// Montgomery form and the sliding-window exponent schedule. Values are
+// little-endian vectors of 32-bit limbs. Everything here runs once per key or
+// per exponent, never per item, so plain shift-and-subtract arithmetic is used.
+#pragma once
+#include <stdint.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace chx {
+namespace host {
+
+typedef std::vector<uint32_t> BN;
+
+inline BN bn_from(const uint32_t* p, int n, int width) {
+  if (n > width) {
+    for (int i = width; i < n; ++i)
+      if (p[i]) throw std::invalid_argument("value does not fit in the target width");
+    n = width;
+  }
+  BN r(width, 0u);
+  for (int i = 0; i < n; ++i) r[i] = p[i];
+  return r;
+}
+
+inline int bn_cmp(const BN& a, const BN& b) {
+  for (int i = (int)a.size() - 1; i >= 0; --i) {
+    if (a[i] != b[i]) return a[i] < b[i] ? -1 : 1;
+  }
+  return 0;
+}
+
+inline bool bn_is_zero(const BN& a) {
+  for (uint32_t w : a)
+    if (w) return false;
+  return true;
+}
+
+inline int bn_bits(const BN& a) {
+  for (int i = (int)a.size() - 1; i >= 0; --i) {
+    if (a[i]) return 32 * i + 32 - __builtin_clz(a[i]);
+  }
+  return 0;
+}
+
+// x*x truncated to x.size() limbs (callers pass values whose square fits)
+inline BN bn_sqr_lo(const BN& x) {
+  const size_t W = x.size();
+  BN r(W, 0u);
+  for (size_t i = 0; i < W; ++i) {
+    uint64_t c = 0;
+    for (size_t j = 0; i + j < W; ++j) {
+      uint64_t t = (uint64_t)x[i] * x[j] + r[i + j] + c;
+      r[i + j] = (uint32_t)t;
+      c = t >> 32;
+    }
+  }
+  return r;
+}
+
+// a -= b, returns borrow
+inline uint32_t bn_sub(BN& a, const BN& b) {
+  uint64_t br = 0;
+  for (size_t i = 0; i < a.size(); ++i) {
+    uint64_t d = (uint64_t)a[i] - b[i] - br;
+    a[i] = (uint32_t)d;
+    br = (d >> 63) & 1;
+  }
+  return (uint32_t)br;
+}
+
+// a = 2a mod n, for a < n (a, n same width; one extra carry bit tracked)
+inline void bn_dbl_mod(BN& a, const BN& n) {
+  uint32_t carry = 0;
+  for (size_t i = 0; i < a.size(); ++i) {
+    uint32_t nc = a[i] >> 31;
+    a[i] = (a[i] << 1) | carry;
+    carry = nc;
+  }
+  if (carry || bn_cmp(a, n) >= 0) bn_sub(a, n);
+}
+
+// v * 2^k mod n (v < n)
+inline BN bn_shl_mod(BN v, int k, const BN& n) {
+  for (int i = 0; i < k; ++i) bn_dbl_mod(v, n);
+  return v;
+}
+
+// x mod n for arbitrary x of the same width (binary long division)
+inline BN bn_mod(const BN& x, const BN& n) {
+  BN r(n.size(), 0u);
+  for (int bit = bn_bits(x) - 1; bit >= 0; --bit) {
+    bn_dbl_mod(r, n);
+    if ((x[bit / 32] >> (bit % 32)) & 1u) {
+      // r = r + 1 mod n; r < n and r even after doubling unless it wrapped, so
+      // r + 1 <= n; subtract n when equal
+      uint64_t c = 1;
+      for (size_t i = 0; i < r.size() && c; ++i) {
+        uint64_t s = (uint64_t)r[i] + c;
+        r[i] = (uint32_t)s;
+        c = s >> 32;
+      }
+      if (bn_cmp(r, n) >= 0) bn_sub(r, n);
+    }
+  }
+  return r;
+}
+
+// -n^-1 mod 2^32 for odd n0 (Newton iteration)
+inline uint32_t mont_n0inv(uint32_t n0) {
+  uint32_t inv = 1;
+  for (int i = 0; i < 5; ++i) inv *= 2u - n0 * inv;
+  return (uint32_t)(0u - inv);
+}
+
+struct MontConsts {
+  int L = 0;
+  uint32_t n0inv = 0;
+  int bits = 0;
+  BN N, R1, R2, R3;
+};
+
+inline MontConsts mont_consts(const BN& N) {
+  if (N.empty() || !(N[0] & 1u)) throw std::invalid_argument("modulus must be odd");
+  MontConsts c;
+  c.L = (int)N.size();
+  c.N = N;
+  c.bits = bn_bits(N);
+  c.n0inv = mont_n0inv(N[0]);
+  BN one(N.size(), 0u);
+  one[0] = 1;
+  if (bn_cmp(one, N) >= 0) throw std::invalid_argument("modulus must exceed 1");
+  c.R1 = bn_shl_mod(one, 32 * c.L, N);
+  c.R2 = bn_shl_mod(c.R1, 32 * c.L, N);
+  c.R3 = bn_shl_mod(c.R2, 32 * c.L, N);
+  return c;
+}
+
+// v * R mod N (v < N required)
+inline BN to_mont_host(const BN& v, const MontConsts& c) {
+  if (bn_cmp(v, c.N) >= 0) throw std::invalid_argument("value must be below the modulus");
+  return bn_shl_mod(v, 32 * c.L, c.N);
+}
+
+// Left-to-right sliding-window schedule (Menezes, van Oorschot, Vanstone,
+// Handbook of Applied Cryptography, Alg. 14.85) for exponent e with window w.
+// Output: sched[0] = index of the first odd window value ((v-1)/2);
+// sched[s >= 1] = (squarings << 16) | (index + 1), index + 1 == 0 = no multiply.
+// e == 0 yields an empty schedule (the caller returns 1).
+inline std::vector<uint32_t> sliding_schedule(const BN& e, int w) {
+  std::vector<uint32_t> s;
+  int i = bn_bits(e) - 1;
+  auto bit = [&](int j) -> uint32_t { return (e[j / 32] >> (j % 32)) & 1u; };
+  uint32_t pending_sq = 0;
+  bool first = true;
+  while (i >= 0) {
+    if (!bit(i)) {
+      ++pending_sq;
+      --i;
+      continue;
+    }
+    int l = i - w + 1;
+    if (l < 0) l = 0;
+    while (!bit(l)) ++l;
+    uint32_t v = 0;
+    for (int j = i; j >= l; --j) v = (v << 1) | bit(j);
+    const uint32_t idx = (v - 1) / 2;
+    const uint32_t len = (uint32_t)(i - l + 1);
+    if (first) {
+      s.push_back(idx);
+      first = false;
+    } else {
+      pending_sq += len;
+      while (pending_sq > 0xFFFFu) {  // split very long runs of zeros
+        s.push_back(0xFFFFu << 16);
+        pending_sq -= 0xFFFFu;
+      }
+      s.push_back((pending_sq << 16) | (idx + 1));
+    }
+    pending_sq = 0;
+    i = l - 1;
+  }
+  if (!first && pending_sq) {
+    while (pending_sq > 0xFFFFu) {
+      s.push_back(0xFFFFu << 16);
+      pending_sq -= 0xFFFFu;
+    }
+    s.push_back(pending_sq << 16);
+  }
+  return s;
+}
+
+inline int default_window(int ebits) {
+  if (ebits >= 768) return 5;
+  if (ebits >= 192) return 4;
+  if (ebits >= 48) return 3;
+  return 2;
+}
+
+}  // namespace host
+}  // namespace chx

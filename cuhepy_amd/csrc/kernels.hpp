@@ -1,0 +1,511 @@
+// Batched Paillier kernels for gfx950 built on the lane-group Montgomery core.
+//
+// Device data layout ("limb-interleaved"): a batch of `count` bignums of L
+// limbs is stored as L rows of `stride` u32 words, limb j of item i at
+// [j*stride + i] (stride = count rounded up to 64). A wave that loads limb j
+// of consecutive items touches consecutive words, so HBM loads coalesce.
+//
+// Per-modulus constants live in a ModDev block in device memory; every kernel
+// reads the modulus into registers once and then works out of VGPRs/LDS.
+#pragma once
+#include "mont.hpp"
+
+namespace chx {
+
+constexpr int LMAX = 128;
+constexpr int WAVES_PER_BLOCK = 2;
+constexpr int BLOCK_THREADS = 64 * WAVES_PER_BLOCK;
+
+// Montgomery constants of one odd modulus N of `L` limbs, R = 2^(32L).
+struct ModDev {
+  u32 L;
+  u32 n0inv;  // -N^-1 mod 2^32
+  u32 bits;   // bit length of N
+  u32 pad;
+  u32 N[LMAX];
+  u32 R1[LMAX];  // R mod N      (Montgomery form of 1)
+  u32 R2[LMAX];  // R^2 mod N    (to Montgomery form)
+  u32 R3[LMAX];  // R^3 mod N    (Montgomery form of R; folds a 2L-limb input)
+};
+
+// A plain value of up to LMAX limbs (e.g. a constant already in Montgomery form).
+struct ValDev {
+  u32 v[LMAX];
+};
+
+// Everything a Paillier key needs on the device.
+//   width W = limbs of n^2 (64 or 128); n is zero-padded to W limbs.
+struct PaillierDev {
+  ModDev n2;      // n^2
+  ModDev n;       // n, padded to W limbs (exact division by n, arithmetic mod n)
+  ValDev nR;      // n * R mod n^2     (m*n = MontMul(m, nR))
+  ValDev muR;     // inv * R mod n     (decrypt: m = L(u) * inv mod n)
+  // CRT part (half width H = W/2): p^2, q^2, p, q and n at H limbs
+  ModDev p2, q2, p, q, nh;
+  ValDev hpR;     // h_p * R_H mod p
+  ValDev hqR;     // h_q * R_H mod q
+  ValDev qinvR;   // q^-1 * R_H mod p
+  ValDev qRn;     // q * R_H mod n     (q*h = MontMul_n(h, qRn))
+};
+
+template <int L>
+__device__ __forceinline__ size_t limb_index(int limb, int item, int stride) {
+  return (size_t)limb * (size_t)stride + (size_t)item;
+}
+
+// load / store a lane's K limbs of item `item` from an interleaved buffer
+template <int L, int TPI>
+__device__ __forceinline__ void load_dist(u32 (&x)[L / TPI], const u32* __restrict__ src, int item, int stride) {
+  constexpr int K = L / TPI;
+  const int r = Grp<TPI>::rank();
+#pragma unroll
+  for (int k = 0; k < K; ++k) x[k] = src[limb_index<L>(r * K + k, item, stride)];
+}
+
+template <int L, int TPI>
+__device__ __forceinline__ void store_dist(u32* __restrict__ dst, const u32 (&x)[L / TPI], int item, int stride) {
+  constexpr int K = L / TPI;
+  const int r = Grp<TPI>::rank();
+#pragma unroll
+  for (int k = 0; k < K; ++k) dst[limb_index<L>(r * K + k, item, stride)] = x[k];
+}
+
+// load a lane's slice of a constant bignum (same for all groups)
+template <int L, int TPI>
+__device__ __forceinline__ void load_const(u32 (&x)[L / TPI], const u32* __restrict__ src) {
+  constexpr int K = L / TPI;
+  const int r = Grp<TPI>::rank();
+#pragma unroll
+  for (int k = 0; k < K; ++k) x[k] = src[r * K + k];
+}
+
+template <int L, int TPI>
+__device__ __forceinline__ void set_zero(u32 (&x)[L / TPI]) {
+#pragma unroll
+  for (int k = 0; k < L / TPI; ++k) x[k] = 0;
+}
+
+template <int L, int TPI>
+__device__ __forceinline__ void set_one(u32 (&x)[L / TPI]) {
+  set_zero<L, TPI>(x);
+  x[0] = (Grp<TPI>::rank() == 0) ? 1u : 0u;
+}
+
+// per-wave LDS staging area
+template <int L, int TPI>
+__device__ __forceinline__ uint4* wave_lds(uint4* smem) {
+  constexpr int slots = (L / 4) * (64 / TPI);
+  return smem + (threadIdx.x / 64) * slots;
+}
+
+template <int L, int TPI>
+__device__ __forceinline__ void mont_init(Mont<L, TPI>& M, const ModDev* __restrict__ md, uint4* smem) {
+  M.lds = wave_lds<L, TPI>(smem);
+  M.n0inv = md->n0inv;
+  load_const<L, TPI>(M.N, md->N);
+}
+
+// item handled by the caller's group; groups past `count` work on a clamped
+// item (so every lane of a wave takes part in the cross-lane steps) and skip
+// their stores.
+template <int TPI>
+__device__ __forceinline__ int group_item() {
+  return (int)(blockIdx.x * (BLOCK_THREADS / TPI) + threadIdx.x / TPI);
+}
+
+// ---------------------------------------------------------------------------
+// Sliding-window exponentiation with a host-built schedule shared by all items
+// ---------------------------------------------------------------------------
+// sched[0] = index of the first window; sched[s] for s >= 1 packs
+// (squarings << 16) | (odd-power index + 1), index + 1 == 0 meaning "square
+// only". The table of odd powers x^1, x^3, ..., x^(2^w - 1) is kept per item
+// in global scratch (interleaved like every other batch buffer).
+template <int L, int TPI>
+__device__ void modexp_sched(Mont<L, TPI>& M, u32 (&x)[L / TPI], const u32* __restrict__ sched, int nsched,
+                             int tentries, u32* __restrict__ table, int item, int tstride) {
+  constexpr int K = L / TPI;
+  // table build: T0 = x, T_j = T_{j-1} * x^2
+  u32 x2[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) x2[k] = x[k];
+  M.sqr(x2);
+  store_dist<L, TPI>(table, x, item, tstride);
+  u32 cur[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) cur[k] = x[k];
+  for (int j = 1; j < tentries; ++j) {
+    M.mul_into(cur, x2);
+    store_dist<L, TPI>(table + (size_t)j * L * tstride, cur, item, tstride);
+  }
+  // first window
+  load_dist<L, TPI>(x, table + (size_t)sched[0] * L * tstride, item, tstride);
+  for (int s = 1; s < nsched; ++s) {
+    const u32 op = sched[s];
+    const int sq = (int)(op >> 16);
+    const int idx = (int)(op & 0xFFFFu);
+    for (int i = 0; i < sq; ++i) M.sqr(x);
+    if (idx) {
+      u32 y[K];
+      load_dist<L, TPI>(y, table + (size_t)(idx - 1) * L * tstride, item, tstride);
+      M.mul_into(x, y);
+    }
+  }
+}
+
+// to / from Montgomery form
+template <int L, int TPI>
+__device__ __forceinline__ void to_mont(Mont<L, TPI>& M, u32 (&x)[L / TPI], const ModDev* __restrict__ md) {
+  u32 r2[L / TPI];
+  load_const<L, TPI>(r2, md->R2);
+  M.mul_into(x, r2);
+}
+
+template <int L, int TPI>
+__device__ __forceinline__ void from_mont(Mont<L, TPI>& M, u32 (&x)[L / TPI]) {
+  u32 one[L / TPI];
+  set_one<L, TPI>(one);
+  M.mul_into(x, one);
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// out = a * b mod N (pointwise); a, b may be any values below R = 2^(32L)
+template <int L, int TPI>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_mulmod(const ModDev* __restrict__ md, const u32* __restrict__ A,
+                                                          const u32* __restrict__ B, u32* __restrict__ out,
+                                                          int count, int stride) {
+  extern __shared__ uint4 smem[];
+  constexpr int K = L / TPI;
+  Mont<L, TPI> M;
+  mont_init<L, TPI>(M, md, smem);
+  const int item = group_item<TPI>();
+  const int it = item < count ? item : count - 1;
+  u32 a[K], b[K], t[K], r2[K];
+  load_dist<L, TPI>(a, A, it, stride);
+  load_dist<L, TPI>(b, B, it, stride);
+  load_const<L, TPI>(r2, md->R2);
+  M.mul(t, b, r2);  // b*R mod N  (any b < R)
+  M.mul(b, a, t);   // a*b mod N  (any a < R)
+  if (item < count) store_dist<L, TPI>(out, b, item, stride);
+}
+
+// out_i = x_i * y mod N with one operand shared by the batch; yR = y*R mod N
+template <int L, int TPI>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_mulmod_bcast(const ModDev* __restrict__ md,
+                                                                const u32* __restrict__ X,
+                                                                const u32* __restrict__ yR, u32* __restrict__ out,
+                                                                int count, int stride) {
+  extern __shared__ uint4 smem[];
+  constexpr int K = L / TPI;
+  Mont<L, TPI> M;
+  mont_init<L, TPI>(M, md, smem);
+  const int item = group_item<TPI>();
+  const int it = item < count ? item : count - 1;
+  u32 x[K], y[K], t[K];
+  load_dist<L, TPI>(x, X, it, stride);
+  load_const<L, TPI>(y, yR);
+  M.mul(t, y, x);
+  if (item < count) store_dist<L, TPI>(out, t, item, stride);
+}
+
+// out = base^e mod N, e shared by the batch (schedule built on the host)
+template <int L, int TPI>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_modexp(const ModDev* __restrict__ md, const u32* __restrict__ base,
+                                                          u32* __restrict__ out, int count, int stride,
+                                                          const u32* __restrict__ sched, int nsched,
+                                                          int tentries, u32* __restrict__ table, int tstride) {
+  extern __shared__ uint4 smem[];
+  constexpr int K = L / TPI;
+  Mont<L, TPI> M;
+  mont_init<L, TPI>(M, md, smem);
+  const int item = group_item<TPI>();
+  const int it = item < count ? item : count - 1;
+  u32 x[K];
+  load_dist<L, TPI>(x, base, it, stride);
+  to_mont<L, TPI>(M, x, md);
+  modexp_sched<L, TPI>(M, x, sched, nsched, tentries, table, item, tstride);
+  from_mont<L, TPI>(M, x);
+  if (item < count) store_dist<L, TPI>(out, x, item, stride);
+}
+
+// m = ((u - 1) // n) * c mod n with Python floor-division semantics, as the
+// reference computes it (scheme.py:101-104, lookup.py:275-278):
+//   u >= 1:  w = u - 1, q = (w - (w mod n)) / n  (exact division by n)
+//   u == 0:  (-1) // n = -1, so q = n - 1 (= -1 mod n)
+// then MontMul_n(q, cR) = q * c mod n for cR = c*R mod n. For a valid
+// ciphertext u = 1 + q*n and w mod n = 0.
+template <int L, int TPI>
+__device__ __forceinline__ void paillier_finish(Mont<L, TPI>& Mn, const ModDev* __restrict__ mdn, u32 (&u)[L / TPI],
+                                                const u32* __restrict__ cR) {
+  constexpr int K = L / TPI;
+  u32 nz = 0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) nz |= u[k];
+  const bool is_zero = Grp<TPI>::group_bits(__ballot(nz != 0)) == 0ull;
+  (void)sub_small<K, TPI>(u, 1u);
+  // w mod n = MontMul(MontMul(w, R^2), 1)
+  u32 r0[K], r2[K];
+  load_const<L, TPI>(r2, mdn->R2);
+  Mn.mul(r0, u, r2);
+  from_mont<L, TPI>(Mn, r0);
+  u32 w[K];
+  (void)sub_dist<K, TPI>(w, u, r0);
+  u32 q[K];
+  Mn.exact_div(q, w);
+  u32 nm1[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) nm1[k] = Mn.N[k];
+  (void)sub_small<K, TPI>(nm1, 1u);
+#pragma unroll
+  for (int k = 0; k < K; ++k) q[k] = is_zero ? nm1[k] : q[k];
+  u32 c[K];
+  load_const<L, TPI>(c, cR);
+  Mn.mul(u, q, c);
+}
+
+template <int L, int TPI>
+__device__ __forceinline__ u32 group_popcount(const u32 (&m)[L / TPI], const u32* __restrict__ mask) {
+  constexpr int K = L / TPI;
+  u32 mk[K];
+  load_const<L, TPI>(mk, mask);
+  u32 c = 0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) c += __popc(m[k] & mk[k]);
+#pragma unroll
+  for (int off = 1; off < TPI; off <<= 1) c += (u32)__shfl_xor((int)c, off, 64);
+  return c;
+}
+
+// m = L(c^e mod n^2) * cR mod n  (plain decrypt e = phi, cR = inv*R;
+// lookup decrypt e = a, cR = g_a_inv*R). popcnt != nullptr: also write the
+// Hamming count of m under `mask`.
+template <int L, int TPI>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_decrypt(const ModDev* __restrict__ n2, const ModDev* __restrict__ nm,
+                                                           const u32* __restrict__ cR, const u32* __restrict__ C,
+                                                           u32* __restrict__ out, int count, int stride,
+                                                           const u32* __restrict__ sched, int nsched, int tentries,
+                                                           u32* __restrict__ table, int tstride,
+                                                           const u32* __restrict__ mask, u32* __restrict__ popcnt) {
+  extern __shared__ uint4 smem[];
+  constexpr int K = L / TPI;
+  const int item = group_item<TPI>();
+  const int it = item < count ? item : count - 1;
+  u32 x[K];
+  {
+    Mont<L, TPI> M;
+    mont_init<L, TPI>(M, n2, smem);
+    load_dist<L, TPI>(x, C, it, stride);
+    to_mont<L, TPI>(M, x, n2);
+    modexp_sched<L, TPI>(M, x, sched, nsched, tentries, table, item, tstride);
+    from_mont<L, TPI>(M, x);
+  }
+  Mont<L, TPI> Mn;
+  mont_init<L, TPI>(Mn, nm, smem);
+  paillier_finish<L, TPI>(Mn, nm, x, cR);
+  if (out != nullptr && item < count) store_dist<L, TPI>(out, x, item, stride);
+  if (popcnt != nullptr) {
+    const u32 c = group_popcount<L, TPI>(x, mask);
+    if (item < count && Grp<TPI>::rank() == 0) popcnt[item] = c;
+  }
+}
+
+// c = (1 + m*n) * r^n mod n^2   (cu:522-546 form of scheme.py:63-83)
+template <int L, int TPI>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_encrypt(const PaillierDev* __restrict__ key,
+                                                           const u32* __restrict__ Mv, const u32* __restrict__ Rv,
+                                                           u32* __restrict__ out, int count, int stride,
+                                                           const u32* __restrict__ sched, int nsched, int tentries,
+                                                           u32* __restrict__ table, int tstride) {
+  extern __shared__ uint4 smem[];
+  constexpr int K = L / TPI;
+  Mont<L, TPI> M;
+  mont_init<L, TPI>(M, &key->n2, smem);
+  const int item = group_item<TPI>();
+  const int it = item < count ? item : count - 1;
+  u32 y[K];
+  load_dist<L, TPI>(y, Rv, it, stride);
+  to_mont<L, TPI>(M, y, &key->n2);
+  modexp_sched<L, TPI>(M, y, sched, nsched, tentries, table, item, tstride);  // y = r^n * R
+  u32 a[K], nr[K];
+  load_dist<L, TPI>(a, Mv, it, stride);
+  load_const<L, TPI>(nr, key->nR.v);
+  M.mul_into(a, nr);                 // m*n (< n^2, exact)
+  (void)add_small<K, TPI>(a, 1u);    // 1 + m*n
+  M.mul_into(a, y);                  // (1 + m n) * r^n
+  if (item < count) store_dist<L, TPI>(out, a, item, stride);
+}
+
+// CRT decryption, one half per launch row (blockIdx.y = 0: p, 1: q).
+// Input: ciphertexts of 2H limbs; output m_p (or m_q) of H limbs.
+//   x  = c mod p^2 in Montgomery form = MontMul(c_lo, R^2) + MontMul(c_hi, R^3)
+//   u  = x^(p-1) mod p^2;  m_p = L_p(u) * h_p mod p
+template <int H, int TPI>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_crt_half(const PaillierDev* __restrict__ key,
+                                                            const u32* __restrict__ C, u32* __restrict__ out_half,
+                                                            int count, int stride, const u32* __restrict__ sched_p,
+                                                            int nsched_p, const u32* __restrict__ sched_q,
+                                                            int nsched_q, int tentries, u32* __restrict__ table,
+                                                            int tstride) {
+  extern __shared__ uint4 smem[];
+  constexpr int K = H / TPI;
+  const int half = (int)blockIdx.y;
+  const ModDev* sq = half == 0 ? &key->p2 : &key->q2;
+  const ModDev* pr = half == 0 ? &key->p : &key->q;
+  const u32* hR = half == 0 ? key->hpR.v : key->hqR.v;
+  const u32* sched = half == 0 ? sched_p : sched_q;
+  const int nsched = half == 0 ? nsched_p : nsched_q;
+  const int item = group_item<TPI>();
+  const int it = item < count ? item : count - 1;
+  u32 x[K];
+  {
+    Mont<H, TPI> M;
+    mont_init<H, TPI>(M, sq, smem);
+    u32 lo[K], hi[K], r[K];
+    load_dist<H, TPI>(lo, C, it, stride);
+    load_dist<H, TPI>(hi, C + (size_t)H * stride, it, stride);
+    load_const<H, TPI>(r, sq->R2);
+    M.mul(x, lo, r);  // c_lo * R
+    load_const<H, TPI>(r, sq->R3);
+    M.mul_into(hi, r);  // c_hi * R^2
+    add_mod<K, TPI>(x, hi, M.N);
+    u32* tab = table + (size_t)half * tentries * H * tstride;
+    modexp_sched<H, TPI>(M, x, sched, nsched, tentries, tab, item, tstride);
+    from_mont<H, TPI>(M, x);
+  }
+  Mont<H, TPI> Mp;
+  mont_init<H, TPI>(Mp, pr, smem);
+  paillier_finish<H, TPI>(Mp, pr, x, hR);
+  if (item < count) store_dist<H, TPI>(out_half + (size_t)half * H * stride, x, item, stride);
+}
+
+// Garner recombination: m = m_q + q * ((m_p - m_q) * q^-1 mod p)
+template <int H, int TPI>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_crt_combine(const PaillierDev* __restrict__ key,
+                                                               const u32* __restrict__ halves, u32* __restrict__ out,
+                                                               int count, int stride) {
+  extern __shared__ uint4 smem[];
+  constexpr int K = H / TPI;
+  const int item = group_item<TPI>();
+  const int it = item < count ? item : count - 1;
+  u32 mp[K], mq[K], qi[K];
+  load_dist<H, TPI>(mp, halves, it, stride);
+  load_dist<H, TPI>(mq, halves + (size_t)H * stride, it, stride);
+  load_const<H, TPI>(qi, key->qinvR.v);
+  Mont<H, TPI> Mp;
+  mont_init<H, TPI>(Mp, &key->p, smem);
+  u32 h1[K], h2[K];
+  Mp.mul(h1, mp, qi);  // m_p * q^-1 mod p
+  Mp.mul(h2, mq, qi);  // m_q * q^-1 mod p   (m_q < R, so the product is reduced)
+  sub_mod<K, TPI>(h1, h2, Mp.N);
+  Mont<H, TPI> Mn;
+  mont_init<H, TPI>(Mn, &key->nh, smem);
+  u32 qr[K], m[K];
+  load_const<H, TPI>(qr, key->qRn.v);
+  Mn.mul(m, h1, qr);  // q*h  (< n, exact)
+  (void)add_dist<K, TPI>(m, mq);
+  if (item < count) store_dist<H, TPI>(out, m, item, stride);
+}
+
+// Lookup-table encryption (lookup.py:213-252, lcu:839-948):
+//   c = prod_i g_table[i][d_i] * prod_k noise[idx_k] mod n^2
+// Tables are in Montgomery form (entry*R mod n^2), AoS: entry e at [e*L].
+// digits: [count][chunks] bytes, noise_idx: [count][14] bytes.
+// Rows where every item of the wave has digit 0 are skipped (g_table[i][0] = 1);
+// otherwise items with digit 0 multiply by the j = 0 entry (R mod n^2), so the
+// wave stays convergent and the product is unchanged.
+template <int L, int TPI>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_lookup_encrypt(const ModDev* __restrict__ md,
+                                                                  const u32* __restrict__ gtab,
+                                                                  const u32* __restrict__ ntab,
+                                                                  const unsigned char* __restrict__ digits,
+                                                                  int chunks, const unsigned char* __restrict__ nidx,
+                                                                  int nmult, int nnoise, u32* __restrict__ out,
+                                                                  int count, int stride) {
+  extern __shared__ uint4 smem[];
+  constexpr int K = L / TPI;
+  Mont<L, TPI> M;
+  mont_init<L, TPI>(M, md, smem);
+  const int item = group_item<TPI>();
+  const int it = item < count ? item : count - 1;
+  const int r = Grp<TPI>::rank();
+  u32 acc[K];
+  load_const<L, TPI>(acc, md->R1);
+  const unsigned char* dg = digits + (size_t)it * chunks;
+  for (int i = 0; i < chunks; ++i) {
+    const int d = dg[i];
+    if (__ballot(d != 0) == 0ull) continue;  // uniform skip
+    u32 y[K];
+    const u32* e = gtab + ((size_t)i * 256 + d) * L + r * K;
+#pragma unroll
+    for (int k = 0; k < K; ++k) y[k] = e[k];
+    M.mul_into(acc, y);
+  }
+  const unsigned char* ni = nidx + (size_t)it * nmult;
+  for (int j = 0; j < nmult; ++j) {
+    u32 y[K];
+    const u32* e = ntab + (size_t)(ni[j] % nnoise) * L + r * K;  // index mod table size, as lcu:933-934
+#pragma unroll
+    for (int k = 0; k < K; ++k) y[k] = e[k];
+    M.mul_into(acc, y);
+  }
+  from_mont<L, TPI>(M, acc);
+  if (item < count) store_dist<L, TPI>(out, acc, item, stride);
+}
+
+// ---------------------------------------------------------------------------
+// Layout conversion: AoS [count][L] <-> interleaved [L][stride]
+// ---------------------------------------------------------------------------
+__global__ void k_aos_to_il(const u32* __restrict__ aos, u32* __restrict__ il, int count, int L, int stride) {
+  __shared__ u32 tile[32][33];
+  const int i0 = blockIdx.x * 32, l0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 256 threads: ty 0..7
+  for (int y = ty; y < 32; y += 8) {
+    const int i = i0 + y, l = l0 + tx;
+    tile[y][tx] = (i < count && l < L) ? aos[(size_t)i * L + l] : 0u;
+  }
+  __syncthreads();
+  for (int y = ty; y < 32; y += 8) {
+    const int l = l0 + y, i = i0 + tx;
+    if (l < L && i < stride) il[(size_t)l * stride + i] = tile[tx][y];
+  }
+}
+
+__global__ void k_il_to_aos(const u32* __restrict__ il, u32* __restrict__ aos, int count, int L, int stride) {
+  __shared__ u32 tile[32][33];
+  const int i0 = blockIdx.x * 32, l0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int y = ty; y < 32; y += 8) {
+    const int l = l0 + y, i = i0 + tx;
+    tile[y][tx] = (l < L && i < count) ? il[(size_t)l * stride + i] : 0u;
+  }
+  __syncthreads();
+  for (int y = ty; y < 32; y += 8) {
+    const int i = i0 + y, l = l0 + tx;
+    if (i < count && l < L) aos[(size_t)i * L + l] = tile[tx][y];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Instruction-rate probe: `chains` independent v_mad_u64_u32 accumulators per
+// lane, `iters` rounds; the host times it to pin the integer-multiply peak.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_probe_mad(u64* __restrict__ sink, int iters, u32 seed) {
+  u64 acc[8];
+  u32 a = seed ^ threadIdx.x, b = seed * 2654435761u + blockIdx.x;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) acc[c] = (u64)(a + c) << 7;
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[c] = (u64)a * (u32)(b + c) + acc[c];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[c] = (u64)b * (u32)(a + c) + acc[c];
+  }
+  u64 s = 0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) s ^= acc[c];
+  if (s == 0x1234567ull) sink[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+}  // namespace chx

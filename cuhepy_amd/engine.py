@@ -1,0 +1,347 @@
+"""Typed Python handles over the C ABI: device context, moduli and keys.
+
+Everything here calls ``libchx.so``; there is no CPU path. Host-buffer calls
+take and return numpy ``uint32`` limb batches (see ``limbs.py``); the
+``*_dev`` methods take torch tensors that already live on the GPU in the
+limb-interleaved layout (limb j of item i at ``[j, i]`` of a
+``(limbs, stride)`` tensor) and enqueue on torch's current stream.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _native as N
+from .limbs import ints_to_limbs, limb_count, width_for_bits
+
+_u32p = N.u32p
+
+
+def _p32(a: np.ndarray):
+    assert a.dtype == np.uint32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_u32p)
+
+
+def _p8(a: np.ndarray):
+    assert a.dtype == np.uint8 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(N.u8p)
+
+
+def _batch(x, limbs: int) -> np.ndarray:
+    """Accept a (count, limbs) uint32 array or a sequence of ints."""
+    if isinstance(x, np.ndarray):
+        a = np.ascontiguousarray(x, dtype=np.uint32)
+        if a.ndim != 2 or a.shape[1] != limbs:
+            raise ValueError(f"expected a (count, {limbs}) uint32 batch, got shape {a.shape}")
+        return a
+    return ints_to_limbs(x, limbs)
+
+
+def _dptr(t) -> ctypes.c_void_p:
+    return ctypes.c_void_p(int(t.data_ptr()))
+
+
+def stride_for(count: int) -> int:
+    return max(64, (count + 63) // 64 * 64)
+
+
+class Context:
+    """One device, one stream, reusable device scratch (chx_ctx)."""
+
+    _default: dict[int, "Context"] = {}
+    _lock = threading.Lock()
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        N.check(N.lib.chx_ctx_create(device, ctypes.byref(h)))
+        self.handle = h
+        self.device = device
+
+    @classmethod
+    def default(cls, device: int = 0) -> "Context":
+        with cls._lock:
+            ctx = cls._default.get(device)
+            if ctx is None:
+                ctx = cls._default[device] = Context(device)
+            return ctx
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            N.lib.chx_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream_ptr: Optional[int]) -> None:
+        N.check(N.lib.chx_ctx_set_stream(self.handle, ctypes.c_void_p(stream_ptr or 0)))
+
+    def use_torch_stream(self) -> None:
+        import torch
+        self.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def sync(self) -> None:
+        N.check(N.lib.chx_ctx_sync(self.handle))
+
+    def set_tuning(self, op: int, limbs: int, tpi: int = 0, window: int = 0) -> None:
+        N.check(N.lib.chx_ctx_set_tuning(self.handle, op, limbs, tpi, window))
+
+    @staticmethod
+    def supported_tpi(limbs: int) -> list[int]:
+        buf = (ctypes.c_int * 16)()
+        n = N.lib.chx_supported_tpi(limbs, buf, 16)
+        return list(buf[:n])
+
+    def scratch_bytes(self) -> int:
+        return int(N.lib.chx_ctx_scratch_bytes(self.handle))
+
+    def set_timing(self, enable: bool) -> None:
+        N.check(N.lib.chx_ctx_set_timing(self.handle, int(enable)))
+
+    def kernel_time(self, name: str) -> tuple[float, int]:
+        ms, n = ctypes.c_double(), ctypes.c_int()
+        N.check(N.lib.chx_ctx_kernel_time(self.handle, name.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def reset_timing(self) -> None:
+        N.check(N.lib.chx_ctx_reset_timing(self.handle))
+
+    def probe_mad(self, blocks: int, iters: int) -> tuple[float, float]:
+        s, m = ctypes.c_double(), ctypes.c_double()
+        N.check(N.lib.chx_probe_mad(self.handle, blocks, iters, ctypes.byref(s), ctypes.byref(m)))
+        return s.value, m.value
+
+    # layout conversion on device
+    def aos_to_il(self, aos, il, count: int, limbs: int, stride: int) -> None:
+        N.check(N.lib.chx_aos_to_il_dev(self.handle, _dptr(aos), _dptr(il), count, limbs, stride))
+
+    def il_to_aos(self, il, aos, count: int, limbs: int, stride: int) -> None:
+        N.check(N.lib.chx_il_to_aos_dev(self.handle, _dptr(il), _dptr(aos), count, limbs, stride))
+
+
+class Modulus:
+    """An odd modulus with its Montgomery constants resident on the device."""
+
+    def __init__(self, ctx: Context, n: int, limbs: Optional[int] = None):
+        if n <= 1 or n % 2 == 0:
+            raise ValueError("modulus must be odd and > 1")
+        self.ctx = ctx
+        self.n = n
+        self.limbs = limbs or width_for_bits(n.bit_length())
+        arr = ints_to_limbs([n], self.limbs)
+        h = ctypes.c_void_p()
+        N.check(N.lib.chx_mod_create(ctx.handle, _p32(arr), self.limbs, ctypes.byref(h)))
+        self.handle = h
+
+    def __del__(self):  # pragma: no cover
+        try:
+            if getattr(self, "handle", None):
+                N.lib.chx_mod_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+    def mulmod(self, a, b) -> np.ndarray:
+        A, B = _batch(a, self.limbs), _batch(b, self.limbs)
+        if A.shape != B.shape:
+            raise ValueError("a and b must have the same length")
+        out = np.empty_like(A)
+        if len(A):
+            N.check(N.lib.chx_mulmod(self.ctx.handle, self.handle, _p32(A), _p32(B), _p32(out), len(A)))
+        return out
+
+    def modexp(self, bases, e: int) -> np.ndarray:
+        if e < 0:
+            raise ValueError("negative exponent")
+        B = _batch(bases, self.limbs)
+        E = ints_to_limbs([e], limb_count(e))
+        out = np.empty_like(B)
+        if len(B):
+            N.check(N.lib.chx_modexp(self.ctx.handle, self.handle, _p32(B), len(B), _p32(E), E.shape[1],
+                                     _p32(out)))
+        return out
+
+    # device-resident forms
+    def mulmod_dev(self, a, b, out, count: int, stride: int) -> None:
+        N.check(N.lib.chx_mulmod_dev(self.ctx.handle, self.handle, _dptr(a), _dptr(b), _dptr(out), count, stride))
+
+    def mulmod_bcast_dev(self, x, y: int, out, count: int, stride: int) -> None:
+        Y = ints_to_limbs([y], self.limbs)
+        N.check(N.lib.chx_mulmod_bcast_dev(self.ctx.handle, self.handle, _dptr(x), _p32(Y), _dptr(out), count,
+                                           stride))
+
+    def modexp_dev(self, base, out, count: int, stride: int, e: int) -> None:
+        E = ints_to_limbs([e], limb_count(e))
+        N.check(N.lib.chx_modexp_dev(self.ctx.handle, self.handle, _dptr(base), _dptr(out), count, stride,
+                                     _p32(E), E.shape[1]))
+
+
+def crt_constants(p: int, q: int) -> dict:
+    """h_p = L_p(g^(p-1) mod p^2)^-1 mod p for g = n + 1 (and h_q), q^-1 mod p.
+
+    For g = n + 1, g^(p-1) = 1 + (p-1) n (mod p^2), so L_p(g^(p-1)) =
+    (p-1) q mod p = -q mod p and h_p = (-q)^-1 mod p; likewise h_q = (-p)^-1 mod q.
+    """
+    return {"hp": pow(-q % p, -1, p), "hq": pow(-p % q, -1, q), "qinv": pow(q, -1, p)}
+
+
+class PaillierKey:
+    """Device-resident Paillier key (g = n + 1): encrypt, decrypt, CRT decrypt."""
+
+    def __init__(self, ctx: Context, n: int, phi: int, inv: int, p: Optional[int] = None,
+                 q: Optional[int] = None):
+        self.ctx = ctx
+        self.n, self.phi, self.inv = n, phi, inv
+        self.n2 = n * n
+        nl = limb_count(n)
+        self.width = width_for_bits(self.n2.bit_length())
+        self.half = self.width // 2
+        nn = ints_to_limbs([n], nl)
+        ph = ints_to_limbs([phi], nl)
+        iv = ints_to_limbs([inv], nl)
+        h = ctypes.c_void_p()
+        self.crt = p is not None and q is not None
+        if self.crt:
+            if p * q != n:
+                raise ValueError("p * q != n")
+            if p < q:  # Garner recombination is symmetric; keep the given order
+                pass
+            c = crt_constants(p, q)
+            hl = max(limb_count(p), limb_count(q), limb_count(c["hp"]), limb_count(c["hq"]))
+            arrs = [ints_to_limbs([v], hl) for v in (p, q, c["hp"], c["hq"], c["qinv"])]
+            N.check(N.lib.chx_paillier_key_create(ctx.handle, _p32(nn), nl, _p32(ph), _p32(iv), _p32(arrs[0]),
+                                                  _p32(arrs[1]), hl, _p32(arrs[2]), _p32(arrs[3]), _p32(arrs[4]),
+                                                  ctypes.byref(h)))
+        else:
+            N.check(N.lib.chx_paillier_key_create(ctx.handle, _p32(nn), nl, _p32(ph), _p32(iv), None, None, 0,
+                                                  None, None, None, ctypes.byref(h)))
+        self.handle = h
+        assert N.lib.chx_paillier_width(h) == self.width
+
+    def __del__(self):  # pragma: no cover
+        try:
+            if getattr(self, "handle", None):
+                N.lib.chx_paillier_key_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+    def encrypt(self, m, r) -> np.ndarray:
+        M, R = _batch(m, self.width), _batch(r, self.width)
+        if M.shape != R.shape:
+            raise ValueError("m and r must have the same length")
+        out = np.empty_like(M)
+        if len(M):
+            N.check(N.lib.chx_paillier_encrypt(self.ctx.handle, self.handle, _p32(M), _p32(R), _p32(out), len(M)))
+        return out
+
+    def decrypt(self, c) -> np.ndarray:
+        C = _batch(c, self.width)
+        out = np.empty_like(C)
+        if len(C):
+            N.check(N.lib.chx_paillier_decrypt(self.ctx.handle, self.handle, _p32(C), _p32(out), len(C)))
+        return out
+
+    def decrypt_crt(self, c) -> np.ndarray:
+        if not self.crt:
+            raise ValueError("key has no CRT factors")
+        C = _batch(c, self.width)
+        out = np.empty((len(C), self.half), dtype=np.uint32)
+        if len(C):
+            N.check(N.lib.chx_paillier_decrypt_crt(self.ctx.handle, self.handle, _p32(C), _p32(out), len(C)))
+        return out
+
+    def decode_hamming(self, c, mask: int) -> np.ndarray:
+        C = _batch(c, self.width)
+        Mk = ints_to_limbs([mask], self.width)
+        out = np.empty(len(C), dtype=np.uint32)
+        if len(C):
+            N.check(N.lib.chx_paillier_decode_hamming(self.ctx.handle, self.handle, _p32(C), _p32(Mk), _p32(out),
+                                                      len(C)))
+        return out
+
+    # device-resident forms (interleaved torch tensors)
+    def encrypt_dev(self, m, r, out, count: int, stride: int) -> None:
+        N.check(N.lib.chx_paillier_encrypt_dev(self.ctx.handle, self.handle, _dptr(m), _dptr(r), _dptr(out), count,
+                                               stride))
+
+    def decrypt_dev(self, c, out, count: int, stride: int) -> None:
+        N.check(N.lib.chx_paillier_decrypt_dev(self.ctx.handle, self.handle, _dptr(c), _dptr(out), count, stride))
+
+    def decrypt_crt_dev(self, c, out, count: int, stride: int) -> None:
+        N.check(N.lib.chx_paillier_decrypt_crt_dev(self.ctx.handle, self.handle, _dptr(c), _dptr(out), count,
+                                                   stride))
+
+
+class LookupKey:
+    """Device-resident Paillier-lookup key: tables in HBM (Montgomery form)."""
+
+    def __init__(self, ctx: Context, n: int, a: int, g_a_inv: int, g_table: Sequence[Sequence[int]],
+                 noise_table: Sequence[int]):
+        self.ctx = ctx
+        self.n, self.a, self.g_a_inv = n, a, g_a_inv
+        self.n2 = n * n
+        nl = limb_count(n)
+        self.width = width_for_bits(self.n2.bit_length())
+        self.chunks = len(g_table)
+        flat = [v for row in g_table for v in row]
+        if any(len(row) != 256 for row in g_table):
+            raise ValueError("every g_table row must have 256 entries")
+        G = ints_to_limbs(flat, self.width)
+        NT = ints_to_limbs(list(noise_table), self.width)
+        nn = ints_to_limbs([n], nl)
+        A = ints_to_limbs([a], limb_count(a))
+        GA = ints_to_limbs([g_a_inv], nl)
+        h = ctypes.c_void_p()
+        N.check(N.lib.chx_lookup_key_create(ctx.handle, _p32(nn), nl, _p32(A), A.shape[1], _p32(GA), _p32(G),
+                                            self.chunks, _p32(NT), len(NT), ctypes.byref(h)))
+        self.handle = h
+        self.noise_entries = len(NT)
+
+    def __del__(self):  # pragma: no cover
+        try:
+            if getattr(self, "handle", None):
+                N.lib.chx_lookup_key_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+    def encrypt_digits(self, digits: np.ndarray, noise_idx: np.ndarray) -> np.ndarray:
+        D = np.ascontiguousarray(digits, dtype=np.uint8)
+        NI = np.ascontiguousarray(noise_idx, dtype=np.uint8)
+        if D.ndim != 2 or D.shape[1] != self.chunks:
+            raise ValueError(f"digits must be (count, {self.chunks}) bytes")
+        if NI.ndim != 2 or NI.shape[0] != D.shape[0]:
+            raise ValueError("noise_idx must be (count, nmult) bytes")
+        out = np.empty((len(D), self.width), dtype=np.uint32)
+        if len(D):
+            N.check(N.lib.chx_lookup_encrypt(self.ctx.handle, self.handle, _p8(D), _p8(NI), NI.shape[1], _p32(out),
+                                             len(D)))
+        return out
+
+    def encrypt_digits_dev(self, digits, noise_idx, nmult: int, out, count: int, stride: int) -> None:
+        N.check(N.lib.chx_lookup_encrypt_dev(self.ctx.handle, self.handle, _dptr(digits), _dptr(noise_idx), nmult,
+                                             _dptr(out), count, stride))
+
+    def decrypt(self, c) -> np.ndarray:
+        C = _batch(c, self.width)
+        out = np.empty_like(C)
+        if len(C):
+            N.check(N.lib.chx_lookup_decrypt(self.ctx.handle, self.handle, _p32(C), _p32(out), len(C)))
+        return out
+
+    def decode_hamming(self, c, mask: int) -> np.ndarray:
+        C = _batch(c, self.width)
+        Mk = ints_to_limbs([mask], self.width)
+        out = np.empty(len(C), dtype=np.uint32)
+        if len(C):
+            N.check(N.lib.chx_lookup_decode_hamming(self.ctx.handle, self.handle, _p32(C), _p32(Mk), _p32(out),
+                                                    len(C)))
+        return out

@@ -1,0 +1,190 @@
+/*
+ * chx.h -- C ABI of the MI355X (gfx950) Paillier batched-modexp library.
+ *
+ * This is the drop-in boundary for the reference's two pybind11 extensions,
+ *   cuhepy.paillier._gpu_ext.paillier_GPU_client        (class PaillierGPUClient,
+ *       src/cuhepy/paillier/_gpu_ext/paillier_GPU_client.cu:1136-1165)
+ *   cuhepy.paillier._lookup_gpu_ext.paillier_GPU_lookup_client
+ *       (class PaillierGPULookupClient, .../paillier_GPU_lookup_client.cu:2673-2721)
+ * The Python classes of the same names in cuhepy_amd.paillier are rebuilt on
+ * top of these functions through ctypes; each function below names the
+ * reference method or kernel whose work it performs.
+ *
+ * Conventions
+ *  - Bignums are little-endian arrays of 32-bit limbs. Host-buffer entry points
+ *    take "AoS" batches: item i occupies limbs [i*limbs, (i+1)*limbs), the
+ *    layout of the reference's cgbn_mem_t arrays (cu:310-313).
+ *  - *_dev entry points take device pointers in the limb-interleaved layout:
+ *    limb j of item i at [j*stride + i], stride >= count rounded up to 64.
+ *    They enqueue on the context's stream and return without synchronising.
+ *  - Every function returns CHX_OK (0) or an error code; chx_last_error()
+ *    returns the message of the calling thread's last failure. Inputs are
+ *    validated (values >= modulus are rejected, not reduced or truncated).
+ *  - Thread safety: calls on one context are serialised by a per-context lock;
+ *    different contexts run concurrently. ctypes releases the GIL around every
+ *    call.
+ */
+#ifndef CHX_H
+#define CHX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct chx_ctx chx_ctx;   /* device context: stream, scratch, tuning */
+typedef struct chx_mod chx_mod;   /* one odd modulus + Montgomery constants */
+typedef struct chx_pkey chx_pkey; /* Paillier key (n, n^2, decrypt and CRT constants) */
+typedef struct chx_lkey chx_lkey; /* Paillier-lookup key (tables in HBM) */
+
+enum {
+  CHX_OK = 0,
+  CHX_EINVAL = 1,      /* bad argument or value out of range */
+  CHX_EDEVICE = 2,     /* HIP runtime error */
+  CHX_ENOMEM = 3,      /* device allocation failed */
+  CHX_EUNSUPPORTED = 4 /* width / lane-group combination not built */
+};
+
+/* operation ids for chx_ctx_set_tuning / chx_ctx_get_tuning */
+enum {
+  CHX_OP_MODEXP = 0, /* fixed-exponent modexp, encrypt, decrypt, CRT */
+  CHX_OP_MULMOD = 1, /* pointwise / broadcast mulmod */
+  CHX_OP_LOOKUP = 2, /* lookup-table encryption */
+  CHX_OP_RAGGED = 3  /* per-item exponent modexp */
+};
+
+const char* chx_version(void);
+/* copy the calling thread's last error message; returns its full length */
+int chx_last_error(char* buf, size_t size);
+
+/* ---- context (no reference counterpart: the reference uses the default stream
+ *      and cudaMalloc/cudaFree on every call, cu:917-961) ---- */
+int chx_ctx_create(int device, chx_ctx** out);
+int chx_ctx_destroy(chx_ctx* ctx);
+/* use an external hipStream_t (e.g. torch's); NULL restores the owned stream */
+int chx_ctx_set_stream(chx_ctx* ctx, void* stream);
+void* chx_ctx_get_stream(chx_ctx* ctx);
+int chx_ctx_sync(chx_ctx* ctx);
+/* lanes per bignum (tpi) and window width for `op` at `limbs`; 0 = automatic */
+int chx_ctx_set_tuning(chx_ctx* ctx, int op, int limbs, int tpi, int window);
+/* lanes-per-bignum values built for `limbs` (returns count, fills up to cap) */
+int chx_supported_tpi(int limbs, int* tpis, int cap);
+/* bytes of device scratch currently held by the context */
+size_t chx_ctx_scratch_bytes(chx_ctx* ctx);
+
+/* ---- moduli ---- */
+/* limbs in {32, 64, 128}; n odd, n < 2^(32*limbs) */
+int chx_mod_create(chx_ctx* ctx, const uint32_t* n, int limbs, chx_mod** out);
+int chx_mod_destroy(chx_mod* m);
+int chx_mod_limbs(const chx_mod* m);
+
+/* out[i] = a[i]*b[i] mod n for any a, b below 2^(32*limbs) -- encode_hamming_server /
+ * gpu_pointwise_mulmod_hex
+ * (cu:672-732, modmul_pointwise_kernel cu:430-454; lcu:1505-1561) */
+int chx_mulmod(chx_ctx* ctx, const chx_mod* m, const uint32_t* a, const uint32_t* b, uint32_t* out,
+               size_t count);
+int chx_mulmod_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* a, const uint32_t* b, uint32_t* out,
+                   size_t count, size_t stride);
+/* out[i] = x[i]*y mod n with y shared (the served-kNN evaluator form) */
+int chx_mulmod_bcast_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* x, const uint32_t* y_host,
+                         uint32_t* out, size_t count, size_t stride);
+
+/* out[i] = base[i]^e mod n, one exponent for the batch -- pow_mod_mem (cu:332-360),
+ * pow_mod_small_exp (lcu:728-775); gmpy2.powmod in scheme.py:81-83, 101-104 */
+int chx_modexp(chx_ctx* ctx, const chx_mod* m, const uint32_t* base, size_t count, const uint32_t* e,
+               int e_limbs, uint32_t* out);
+int chx_modexp_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* base, uint32_t* out, size_t count,
+                   size_t stride, const uint32_t* e, int e_limbs);
+
+/* ---- Paillier (g = n + 1) ----
+ * n_limbs: limbs of n (<= 64); the working width is the smallest of 32/64/128
+ * limbs that holds n^2. inv = phi^-1 mod n. When p != NULL, CRT decryption is
+ * enabled: p, q (half_limbs each, p*q = n), hp = L_p(g^(p-1) mod p^2)^-1 mod p,
+ * hq likewise, qinv = q^-1 mod p. Replaces PaillierCPU::key_gen output handling
+ * (cu:183-216) and load_stringified_keys (cu:826-851). */
+int chx_paillier_key_create(chx_ctx* ctx, const uint32_t* n, int n_limbs, const uint32_t* phi,
+                            const uint32_t* inv, const uint32_t* p, const uint32_t* q, int half_limbs,
+                            const uint32_t* hp, const uint32_t* hq, const uint32_t* qinv, chx_pkey** out);
+int chx_paillier_key_destroy(chx_pkey* k);
+int chx_paillier_width(const chx_pkey* k); /* limbs of n^2 */
+
+/* c[i] = (1 + m[i]*n) * r[i]^n mod n^2; m < n, 0 < r < n, gcd(r, n) = 1 is the
+ * caller's draw -- encrypt_kernel (cu:458-546) / Paillier.encrypt (scheme.py:63-83).
+ * m, r, c: `width` limbs per item. */
+int chx_paillier_encrypt(chx_ctx* ctx, const chx_pkey* k, const uint32_t* m, const uint32_t* r,
+                         uint32_t* c, size_t count);
+int chx_paillier_encrypt_dev(chx_ctx* ctx, const chx_pkey* k, const uint32_t* m, const uint32_t* r,
+                             uint32_t* c, size_t count, size_t stride);
+/* m[i] = L(c[i]^phi mod n^2) * inv mod n -- decrypt_only_kernel (cu:549-587) /
+ * Paillier.decrypt (scheme.py:86-104). c < n^2 required. */
+int chx_paillier_decrypt(chx_ctx* ctx, const chx_pkey* k, const uint32_t* c, uint32_t* m, size_t count);
+int chx_paillier_decrypt_dev(chx_ctx* ctx, const chx_pkey* k, const uint32_t* c, uint32_t* m, size_t count,
+                             size_t stride);
+/* CRT decryption (no reference counterpart; same result as chx_paillier_decrypt
+ * for c coprime to n). m has width/2 limbs per item. */
+int chx_paillier_decrypt_crt(chx_ctx* ctx, const chx_pkey* k, const uint32_t* c, uint32_t* m, size_t count);
+int chx_paillier_decrypt_crt_dev(chx_ctx* ctx, const chx_pkey* k, const uint32_t* c, uint32_t* m,
+                                 size_t count, size_t stride);
+/* fused decrypt + odd-bit popcount (decode_hamming_kernel cu:592-668, never
+ * launched there; decode_hamming_client cu:995-1038): counts[i] =
+ * popcount(m[i] & mask), mask = `width` limbs. */
+int chx_paillier_decode_hamming(chx_ctx* ctx, const chx_pkey* k, const uint32_t* c, const uint32_t* mask,
+                                uint32_t* counts, size_t count);
+
+/* ---- Paillier-lookup ----
+ * n_limbs: limbs of n; width = limbs of n^2 as above. a: decrypt exponent
+ * (a_limbs), g_a_inv < n. Tables are plain residues mod n^2, `width` limbs per
+ * entry: g_table[chunks][256] (lookup.py:104-110, lcu:550-588) and
+ * noise_table[256] (lookup.py:113-117); they are converted to Montgomery form
+ * and kept resident in HBM (ensure_device_tables_, lcu:2625-2652). */
+int chx_lookup_key_create(chx_ctx* ctx, const uint32_t* n, int n_limbs, const uint32_t* a, int a_limbs,
+                          const uint32_t* g_a_inv, const uint32_t* g_table, int chunks,
+                          const uint32_t* noise_table, int noise_entries, chx_lkey** out);
+int chx_lookup_key_destroy(chx_lkey* k);
+int chx_lookup_width(const chx_lkey* k);
+/* c[i] = prod_j g_table[j][digits[i][j]] * prod_t noise[noise_idx[i][t]] mod n^2
+ * -- encrypt_kernel (lcu:839-948), encrypt_cts_kernel (lcu:951-1011).
+ * digits: count x chunks bytes; noise_idx: count x nmult bytes. */
+int chx_lookup_encrypt(chx_ctx* ctx, const chx_lkey* k, const uint8_t* digits, const uint8_t* noise_idx,
+                       int nmult, uint32_t* c, size_t count);
+int chx_lookup_encrypt_dev(chx_ctx* ctx, const chx_lkey* k, const uint8_t* digits, const uint8_t* noise_idx,
+                           int nmult, uint32_t* c, size_t count, size_t stride);
+/* m[i] = L(c[i]^a mod n^2) * g_a_inv mod n -- decrypt_only_kernel (lcu:1014-1050),
+ * decrypt_cts (lcu:2440-2499) */
+int chx_lookup_decrypt(chx_ctx* ctx, const chx_lkey* k, const uint32_t* c, uint32_t* m, size_t count);
+/* fused decrypt + odd-bit popcount -- decode_hamming_fused_kernel (lcu:1140-1192)
+ * replacing K8 + odd_popcount_kernel (lcu:1279-1298) */
+int chx_lookup_decode_hamming(chx_ctx* ctx, const chx_lkey* k, const uint32_t* c, const uint32_t* mask,
+                              uint32_t* counts, size_t count);
+
+/* ---- layout conversion (device) ---- */
+int chx_aos_to_il_dev(chx_ctx* ctx, const uint32_t* aos, uint32_t* il, size_t count, int limbs, size_t stride);
+int chx_il_to_aos_dev(chx_ctx* ctx, const uint32_t* il, uint32_t* aos, size_t count, int limbs, size_t stride);
+
+/* ---- host-side helpers (no device needed; exposed so the host logic can be
+ *      checked on machines without a GPU) ---- */
+/* sliding-window schedule of exponent e (packed as the kernels read it);
+ * returns the number of entries (may exceed cap; only cap are written) */
+int chx_host_schedule(const uint32_t* e, int e_limbs, int window, uint32_t* out, int cap);
+/* Montgomery constants of odd n (limbs): n0inv = -n^-1 mod 2^32, r2 = R^2 mod n */
+int chx_host_mont_consts(const uint32_t* n, int limbs, uint32_t* n0inv, uint32_t* r2);
+
+/* ---- measurement ---- */
+/* time `iters` rounds of 16 dependent-free v_mad_u64_u32 per lane on `blocks`
+ * blocks of 256 lanes; returns seconds and the number of 32x32->64 MACs */
+int chx_probe_mad(chx_ctx* ctx, int blocks, int iters, double* seconds, double* macs);
+/* HIP-event timing of the compute kernels enqueued by this context (events
+ * recorded on the context's stream around every launch while enabled) */
+int chx_ctx_set_timing(chx_ctx* ctx, int enable);
+/* total milliseconds and launch count of kernel `name` (e.g. "crt_half",
+ * "modexp", "mulmod_bcast") since the last reset; synchronises the stream */
+int chx_ctx_kernel_time(chx_ctx* ctx, const char* name, double* total_ms, int* launches);
+int chx_ctx_reset_timing(chx_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CHX_H */

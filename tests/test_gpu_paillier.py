@@ -1,0 +1,145 @@
+"""GPU parity of Paillier encrypt / decrypt / CRT decrypt / decode and the
+lookup scheme against the oracle, on the golden vectors and seeded batches."""
+
+import random
+
+import numpy as np
+import pytest
+
+from conftest import H
+from oracle import paillier_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+OP_MODEXP, OP_MULMOD, OP_LOOKUP = 0, 1, 2
+
+
+@pytest.fixture(autouse=True)
+def _reset_tuning(ctx):
+    yield
+    for limbs in (32, 64, 128):
+        for op in (0, 1, 2, 3):
+            ctx.set_tuning(op, limbs, 0, 0)
+
+
+def _key_from_golden(ctx, g):
+    from cuhepy_amd.engine import PaillierKey
+    return PaillierKey(ctx, H(g["n"]), H(g["phi"]), H(g["inv"]), H(g["p"]), H(g["q"]))
+
+
+@pytest.mark.parametrize("case", ["paillier_w64", "paillier_w128"])
+def test_encrypt_decrypt_golden(ctx, golden, case):
+    from cuhepy_amd.limbs import limbs_to_ints
+    g = golden[case]
+    key = _key_from_golden(ctx, g)
+    ms, rs, cs = [H(v) for v in g["m"]], [H(v) for v in g["r"]], [H(v) for v in g["c"]]
+    assert limbs_to_ints(key.encrypt(ms, rs)) == cs
+    assert limbs_to_ints(key.decrypt(cs)) == ms
+    assert limbs_to_ints(key.decrypt_crt(cs)) == ms
+    edge = [H(v) for v in g["edge_c"]]
+    assert limbs_to_ints(key.decrypt(edge)) == [H(v) for v in g["edge_dec"]]
+    assert limbs_to_ints(key.decrypt_crt(edge)) == [H(v) for v in g["edge_crt"]]
+
+
+@pytest.mark.parametrize("key_len", [256, 512, 1024])
+def test_seeded_batches_every_variant(ctx, key_len):
+    """encrypt / decrypt / CRT on seeded batches for every built lane-group size."""
+    from cuhepy_amd.engine import Context, PaillierKey
+    from cuhepy_amd.limbs import limbs_to_ints
+    keys = O.key_gen(key_len, 300 + key_len)
+    n = keys["pk"]["n"]
+    key = PaillierKey(ctx, n, keys["sk"]["phi"], keys["sk"]["inv"], keys["p"], keys["q"])
+    rng = random.Random(key_len)
+    count = 97
+    ms = [rng.randrange(n) for _ in range(count)]
+    rs = [O.random_r(n, rng) for _ in range(count)]
+    want_c = [O.encrypt(m, keys["pk"], r) for m, r in zip(ms, rs)]
+    for tpi in Context.supported_tpi(key.width):
+        ctx.set_tuning(OP_MODEXP, key.width, tpi, 0)
+        assert limbs_to_ints(key.encrypt(ms, rs)) == want_c, f"encrypt tpi={tpi}"
+        assert limbs_to_ints(key.decrypt(want_c)) == ms, f"decrypt tpi={tpi}"
+    ctx.set_tuning(OP_MODEXP, key.width, 0, 0)
+    for tpi in Context.supported_tpi(key.half):
+        ctx.set_tuning(OP_MODEXP, key.half, tpi, 0)
+        ctx.set_tuning(OP_MULMOD, key.half, tpi, 0)
+        assert limbs_to_ints(key.decrypt_crt(want_c)) == ms, f"crt tpi={tpi}"
+
+
+def test_decode_hamming_fused(ctx, golden):
+    """decrypt + odd-bit popcount on the GPU = plaintext Hamming distance."""
+    from cuhepy_amd.engine import Modulus, PaillierKey
+    from cuhepy_amd.limbs import ints_to_limbs, limbs_to_ints
+    from cuhepy_amd.paillier.packing import embeddings_to_plaintexts, odd_mask
+    keys = O.key_gen(512, 77)
+    n, n2 = keys["pk"]["n"], keys["pk"]["n_squared"]
+    key = PaillierKey(ctx, n, keys["sk"]["phi"], keys["sk"]["inv"], keys["p"], keys["q"])
+    rng = random.Random(5)
+    embed_len = 400
+    A = np.array([[rng.getrandbits(1) for _ in range(embed_len)] for _ in range(33)], dtype=np.uint8)
+    B = np.array([[rng.getrandbits(1) for _ in range(embed_len)] for _ in range(33)], dtype=np.uint8)
+    ma = embeddings_to_plaintexts(A, 512, key.width)
+    mb = embeddings_to_plaintexts(B, 512, key.width)
+    ra = [O.random_r(n, rng) for _ in range(len(ma))]
+    rb = [O.random_r(n, rng) for _ in range(len(mb))]
+    ca, cb = key.encrypt(ma, ra), key.encrypt(mb, rb)
+    prod = Modulus(ctx, n2, key.width).mulmod(ca, cb)
+    counts = key.decode_hamming(prod, odd_mask(1024))
+    want = [O.hamming(a, b) for a, b in zip(A.tolist(), B.tolist())]
+    assert counts.tolist() == want
+    # the oracle path over the same ciphertexts agrees
+    assert [O.decode_hamming([O.decrypt(c, keys)], 1024) for c in limbs_to_ints(prod)] == want
+
+
+def test_decrypt_rejects_out_of_range(ctx, golden):
+    key = _key_from_golden(ctx, golden["paillier_w64"])
+    with pytest.raises(ValueError):
+        key.decrypt([key.n2])
+    with pytest.raises(ValueError):
+        key.encrypt([key.n], [1])
+
+
+def _lookup_key(ctx, g):
+    from cuhepy_amd.engine import LookupKey
+    n, gg = H(g["n"]), H(g["g"])
+    table = O.precompute_g_table(gg, n, g["key_len"])
+    noise = O.precompute_noise_table(gg, n, [H(x) for x in g["noise_r"]])
+    key = LookupKey(ctx, n, H(g["a"]), H(g["g_a_inv"]), [table[i] for i in range(len(table))], noise)
+    return key, table, noise
+
+
+def test_lookup_golden(ctx, golden):
+    from cuhepy_amd.limbs import ints_to_limbs, limbs_to_ints
+    from cuhepy_amd.paillier.packing import plaintexts_to_digits
+    g = golden["lookup_w32"]
+    key, _, _ = _lookup_key(ctx, g)
+    ms = [H(v) for v in g["m"]]
+    digits = plaintexts_to_digits(ints_to_limbs(ms, key.width), key.chunks)
+    idx = np.array(g["noise_idx"], dtype=np.uint8)
+    got = limbs_to_ints(key.encrypt_digits(digits, idx))
+    assert got == [H(v) for v in g["c"]]
+    assert limbs_to_ints(key.decrypt(got)) == ms
+
+
+def test_lookup_seeded_every_variant(ctx):
+    from cuhepy_amd.engine import Context, LookupKey
+    from cuhepy_amd.limbs import ints_to_limbs, limbs_to_ints
+    from cuhepy_amd.paillier.packing import plaintexts_to_digits
+    keys = O.lookup_key_gen(512, 280, 4242, with_tables=True)
+    n = keys["pk"]["n"]
+    table = keys["g_table"]
+    key = LookupKey(ctx, n, keys["sk"]["a"], keys["sk"]["g_a_inv"], [table[i] for i in range(len(table))],
+                    keys["noise_table"])
+    rng = random.Random(11)
+    count = 75
+    # sparse and dense digit patterns (rows where every item has digit 0 are skipped)
+    ms = [0, n - 1] + [rng.randrange(n) for _ in range(count - 12)] + [rng.getrandbits(64) for _ in range(10)]
+    idx = np.array([[rng.randrange(256) for _ in range(14)] for _ in ms], dtype=np.uint8)
+    want = [O.lookup_encrypt(m, keys["pk"], table, keys["noise_table"], keys["message_chunks"], ix.tolist())
+            for m, ix in zip(ms, idx)]
+    digits = plaintexts_to_digits(ints_to_limbs(ms, key.width), key.chunks)
+    for tpi in Context.supported_tpi(key.width):
+        ctx.set_tuning(OP_LOOKUP, key.width, tpi, 0)
+        ctx.set_tuning(OP_MODEXP, key.width, tpi, 0)
+        got = limbs_to_ints(key.encrypt_digits(digits, idx))
+        assert got == want, f"tpi={tpi}"
+        assert limbs_to_ints(key.decrypt(got)) == ms, f"decrypt tpi={tpi}"
