@@ -1094,26 +1094,41 @@ int chx_host_mont_consts(const uint32_t* n, int limbs, uint32_t* n0inv, uint32_t
   });
 }
 
-int chx_probe_mad(chx_ctx* ctx, int blocks, int iters, double* seconds, double* macs) {
+int chx_probe_rate(chx_ctx* ctx, int kind, int blocks, int iters, double* seconds, double* ops) {
   return with_ctx(ctx, [&] {
-    if (blocks <= 0 || iters <= 0 || !seconds || !macs) fail(CHX_EINVAL, "bad argument");
+    if (blocks <= 0 || iters <= 0 || !seconds || !ops || kind < 0 || kind > 5) fail(CHX_EINVAL, "bad argument");
     ctx->slot[S_AUX].ensure((size_t)blocks * 256 * 8);
+    u64* sink = ctx->slot[S_AUX].as<u64>();
+    auto launch = [&](int it) {
+      switch (kind) {
+        case 0: hipLaunchKernelGGL(k_probe<0>, dim3(blocks), dim3(256), 0, ctx->stream, sink, it, 7u); break;
+        case 1: hipLaunchKernelGGL(k_probe<1>, dim3(blocks), dim3(256), 0, ctx->stream, sink, it, 7u); break;
+        case 2: hipLaunchKernelGGL(k_probe<2>, dim3(blocks), dim3(256), 0, ctx->stream, sink, it, 7u); break;
+        case 3: hipLaunchKernelGGL(k_probe<3>, dim3(blocks), dim3(256), 0, ctx->stream, sink, it, 7u); break;
+        case 4: hipLaunchKernelGGL(k_probe<4>, dim3(blocks), dim3(256), 0, ctx->stream, sink, it, 7u); break;
+        default: hipLaunchKernelGGL(k_probe<5>, dim3(blocks), dim3(256), 0, ctx->stream, sink, it, 7u); break;
+      }
+    };
     hipEvent_t a, b;
     hip_check(hipEventCreate(&a), "hipEventCreate");
     hip_check(hipEventCreate(&b), "hipEventCreate");
-    hipLaunchKernelGGL(k_probe_mad, dim3(blocks), dim3(256), 0, ctx->stream, ctx->slot[S_AUX].as<u64>(), 8, 7u);
+    launch(8);
     hip_check(hipEventRecord(a, ctx->stream), "hipEventRecord");
-    hipLaunchKernelGGL(k_probe_mad, dim3(blocks), dim3(256), 0, ctx->stream, ctx->slot[S_AUX].as<u64>(), iters, 7u);
+    launch(iters);
     hip_check(hipEventRecord(b, ctx->stream), "hipEventRecord");
-    check_launch("k_probe_mad");
+    check_launch("k_probe");
     hip_check(hipEventSynchronize(b), "hipEventSynchronize");
     float ms = 0.f;
     hip_check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     *seconds = ms * 1e-3;
-    *macs = (double)blocks * 256.0 * (double)iters * 16.0;
+    *ops = (double)blocks * 256.0 * (double)iters * 16.0;
   });
+}
+
+int chx_probe_mad(chx_ctx* ctx, int blocks, int iters, double* seconds, double* macs) {
+  return chx_probe_rate(ctx, 0, blocks, iters, seconds, macs);
 }
 
 int chx_ctx_set_timing(chx_ctx* ctx, int enable) {

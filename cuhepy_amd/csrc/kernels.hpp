@@ -488,23 +488,50 @@ __global__ void k_il_to_aos(const u32* __restrict__ il, u32* __restrict__ aos, i
 }
 
 // ---------------------------------------------------------------------------
-// Instruction-rate probe: `chains` independent v_mad_u64_u32 accumulators per
-// lane, `iters` rounds; the host times it to pin the integer-multiply peak.
+// Instruction-rate probes: `iters` rounds of 16 independent operations per
+// lane (8 chains x 2); the host times them to pin issue rates on the device.
+//   kind 0: v_mad_u64_u32   (u64)a*b + acc
+//   kind 1: v_mul_lo_u32    a*b (low 32 bits)
+//   kind 2: v_mul_hi_u32    high 32 bits of a*b
+//   kind 3: 64-bit add      acc + v
+//   kind 4: v_fma_f64
+//   kind 5: v_mad_u32_u24   24x24 + 32
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_probe_mad(u64* __restrict__ sink, int iters, u32 seed) {
+// The operations are written as inline asm so the compiler can neither hoist
+// nor fold them; each asm statement is one ALU instruction on registers (no
+// memory access), 8 independent chains per lane.
+template <int KIND>
+__global__ __launch_bounds__(256) void k_probe(u64* __restrict__ sink, int iters, u32 seed) {
+  u32 b = seed * 2654435761u + blockIdx.x;
   u64 acc[8];
-  u32 a = seed ^ threadIdx.x, b = seed * 2654435761u + blockIdx.x;
+  u32 w[8];
+  double d[8];
 #pragma unroll
-  for (int c = 0; c < 8; ++c) acc[c] = (u64)(a + c) << 7;
+  for (int c = 0; c < 8; ++c) {
+    acc[c] = ((u64)(threadIdx.x + c) << 33) | (seed + c);
+    w[c] = seed ^ (threadIdx.x * 7 + c);
+    d[c] = (double)(threadIdx.x + c);
+  }
+  const u64 b64 = ((u64)b << 17) | 1u;
+  const double x = 1.0000001, y = 0.25;
   for (int it = 0; it < iters; ++it) {
 #pragma unroll
-    for (int c = 0; c < 8; ++c) acc[c] = (u64)a * (u32)(b + c) + acc[c];
+    for (int rep = 0; rep < 2; ++rep) {
 #pragma unroll
-    for (int c = 0; c < 8; ++c) acc[c] = (u64)b * (u32)(a + c) + acc[c];
+      for (int c = 0; c < 8; ++c) {
+        if constexpr (KIND == 0)
+          asm volatile("v_mad_u64_u32 %0, vcc, %1, %2, %0" : "+v"(acc[c]) : "v"(w[c]), "v"(b) : "vcc");
+        if constexpr (KIND == 1) asm volatile("v_mul_lo_u32 %0, %0, %1" : "+v"(w[c]) : "v"(b));
+        if constexpr (KIND == 2) asm volatile("v_mul_hi_u32 %0, %0, %1" : "+v"(w[c]) : "v"(b));
+        if constexpr (KIND == 3) asm volatile("v_lshl_add_u64 %0, %0, 0, %1" : "+v"(acc[c]) : "v"(b64));
+        if constexpr (KIND == 4) asm volatile("v_fma_f64 %0, %0, %1, %2" : "+v"(d[c]) : "v"(x), "v"(y));
+        if constexpr (KIND == 5) asm volatile("v_mad_u32_u24 %0, %0, %1, %0" : "+v"(w[c]) : "v"(b));
+      }
+    }
   }
   u64 s = 0;
 #pragma unroll
-  for (int c = 0; c < 8; ++c) s ^= acc[c];
+  for (int c = 0; c < 8; ++c) s ^= acc[c] ^ (u64)w[c] ^ (u64)__double_as_longlong(d[c]);
   if (s == 0x1234567ull) sink[blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 

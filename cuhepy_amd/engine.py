@@ -118,6 +118,11 @@ class Context:
         N.check(N.lib.chx_probe_mad(self.handle, blocks, iters, ctypes.byref(s), ctypes.byref(m)))
         return s.value, m.value
 
+    def probe_rate(self, kind: int, blocks: int, iters: int) -> tuple[float, float]:
+        s, m = ctypes.c_double(), ctypes.c_double()
+        N.check(N.lib.chx_probe_rate(self.handle, kind, blocks, iters, ctypes.byref(s), ctypes.byref(m)))
+        return s.value, m.value
+
     # layout conversion on device
     def aos_to_il(self, aos, il, count: int, limbs: int, stride: int) -> None:
         N.check(N.lib.chx_aos_to_il_dev(self.handle, _dptr(aos), _dptr(il), count, limbs, stride))
@@ -207,7 +212,8 @@ class PaillierKey:
         ph = ints_to_limbs([phi], nl)
         iv = ints_to_limbs([inv], nl)
         h = ctypes.c_void_p()
-        self.crt = p is not None and q is not None
+        # CRT kernels are built for 32- and 64-limb halves (n^2 of 2048 or 4096 bits)
+        self.crt = p is not None and q is not None and self.half in (32, 64)
         if self.crt:
             if p * q != n:
                 raise ValueError("p * q != n")

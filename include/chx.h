@@ -175,6 +175,9 @@ int chx_host_mont_consts(const uint32_t* n, int limbs, uint32_t* n0inv, uint32_t
 /* time `iters` rounds of 16 dependent-free v_mad_u64_u32 per lane on `blocks`
  * blocks of 256 lanes; returns seconds and the number of 32x32->64 MACs */
 int chx_probe_mad(chx_ctx* ctx, int blocks, int iters, double* seconds, double* macs);
+/* same for other instruction kinds: 0 v_mad_u64_u32, 1 v_mul_lo_u32, 2 v_mul_hi_u32,
+ * 3 64-bit add, 4 v_fma_f64, 5 v_mad_u32_u24; *ops = operations executed */
+int chx_probe_rate(chx_ctx* ctx, int kind, int blocks, int iters, double* seconds, double* ops);
 /* HIP-event timing of the compute kernels enqueued by this context (events
  * recorded on the context's stream around every launch while enabled) */
 int chx_ctx_set_timing(chx_ctx* ctx, int enable);
