@@ -27,6 +27,8 @@ HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
+    # the exponentiation kernels fully unroll K rows x K positions
+    "-mllvm", "-pragma-unroll-threshold=1000000",
 ]
 
 # translation units; each compiles the API object or a slice of kernels

@@ -43,6 +43,7 @@ SIGNATURES = {
     "chx_ctx_sync": (c_int, [vp]),
     "chx_ctx_set_tuning": (c_int, [vp, c_int, c_int, c_int, c_int]),
     "chx_supported_tpi": (c_int, [c_int, POINTER(c_int), c_int]),
+    "chx_supported_tpi_op": (c_int, [c_int, c_int, POINTER(c_int), c_int]),
     "chx_ctx_scratch_bytes": (c_size_t, [vp]),
     "chx_mod_create": (c_int, [vp, u32p, c_int, POINTER(vp)]),
     "chx_mod_destroy": (c_int, [vp]),

@@ -15,6 +15,7 @@
 #include "../../include/chx.h"
 #include "host_bn.hpp"
 #include "kernels.hpp"
+#include "lazy.hpp"
 
 using namespace chx;
 using chx::host::BN;
@@ -92,7 +93,7 @@ struct DevBuf {
 
 }  // namespace
 
-enum Slot { S_IN0, S_IN1, S_OUT, S_AOS, S_TAB, S_TMP, S_AUX, S_NSLOT };
+enum Slot { S_IN0, S_IN1, S_OUT, S_AOS, S_TAB, S_TMP, S_AUX, S_MID0, S_MID1, S_NSLOT };
 
 struct TimedLaunch {
   std::string name;
@@ -112,10 +113,18 @@ struct chx_ctx {
   std::map<std::string, std::pair<double, int>> stats;
 };
 
+// Lazy-radix constants of one modulus, one block per built lazy configuration
+// of its width (the configurations differ in positions per number).
+struct LazySet {
+  int L32 = 0;
+  LazyDev* d[2] = {nullptr, nullptr};
+};
+
 struct chx_mod {
   chx_ctx* ctx;
   host::MontConsts hc;
   ModDev* d = nullptr;
+  LazySet lz;
 };
 
 struct ExpSched {
@@ -132,6 +141,7 @@ struct chx_pkey {
   bool crt = false;
   host::MontConsts n2, n;
   PaillierDev* d = nullptr;
+  LazySet lz_n2, lz_p2, lz_q2;
   BN e_n, e_phi, e_pm1, e_qm1;
   std::map<std::pair<int, int>, ExpSched> scheds;  // (which, window) -> schedule
 };
@@ -147,6 +157,7 @@ struct chx_lkey {
   uint32_t* d_gar = nullptr;   // g_a_inv * R mod n
   uint32_t* d_gtab = nullptr;  // Montgomery-form g table, AoS
   uint32_t* d_ntab = nullptr;  // Montgomery-form noise table, AoS
+  LazySet lz_n2;
   BN e_a;
   std::map<int, ExpSched> scheds;
 };
@@ -366,6 +377,129 @@ int exp_bits_p(const chx_pkey* k, int which) {
   return host::bn_bits(e);
 }
 
+
+// ---------------------------------------------------------------------------
+// lazy-radix exponentiation configurations: (radix bits, positions, lanes)
+// ---------------------------------------------------------------------------
+struct LazyCfg {
+  int rb, lp, tpi;
+};
+
+const std::vector<LazyCfg>& lazy_cfgs(int L32) {
+  // variants whose accumulators fit in VGPRs without spilling (checked with
+  // -Rpass-analysis=kernel-resource-usage); index = dispatch slot
+  static const std::vector<LazyCfg> c32 = {{28, 38, 1}, {28, 38, 2}};
+  static const std::vector<LazyCfg> c64 = {{28, 76, 4}};
+  static const std::vector<LazyCfg> c128 = {{27, 152, 8}};
+  static const std::vector<LazyCfg> none = {};
+  return L32 == 32 ? c32 : L32 == 64 ? c64 : L32 == 128 ? c128 : none;
+}
+
+// split a value (width L32 limbs) into `lp` positions of `rb` bits
+void to_positions(const BN& v, int rb, int lp, uint32_t* out) {
+  const uint32_t mask = (1u << rb) - 1u;
+  for (int p = 0; p < lp; ++p) {
+    const int bit = rb * p, j = bit >> 5, off = bit & 31;
+    uint64_t lo = j < (int)v.size() ? v[j] : 0u;
+    uint64_t hi = j + 1 < (int)v.size() ? v[j + 1] : 0u;
+    out[p] = (uint32_t)(((hi << 32) | lo) >> off) & mask;
+  }
+}
+
+LazySet make_lazy_set(const host::MontConsts& c, hipStream_t st) {
+  LazySet ls;
+  ls.L32 = c.L;
+  const auto& cfgs = lazy_cfgs(c.L);
+  for (size_t i = 0; i < cfgs.size() && i < 2; ++i) {
+    const int rb = cfgs[i].rb, lp = cfgs[i].lp;
+    if (rb * lp < 32 * c.L + 2) fail(CHX_EUNSUPPORTED, "lazy configuration too narrow");
+    std::unique_ptr<LazyDev> ld(new LazyDev());
+    memset(ld.get(), 0, sizeof(LazyDev));
+    ld->n0 = c.n0inv & ((1u << rb) - 1u);
+    ld->rb = rb;
+    ld->lp = lp;
+    ld->l32 = c.L;
+    to_positions(c.N, rb, lp, ld->N);
+    BN one(c.L, 0u);
+    one[0] = 1;
+    BN r2 = host::bn_shl_mod(one, 2 * rb * lp, c.N);  // R'^2 mod N
+    to_positions(r2, rb, lp, ld->R2);
+    for (int k = 0; k < c.L; ++k) ld->N32[k] = c.N[k];
+    ls.d[i] = dev_upload(ld.get(), 1, st);
+  }
+  return ls;
+}
+
+void free_lazy_set(LazySet& ls) {
+  for (auto& p : ls.d)
+    if (p) (void)hipFree(p), p = nullptr;
+}
+
+// pick the lazy configuration for a batch (tuning override, else fill the chip)
+int pick_lazy(chx_ctx* ctx, int L32, size_t count) {
+  const int li = lidx(L32);
+  if (li < 0) fail(CHX_EUNSUPPORTED, "width of %d limbs is not built", L32);
+  const auto& cfgs = lazy_cfgs(L32);
+  const int forced = ctx->tune_tpi[CHX_OP_MODEXP][li];
+  if (forced) {
+    for (size_t i = 0; i < cfgs.size(); ++i)
+      if (cfgs[i].tpi == forced) return (int)i;
+    fail(CHX_EUNSUPPORTED, "tpi=%d is not built for %d-limb exponentiation", forced, L32);
+  }
+  const size_t want = target_lanes(ctx->device);
+  for (size_t i = 0; i < cfgs.size(); ++i)
+    if (count * (size_t)cfgs[i].tpi >= want) return (int)i;
+  return (int)cfgs.size() - 1;
+}
+
+size_t lazy_lds_bytes(int lp, int l32, int tpi) {
+  return (size_t)WAVES_PER_BLOCK * (size_t)std::max(lp, l32) * (64 / tpi) * 4;
+}
+
+#define CHX_DISPATCH_LAZY(L32_, IDX_, CALL)                                                     \
+  do {                                                                                          \
+    const int l__ = (L32_), i__ = (IDX_);                                                       \
+    if (l__ == 32 && i__ == 0) { constexpr int RB = 28, LP = 38, TPI = 1, L32 = 32; CALL; }      \
+    else if (l__ == 32 && i__ == 1) { constexpr int RB = 28, LP = 38, TPI = 2, L32 = 32; CALL; } \
+    else if (l__ == 64 && i__ == 0) { constexpr int RB = 28, LP = 76, TPI = 4, L32 = 64; CALL; }  \
+    else if (l__ == 128 && i__ == 0) { constexpr int RB = 27, LP = 152, TPI = 8, L32 = 128; CALL; } \
+    else fail(CHX_EUNSUPPORTED, "no lazy kernel for %d limbs (configuration %d)", l__, i__);   \
+  } while (0)
+
+// One launch exponentiating one or two batches (CRT halves) of `count` items.
+// jobs[j]: input/output interleaved 32-bit buffers, schedule; the modulus
+// block of the chosen configuration is taken from sets[j].
+void run_lazy(chx_ctx* ctx, int L32, const LazySet* sets, const ExpSched* const* scheds, const uint32_t* const* in,
+              uint32_t* const* out, int njobs, size_t count, size_t stride) {
+  const int ci = pick_lazy(ctx, L32, count * njobs);
+  const LazyCfg cfg = lazy_cfgs(L32)[ci];
+  int window = scheds[0]->window;
+  for (int j = 1; j < njobs; ++j)
+    if (scheds[j]->window != window) fail(CHX_EINVAL, "jobs of one launch must share the window width");
+  const int tentries = 1 << (window - 1);
+  const size_t tstride = round_up(count, BLOCK_THREADS);
+  const size_t tab_words = (size_t)tentries * cfg.lp * tstride;
+  ctx->slot[S_TAB].ensure(tab_words * njobs * sizeof(uint32_t));
+  LazyJob jb[2];
+  for (int j = 0; j < 2; ++j) {
+    const int s = j < njobs ? j : 0;
+    jb[j].md = sets[s].d[ci];
+    jb[j].sched = scheds[s]->d;
+    jb[j].nsched = (int)scheds[s]->h.size();
+    jb[j].in = in[s];
+    jb[j].out = out[s];
+    jb[j].table = ctx->slot[S_TAB].as<uint32_t>() + (size_t)s * tab_words;
+  }
+  record_start(ctx, "lazy_modexp");
+  CHX_DISPATCH_LAZY(L32, ci, {
+    hipLaunchKernelGGL((k_lazy_modexp<RB, LP, TPI, L32>), grid_for(count, TPI, njobs), dim3(BLOCK_THREADS),
+                       lazy_lds_bytes(LP, L32, TPI), ctx->stream, jb[0], jb[1], (int)count, (int)stride, tentries,
+                       (int)tstride);
+  });
+  record_stop(ctx);
+  check_launch("k_lazy_modexp");
+}
+
 // --------------------------------------------------------------------------
 // launchers (device-resident, interleaved)
 // --------------------------------------------------------------------------
@@ -381,87 +515,82 @@ void launch_mulmod(chx_ctx* ctx, const ModDev* md, int L, const uint32_t* a, con
   check_launch("k_mulmod");
 }
 
-void launch_modexp(chx_ctx* ctx, const ModDev* md, int L, const uint32_t* base, uint32_t* out, size_t count,
+void launch_modexp(chx_ctx* ctx, const chx_mod* m, const uint32_t* base, uint32_t* out, size_t count,
                    size_t stride, const ExpSched& s) {
-  const int tpi = pick_tpi(ctx, CHX_OP_MODEXP, L, count);
-  const int tentries = 1 << (s.window - 1);
-  const size_t tstride = round_up(count, BLOCK_THREADS);
-  ctx->slot[S_TAB].ensure((size_t)tentries * L * tstride * sizeof(uint32_t));
-  record_start(ctx, "modexp");
-  CHX_DISPATCH(L, tpi, {
-    hipLaunchKernelGGL((k_modexp<L, TPI>), grid_for(count, TPI), dim3(BLOCK_THREADS), lds_bytes(L, TPI),
-                       ctx->stream, md, base, out, (int)count, (int)stride, s.d, (int)s.h.size(), tentries,
-                       ctx->slot[S_TAB].as<uint32_t>(), (int)tstride);
-  });
-  record_stop(ctx);
-  check_launch("k_modexp");
+  const ExpSched* sp = &s;
+  run_lazy(ctx, m->hc.L, &m->lz, &sp, &base, &out, 1, count, stride);
 }
 
-void launch_decrypt(chx_ctx* ctx, const ModDev* n2, const ModDev* nm, const uint32_t* cR, int L,
+// m = L(c^e mod n^2) * cR mod n (+ optional Hamming popcount): lazy
+// exponentiation into a scratch buffer, then the canonical finish.
+void launch_decrypt(chx_ctx* ctx, const LazySet& lz, const ModDev* nm, const uint32_t* cR, int L,
                     const uint32_t* c, uint32_t* out, size_t count, size_t stride, const ExpSched& s,
                     const uint32_t* mask, uint32_t* pop) {
-  const int tpi = pick_tpi(ctx, CHX_OP_MODEXP, L, count);
-  const int tentries = 1 << (s.window - 1);
-  const size_t tstride = round_up(count, BLOCK_THREADS);
-  ctx->slot[S_TAB].ensure((size_t)tentries * L * tstride * sizeof(uint32_t));
-  record_start(ctx, "decrypt");
+  ctx->slot[S_MID0].ensure(stride * L * sizeof(uint32_t));
+  uint32_t* u = ctx->slot[S_MID0].as<uint32_t>();
+  const ExpSched* sp = &s;
+  run_lazy(ctx, L, &lz, &sp, &c, &u, 1, count, stride);
+  const int tpi = pick_tpi(ctx, CHX_OP_MULMOD, L, count);
+  record_start(ctx, "decrypt_finish");
   CHX_DISPATCH(L, tpi, {
-    hipLaunchKernelGGL((k_decrypt<L, TPI>), grid_for(count, TPI), dim3(BLOCK_THREADS), lds_bytes(L, TPI),
-                       ctx->stream, n2, nm, cR, c, out, (int)count, (int)stride, s.d, (int)s.h.size(),
-                       tentries, ctx->slot[S_TAB].as<uint32_t>(), (int)tstride, mask, pop);
+    hipLaunchKernelGGL((k_decrypt_finish<L, TPI>), grid_for(count, TPI), dim3(BLOCK_THREADS), lds_bytes(L, TPI),
+                       ctx->stream, nm, cR, u, out, (int)count, (int)stride, mask, pop);
   });
   record_stop(ctx);
-  check_launch("k_decrypt");
+  check_launch("k_decrypt_finish");
 }
 
 void launch_encrypt(chx_pkey* k, const uint32_t* m, const uint32_t* r, uint32_t* c, size_t count,
                     size_t stride) {
   chx_ctx* ctx = k->ctx;
   const int L = k->W;
-  const int tpi = pick_tpi(ctx, CHX_OP_MODEXP, L, count);
   const ExpSched& s = get_sched_p(k, 0, pick_window(ctx, CHX_OP_MODEXP, L, exp_bits_p(k, 0)));
-  const int tentries = 1 << (s.window - 1);
-  const size_t tstride = round_up(count, BLOCK_THREADS);
-  ctx->slot[S_TAB].ensure((size_t)tentries * L * tstride * sizeof(uint32_t));
-  record_start(ctx, "encrypt");
+  ctx->slot[S_MID0].ensure(stride * L * sizeof(uint32_t));
+  uint32_t* y = ctx->slot[S_MID0].as<uint32_t>();
+  const ExpSched* sp = &s;
+  run_lazy(ctx, L, &k->lz_n2, &sp, &r, &y, 1, count, stride);  // y = r^n mod n^2
+  const int tpi = pick_tpi(ctx, CHX_OP_MULMOD, L, count);
+  record_start(ctx, "encrypt_finish");
   CHX_DISPATCH(L, tpi, {
-    hipLaunchKernelGGL((k_encrypt<L, TPI>), grid_for(count, TPI), dim3(BLOCK_THREADS), lds_bytes(L, TPI),
-                       ctx->stream, k->d, m, r, c, (int)count, (int)stride, s.d, (int)s.h.size(), tentries,
-                       ctx->slot[S_TAB].as<uint32_t>(), (int)tstride);
+    hipLaunchKernelGGL((k_encrypt_finish<L, TPI>), grid_for(count, TPI), dim3(BLOCK_THREADS), lds_bytes(L, TPI),
+                       ctx->stream, k->d, m, y, c, (int)count, (int)stride);
   });
   record_stop(ctx);
-  check_launch("k_encrypt");
+  check_launch("k_encrypt_finish");
 }
 
 void launch_crt(chx_pkey* k, const uint32_t* c, uint32_t* m, size_t count, size_t stride) {
   chx_ctx* ctx = k->ctx;
   if (!k->crt) fail(CHX_EINVAL, "key was created without CRT factors");
   const int H = k->H;
-  const int tpi = pick_tpi(ctx, CHX_OP_MODEXP, H, 2 * count);
   const int w = pick_window(ctx, CHX_OP_MODEXP, H, std::max(exp_bits_p(k, 2), exp_bits_p(k, 3)));
   const ExpSched& sp = get_sched_p(k, 2, w);
   const ExpSched& sq = get_sched_p(k, 3, w);
-  const int tentries = 1 << (w - 1);
-  const size_t tstride = round_up(count, BLOCK_THREADS);
-  ctx->slot[S_TAB].ensure((size_t)2 * tentries * H * tstride * sizeof(uint32_t));
-  ctx->slot[S_TMP].ensure((size_t)2 * H * stride * sizeof(uint32_t));
-  uint32_t* halves = ctx->slot[S_TMP].as<uint32_t>();
-  record_start(ctx, "crt_half");
+  const size_t half_words = (size_t)H * stride;
+  ctx->slot[S_MID0].ensure(2 * half_words * sizeof(uint32_t));
+  ctx->slot[S_MID1].ensure(2 * half_words * sizeof(uint32_t));
+  uint32_t* x = ctx->slot[S_MID0].as<uint32_t>();
+  uint32_t* u = ctx->slot[S_MID1].as<uint32_t>();
+  const int tpi = pick_tpi(ctx, CHX_OP_MULMOD, H, count);
+  record_start(ctx, "crt_fold");
   CHX_DISPATCH_H(H, tpi, {
-    hipLaunchKernelGGL((k_crt_half<H, TPI>), grid_for(count, TPI, 2), dim3(BLOCK_THREADS), lds_bytes(H, TPI),
-                       ctx->stream, k->d, c, halves, (int)count, (int)stride, sp.d, (int)sp.h.size(), sq.d,
-                       (int)sq.h.size(), tentries, ctx->slot[S_TAB].as<uint32_t>(), (int)tstride);
+    hipLaunchKernelGGL((k_crt_fold<H, TPI>), grid_for(count, TPI, 2), dim3(BLOCK_THREADS), lds_bytes(H, TPI),
+                       ctx->stream, k->d, c, x, (int)count, (int)stride);
   });
   record_stop(ctx);
-  check_launch("k_crt_half");
-  const int tpi2 = pick_tpi(ctx, CHX_OP_MULMOD, H, count);
-  record_start(ctx, "crt_combine");
-  CHX_DISPATCH_H(H, tpi2, {
-    hipLaunchKernelGGL((k_crt_combine<H, TPI>), grid_for(count, TPI), dim3(BLOCK_THREADS), lds_bytes(H, TPI),
-                       ctx->stream, k->d, halves, m, (int)count, (int)stride);
+  check_launch("k_crt_fold");
+  const LazySet sets[2] = {k->lz_p2, k->lz_q2};
+  const ExpSched* scheds[2] = {&sp, &sq};
+  const uint32_t* ins[2] = {x, x + half_words};
+  uint32_t* outs[2] = {u, u + half_words};
+  run_lazy(ctx, H, sets, scheds, ins, outs, 2, count, stride);
+  record_start(ctx, "crt_finish");
+  CHX_DISPATCH_H(H, tpi, {
+    hipLaunchKernelGGL((k_crt_finish<H, TPI>), grid_for(count, TPI), dim3(BLOCK_THREADS), lds_bytes(H, TPI),
+                       ctx->stream, k->d, u, m, (int)count, (int)stride);
   });
   record_stop(ctx);
-  check_launch("k_crt_combine");
+  check_launch("k_crt_finish");
 }
 
 const ExpSched& get_sched_l(chx_lkey* k, int window) {
@@ -569,9 +698,14 @@ int chx_ctx_set_tuning(chx_ctx* ctx, int op, int limbs, int tpi, int window) {
     const int li = lidx(limbs);
     if (op < 0 || op > 3 || li < 0) fail(CHX_EINVAL, "bad op %d or limbs %d", op, limbs);
     if (tpi) {
-      const auto& opts = built_tpi(limbs);
-      if (std::find(opts.begin(), opts.end(), tpi) == opts.end())
-        fail(CHX_EUNSUPPORTED, "tpi=%d is not built for %d limbs", tpi, limbs);
+      bool ok = false;
+      if (op == CHX_OP_MODEXP) {
+        for (const auto& c : lazy_cfgs(limbs)) ok = ok || c.tpi == tpi;
+      } else {
+        const auto& opts = built_tpi(limbs);
+        ok = std::find(opts.begin(), opts.end(), tpi) != opts.end();
+      }
+      if (!ok) fail(CHX_EUNSUPPORTED, "tpi=%d is not built for op %d at %d limbs", tpi, op, limbs);
     }
     if (window < 0 || window > 7) fail(CHX_EINVAL, "window %d out of range [1, 7]", window);
     ctx->tune_tpi[op][li] = tpi;
@@ -582,6 +716,13 @@ int chx_ctx_set_tuning(chx_ctx* ctx, int op, int limbs, int tpi, int window) {
 int chx_supported_tpi(int limbs, int* tpis, int cap) {
   const auto& v = built_tpi(limbs);
   for (int i = 0; i < (int)v.size() && i < cap; ++i) tpis[i] = v[i];
+  return (int)v.size();
+}
+
+int chx_supported_tpi_op(int op, int limbs, int* tpis, int cap) {
+  if (op != CHX_OP_MODEXP) return chx_supported_tpi(limbs, tpis, cap);
+  const auto& v = lazy_cfgs(limbs);
+  for (int i = 0; i < (int)v.size() && i < cap; ++i) tpis[i] = v[i].tpi;
   return (int)v.size();
 }
 
@@ -603,6 +744,7 @@ int chx_mod_create(chx_ctx* ctx, const uint32_t* n, int limbs, chx_mod** out) {
       ModDev md;
       fill_moddev(md, m->hc);
       m->d = dev_upload(&md, 1, ctx->stream);
+      m->lz = make_lazy_set(m->hc, ctx->stream);
     } catch (...) {
       delete m;
       throw;
@@ -616,6 +758,7 @@ int chx_mod_destroy(chx_mod* m) {
   int rc = guarded([&] {
     hip_check(hipSetDevice(m->ctx->device), "hipSetDevice");
     if (m->d) (void)hipFree(m->d);
+    free_lazy_set(m->lz);
   });
   delete m;
   return rc;
@@ -699,7 +842,7 @@ int chx_modexp(chx_ctx* ctx, const chx_mod* m, const uint32_t* base, size_t coun
               "H2D");
     s.d = ctx->slot[S_AUX].as<uint32_t>();
     ctx->slot[S_OUT].ensure(stride * L * 4);
-    launch_modexp(ctx, m->d, L, db, ctx->slot[S_OUT].as<uint32_t>(), count, stride, s);
+    launch_modexp(ctx, m, db, ctx->slot[S_OUT].as<uint32_t>(), count, stride, s);
     download_il(ctx, ctx->slot[S_OUT].as<uint32_t>(), out, count, L, stride);
     s.d = nullptr;
   });
@@ -718,7 +861,7 @@ int chx_modexp_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* base, uint32_
     hip_check(hipMemcpyAsync(ctx->slot[S_AUX].p, s.h.data(), s.h.size() * 4, hipMemcpyHostToDevice, ctx->stream),
               "H2D");
     s.d = ctx->slot[S_AUX].as<uint32_t>();
-    launch_modexp(ctx, m->d, L, base, out, count, stride, s);
+    launch_modexp(ctx, m, base, out, count, stride, s);
     s.d = nullptr;
   });
 }
@@ -769,6 +912,8 @@ int chx_paillier_key_create(chx_ctx* ctx, const uint32_t* n, int n_limbs, const 
       host::MontConsts cn = host::mont_consts(nH);
       fill_moddev(pd->p2, cp2);
       fill_moddev(pd->q2, cq2);
+      k->lz_p2 = make_lazy_set(cp2, ctx->stream);
+      k->lz_q2 = make_lazy_set(cq2, ctx->stream);
       fill_moddev(pd->p, cp);
       fill_moddev(pd->q, cq);
       fill_moddev(pd->nh, cn);
@@ -784,6 +929,7 @@ int chx_paillier_key_create(chx_ctx* ctx, const uint32_t* n, int n_limbs, const 
       host::bn_sub(k->e_qm1, one);
       k->crt = true;
     }
+    k->lz_n2 = make_lazy_set(k->n2, ctx->stream);
     k->d = dev_upload(pd.get(), 1, ctx->stream);
     *out = k.release();
   });
@@ -794,6 +940,9 @@ int chx_paillier_key_destroy(chx_pkey* k) {
   int rc = guarded([&] {
     hip_check(hipSetDevice(k->ctx->device), "hipSetDevice");
     if (k->d) (void)hipFree(k->d);
+    free_lazy_set(k->lz_n2);
+    free_lazy_set(k->lz_p2);
+    free_lazy_set(k->lz_q2);
     for (auto& s : k->scheds)
       if (s.second.d) (void)hipFree(s.second.d);
   });
@@ -839,7 +988,7 @@ static void paillier_decrypt_impl(chx_pkey* k, const uint32_t* dc, uint32_t* dm,
   const int L = k->W;
   const ExpSched& s = get_sched_p(k, 1, pick_window(ctx, CHX_OP_MODEXP, L, exp_bits_p(k, 1)));
   PaillierDev* d = k->d;
-  launch_decrypt(ctx, &d->n2, &d->n, d->muR.v, L, dc, dm, count, stride, s, mask, pop);
+  launch_decrypt(ctx, k->lz_n2, &d->n, d->muR.v, L, dc, dm, count, stride, s, mask, pop);
 }
 
 int chx_paillier_decrypt(chx_ctx* ctx, const chx_pkey* kc, const uint32_t* c, uint32_t* m, size_t count) {
@@ -941,6 +1090,7 @@ int chx_lookup_key_create(chx_ctx* ctx, const uint32_t* n, int n_limbs, const ui
     k->d_n2 = dev_upload(&md, 1, ctx->stream);
     fill_moddev(md, k->n);
     k->d_n = dev_upload(&md, 1, ctx->stream);
+    k->lz_n2 = make_lazy_set(k->n2, ctx->stream);
     BN gar = host::to_mont_host(host::bn_from(g_a_inv, n_limbs, W), k->n);
     k->d_gar = dev_upload(gar.data(), W, ctx->stream);
     // tables to Montgomery form (host shift-and-reduce; once per key)
@@ -968,6 +1118,7 @@ int chx_lookup_key_destroy(chx_lkey* k) {
     hip_check(hipSetDevice(k->ctx->device), "hipSetDevice");
     for (void* p : {(void*)k->d_n2, (void*)k->d_n, (void*)k->d_gar, (void*)k->d_gtab, (void*)k->d_ntab})
       if (p) (void)hipFree(p);
+    free_lazy_set(k->lz_n2);
     for (auto& s : k->scheds)
       if (s.second.d) (void)hipFree(s.second.d);
   });
@@ -1017,7 +1168,7 @@ static void lookup_decrypt_impl(chx_lkey* k, const uint32_t* dc, uint32_t* dm, s
   chx_ctx* ctx = k->ctx;
   const int L = k->W;
   const ExpSched& s = get_sched_l(k, pick_window(ctx, CHX_OP_MODEXP, L, host::bn_bits(k->e_a)));
-  launch_decrypt(ctx, k->d_n2, k->d_n, k->d_gar, L, dc, dm, count, stride, s, mask, pop);
+  launch_decrypt(ctx, k->lz_n2, k->d_n, k->d_gar, L, dc, dm, count, stride, s, mask, pop);
 }
 
 int chx_lookup_decrypt(chx_ctx* ctx, const chx_lkey* kc, const uint32_t* c, uint32_t* m, size_t count) {

@@ -210,26 +210,6 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_mulmod_bcast(const ModDev* __
   if (item < count) store_dist<L, TPI>(out, t, item, stride);
 }
 
-// out = base^e mod N, e shared by the batch (schedule built on the host)
-template <int L, int TPI>
-__global__ __launch_bounds__(BLOCK_THREADS) void k_modexp(const ModDev* __restrict__ md, const u32* __restrict__ base,
-                                                          u32* __restrict__ out, int count, int stride,
-                                                          const u32* __restrict__ sched, int nsched,
-                                                          int tentries, u32* __restrict__ table, int tstride) {
-  extern __shared__ uint4 smem[];
-  constexpr int K = L / TPI;
-  Mont<L, TPI> M;
-  mont_init<L, TPI>(M, md, smem);
-  const int item = group_item<TPI>();
-  const int it = item < count ? item : count - 1;
-  u32 x[K];
-  load_dist<L, TPI>(x, base, it, stride);
-  to_mont<L, TPI>(M, x, md);
-  modexp_sched<L, TPI>(M, x, sched, nsched, tentries, table, item, tstride);
-  from_mont<L, TPI>(M, x);
-  if (item < count) store_dist<L, TPI>(out, x, item, stride);
-}
-
 // m = ((u - 1) // n) * c mod n with Python floor-division semantics, as the
 // reference computes it (scheme.py:101-104, lookup.py:275-278):
 //   u >= 1:  w = u - 1, q = (w - (w mod n)) / n  (exact division by n)
@@ -278,29 +258,25 @@ __device__ __forceinline__ u32 group_popcount(const u32 (&m)[L / TPI], const u32
   return c;
 }
 
-// m = L(c^e mod n^2) * cR mod n  (plain decrypt e = phi, cR = inv*R;
-// lookup decrypt e = a, cR = g_a_inv*R). popcnt != nullptr: also write the
-// Hamming count of m under `mask`.
+// ---------------------------------------------------------------------------
+// Pre/post-processing around the lazy exponentiation (canonical engine)
+// ---------------------------------------------------------------------------
+
+// m = L(u) * cR mod n for u = c^e mod n^2 (the exponentiation is done by
+// k_lazy_modexp): plain decrypt (e = phi, cR = inv*R), lookup decrypt (e = a,
+// cR = g_a_inv*R). popcnt != nullptr: also the odd-bit Hamming count of m.
 template <int L, int TPI>
-__global__ __launch_bounds__(BLOCK_THREADS) void k_decrypt(const ModDev* __restrict__ n2, const ModDev* __restrict__ nm,
-                                                           const u32* __restrict__ cR, const u32* __restrict__ C,
-                                                           u32* __restrict__ out, int count, int stride,
-                                                           const u32* __restrict__ sched, int nsched, int tentries,
-                                                           u32* __restrict__ table, int tstride,
-                                                           const u32* __restrict__ mask, u32* __restrict__ popcnt) {
+__global__ __launch_bounds__(BLOCK_THREADS) void k_decrypt_finish(const ModDev* __restrict__ nm,
+                                                                  const u32* __restrict__ cR,
+                                                                  const u32* __restrict__ U, u32* __restrict__ out,
+                                                                  int count, int stride, const u32* __restrict__ mask,
+                                                                  u32* __restrict__ popcnt) {
   extern __shared__ uint4 smem[];
   constexpr int K = L / TPI;
   const int item = group_item<TPI>();
   const int it = item < count ? item : count - 1;
   u32 x[K];
-  {
-    Mont<L, TPI> M;
-    mont_init<L, TPI>(M, n2, smem);
-    load_dist<L, TPI>(x, C, it, stride);
-    to_mont<L, TPI>(M, x, n2);
-    modexp_sched<L, TPI>(M, x, sched, nsched, tentries, table, item, tstride);
-    from_mont<L, TPI>(M, x);
-  }
+  load_dist<L, TPI>(x, U, it, stride);
   Mont<L, TPI> Mn;
   mont_init<L, TPI>(Mn, nm, smem);
   paillier_finish<L, TPI>(Mn, nm, x, cR);
@@ -311,91 +287,79 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_decrypt(const ModDev* __restr
   }
 }
 
-// c = (1 + m*n) * r^n mod n^2   (cu:522-546 form of scheme.py:63-83)
+// c = (1 + m*n) * y mod n^2 for y = r^n mod n^2 (cu:522-546 form of scheme.py:63-83)
 template <int L, int TPI>
-__global__ __launch_bounds__(BLOCK_THREADS) void k_encrypt(const PaillierDev* __restrict__ key,
-                                                           const u32* __restrict__ Mv, const u32* __restrict__ Rv,
-                                                           u32* __restrict__ out, int count, int stride,
-                                                           const u32* __restrict__ sched, int nsched, int tentries,
-                                                           u32* __restrict__ table, int tstride) {
+__global__ __launch_bounds__(BLOCK_THREADS) void k_encrypt_finish(const PaillierDev* __restrict__ key,
+                                                                  const u32* __restrict__ Mv,
+                                                                  const u32* __restrict__ Y, u32* __restrict__ out,
+                                                                  int count, int stride) {
   extern __shared__ uint4 smem[];
   constexpr int K = L / TPI;
   Mont<L, TPI> M;
   mont_init<L, TPI>(M, &key->n2, smem);
   const int item = group_item<TPI>();
   const int it = item < count ? item : count - 1;
-  u32 y[K];
-  load_dist<L, TPI>(y, Rv, it, stride);
-  to_mont<L, TPI>(M, y, &key->n2);
-  modexp_sched<L, TPI>(M, y, sched, nsched, tentries, table, item, tstride);  // y = r^n * R
-  u32 a[K], nr[K];
+  u32 a[K], y[K], c[K];
   load_dist<L, TPI>(a, Mv, it, stride);
-  load_const<L, TPI>(nr, key->nR.v);
-  M.mul_into(a, nr);                 // m*n (< n^2, exact)
+  load_dist<L, TPI>(y, Y, it, stride);
+  load_const<L, TPI>(c, key->nR.v);
+  M.mul_into(a, c);                  // m*n (< n^2, exact)
   (void)add_small<K, TPI>(a, 1u);    // 1 + m*n
-  M.mul_into(a, y);                  // (1 + m n) * r^n
+  load_const<L, TPI>(c, key->n2.R2);
+  M.mul_into(y, c);                  // y*R
+  M.mul_into(a, y);                  // (1 + m n) * y
   if (item < count) store_dist<L, TPI>(out, a, item, stride);
 }
 
-// CRT decryption, one half per launch row (blockIdx.y = 0: p, 1: q).
-// Input: ciphertexts of 2H limbs; output m_p (or m_q) of H limbs.
-//   x  = c mod p^2 in Montgomery form = MontMul(c_lo, R^2) + MontMul(c_hi, R^3)
-//   u  = x^(p-1) mod p^2;  m_p = L_p(u) * h_p mod p
+// CRT input folding, one half per launch row (blockIdx.y = 0: p^2, 1: q^2):
+// xout[half] = c mod p^2 (canonical) = MontMul(MontMul(c_lo, R^2) + MontMul(c_hi, R^3), 1)
 template <int H, int TPI>
-__global__ __launch_bounds__(BLOCK_THREADS) void k_crt_half(const PaillierDev* __restrict__ key,
-                                                            const u32* __restrict__ C, u32* __restrict__ out_half,
-                                                            int count, int stride, const u32* __restrict__ sched_p,
-                                                            int nsched_p, const u32* __restrict__ sched_q,
-                                                            int nsched_q, int tentries, u32* __restrict__ table,
-                                                            int tstride) {
+__global__ __launch_bounds__(BLOCK_THREADS) void k_crt_fold(const PaillierDev* __restrict__ key,
+                                                            const u32* __restrict__ C, u32* __restrict__ xout,
+                                                            int count, int stride) {
   extern __shared__ uint4 smem[];
   constexpr int K = H / TPI;
   const int half = (int)blockIdx.y;
   const ModDev* sq = half == 0 ? &key->p2 : &key->q2;
-  const ModDev* pr = half == 0 ? &key->p : &key->q;
-  const u32* hR = half == 0 ? key->hpR.v : key->hqR.v;
-  const u32* sched = half == 0 ? sched_p : sched_q;
-  const int nsched = half == 0 ? nsched_p : nsched_q;
   const int item = group_item<TPI>();
   const int it = item < count ? item : count - 1;
-  u32 x[K];
-  {
-    Mont<H, TPI> M;
-    mont_init<H, TPI>(M, sq, smem);
-    u32 lo[K], hi[K], r[K];
-    load_dist<H, TPI>(lo, C, it, stride);
-    load_dist<H, TPI>(hi, C + (size_t)H * stride, it, stride);
-    load_const<H, TPI>(r, sq->R2);
-    M.mul(x, lo, r);  // c_lo * R
-    load_const<H, TPI>(r, sq->R3);
-    M.mul_into(hi, r);  // c_hi * R^2
-    add_mod<K, TPI>(x, hi, M.N);
-    u32* tab = table + (size_t)half * tentries * H * tstride;
-    modexp_sched<H, TPI>(M, x, sched, nsched, tentries, tab, item, tstride);
-    from_mont<H, TPI>(M, x);
-  }
-  Mont<H, TPI> Mp;
-  mont_init<H, TPI>(Mp, pr, smem);
-  paillier_finish<H, TPI>(Mp, pr, x, hR);
-  if (item < count) store_dist<H, TPI>(out_half + (size_t)half * H * stride, x, item, stride);
+  Mont<H, TPI> M;
+  mont_init<H, TPI>(M, sq, smem);
+  u32 lo[K], hi[K], r[K], x[K];
+  load_dist<H, TPI>(lo, C, it, stride);
+  load_dist<H, TPI>(hi, C + (size_t)H * stride, it, stride);
+  load_const<H, TPI>(r, sq->R2);
+  M.mul(x, lo, r);  // c_lo * R
+  load_const<H, TPI>(r, sq->R3);
+  M.mul_into(hi, r);  // c_hi * R^2
+  add_mod<K, TPI>(x, hi, M.N);
+  from_mont<H, TPI>(M, x);
+  if (item < count) store_dist<H, TPI>(xout + (size_t)half * H * stride, x, item, stride);
 }
 
-// Garner recombination: m = m_q + q * ((m_p - m_q) * q^-1 mod p)
+// CRT finish: m_p = L_p(u_p) h_p mod p, m_q likewise, then Garner
+// m = m_q + q * ((m_p - m_q) * q^-1 mod p). U holds u_p, u_q ([2][H][stride]).
 template <int H, int TPI>
-__global__ __launch_bounds__(BLOCK_THREADS) void k_crt_combine(const PaillierDev* __restrict__ key,
-                                                               const u32* __restrict__ halves, u32* __restrict__ out,
-                                                               int count, int stride) {
+__global__ __launch_bounds__(BLOCK_THREADS) void k_crt_finish(const PaillierDev* __restrict__ key,
+                                                              const u32* __restrict__ U, u32* __restrict__ out,
+                                                              int count, int stride) {
   extern __shared__ uint4 smem[];
   constexpr int K = H / TPI;
   const int item = group_item<TPI>();
   const int it = item < count ? item : count - 1;
-  u32 mp[K], mq[K], qi[K];
-  load_dist<H, TPI>(mp, halves, it, stride);
-  load_dist<H, TPI>(mq, halves + (size_t)H * stride, it, stride);
-  load_const<H, TPI>(qi, key->qinvR.v);
+  u32 mp[K], mq[K];
+  load_dist<H, TPI>(mp, U, it, stride);
+  load_dist<H, TPI>(mq, U + (size_t)H * stride, it, stride);
   Mont<H, TPI> Mp;
   mont_init<H, TPI>(Mp, &key->p, smem);
-  u32 h1[K], h2[K];
+  paillier_finish<H, TPI>(Mp, &key->p, mp, key->hpR.v);
+  {
+    Mont<H, TPI> Mq;
+    mont_init<H, TPI>(Mq, &key->q, smem);
+    paillier_finish<H, TPI>(Mq, &key->q, mq, key->hqR.v);
+  }
+  u32 qi[K], h1[K], h2[K];
+  load_const<H, TPI>(qi, key->qinvR.v);
   Mp.mul(h1, mp, qi);  // m_p * q^-1 mod p
   Mp.mul(h2, mq, qi);  // m_q * q^-1 mod p   (m_q < R, so the product is reduced)
   sub_mod<K, TPI>(h1, h2, Mp.N);

@@ -1,0 +1,287 @@
+// Lazy-carry Montgomery exponentiation for gfx950 (the modexp hot loop).
+//
+// Numbers are held in LP positions of RB bits (RB = 28 or 27) inside 64-bit
+// accumulators. A limb product is < 2^(2 RB), and a position receives at most
+// 2 products per row for LP rows, so 2 * LP * 2^(2 RB) < 2^64 and no carry has
+// to be propagated inside the product: every limb product is exactly one
+// in-place v_mad_u64_u32 (64-bit accumulator as the addend), which issues at
+// the full VALU rate on CDNA4. Carries are resolved once per product.
+//
+// Montgomery product with R' = 2^(RB*LP) > 4N ("almost Montgomery", Walter
+// 2002): inputs and outputs stay below 2N, so no conditional subtraction is
+// needed inside the exponentiation; the result is made canonical once at the
+// end.
+//
+// A group of TPI lanes holds K = LP / TPI positions per lane. Each row i adds
+// a_i * b + m_i * N and shifts down one position. The K rows of a block are
+// fully unrolled and the accumulator registers are renamed instead of moved
+// (position p of row s lives in register (p + s) mod K), so the shift costs no
+// instructions inside a lane; across lanes the bottom position moves one lane
+// down with two DPP row_shl moves (lo/hi words). a_i comes from LDS (one
+// group-uniform read per row), m_i from rank 0 by DPP broadcast.
+#pragma once
+#include "mont.hpp"
+
+namespace chx {
+
+__device__ __forceinline__ u64 make64(u32 lo, u32 hi) { return ((u64)hi << 32) | lo; }
+
+template <int RB, int LP, int TPI, int L32>
+struct Lazy {
+  static constexpr int K = LP / TPI;
+  static constexpr int G = 64 / TPI;
+  static constexpr u32 MASK = (1u << RB) - 1u;
+  static constexpr int K32 = L32 / TPI;
+  static_assert(LP % TPI == 0, "LP must be a multiple of TPI");
+  static_assert(L32 % TPI == 0 && K32 % 4 == 0, "32-bit limbs per lane must be a multiple of 4");
+  static_assert(RB * LP >= 32 * L32 + 2, "R' must exceed 4N");
+  static_assert((double)2 * LP * (double)(1ull << (2 * RB)) < 18446744073709551616.0, "accumulator bound");
+  static constexpr int SLOTS = (LP > L32 ? LP : L32);  // u32 rows of the staging area
+  typedef Grp<TPI> grp;
+
+  u32* lds;     // this wave's staging area: SLOTS rows of G u32
+  u32 n0;       // -N^-1 mod 2^RB
+  u32 N[K];     // modulus positions (this lane's slice)
+  u64 r0mask;   // all ones in rank 0, zero elsewhere
+
+  __device__ __forceinline__ void init(u32* wave_area, const u32* __restrict__ npos, u32 n0inv) {
+    lds = wave_area;
+    n0 = n0inv;
+    const int r = grp::rank();
+#pragma unroll
+    for (int p = 0; p < K; ++p) N[p] = npos[r * K + p];
+    r0mask = grp::rank() == 0 ? ~0ull : 0ull;
+  }
+
+  __device__ __forceinline__ u64 shift_down64(u64 v) {
+    const u32 lo = grp::shift_down((u32)v);
+    const u32 hi = grp::shift_down((u32)(v >> 32));
+    return make64(lo, hi);
+  }
+
+  // stage this lane's positions as the streamed operand: row i = position i
+  __device__ __forceinline__ void stage(const u32 (&x)[K]) {
+    const int g = grp::gid(), r = grp::rank();
+#pragma unroll
+    for (int p = 0; p < K; ++p) lds[(r * K + p) * G + g] = x[p];
+  }
+
+  // carry-resolve accumulators into RB-bit positions (value < R' guaranteed)
+  __device__ __forceinline__ void normalize(u32 (&X)[K], const u64 (&V)[K]) {
+    u64 c = 0;
+#pragma unroll
+    for (int p = 0; p < K; ++p) {
+      const u64 t = V[p] + c;
+      X[p] = (u32)t & MASK;
+      c = t >> RB;
+    }
+    if constexpr (TPI > 1) {
+#pragma unroll 1
+      for (int round = 1; round < TPI; ++round) {
+        u64 cin = make64(grp::shift_up((u32)c), grp::shift_up((u32)(c >> 32)));
+        c = 0;
+#pragma unroll
+        for (int p = 0; p < K; ++p) {
+          const u64 t = (u64)X[p] + cin;
+          X[p] = (u32)t & MASK;
+          cin = t >> RB;
+        }
+        c = cin;
+      }
+    }
+  }
+
+  // b = a*b*R'^-1 mod N (below 2N), in place; a staged in LDS. The result
+  // overwrites b only after the last row has read it.
+  __device__ __forceinline__ void mul_staged(u32 (&b)[K]) {
+    const int g = grp::gid();
+    u64 V[K];
+#pragma unroll
+    for (int p = 0; p < K; ++p) V[p] = 0;
+#pragma unroll 1
+    for (int blk = 0; blk < TPI; ++blk) {
+      const u32* arow = lds + (blk * K) * G + g;
+#pragma unroll
+      for (int s = 0; s < K; ++s) {
+        const u32 a = arow[s * G];
+#pragma unroll
+        for (int p = 0; p < K; ++p) {
+          const int q = (p + s) % K;
+          V[q] = (u64)a * b[p] + V[q];
+        }
+        const u32 m = grp::bcast0(((u32)V[s] * n0) & MASK);
+#pragma unroll
+        for (int p = 0; p < K; ++p) {
+          const int q = (p + s) % K;
+          V[q] = (u64)m * N[p] + V[q];
+        }
+        const u64 v0 = V[s];
+        V[(s + 1) % K] += (v0 >> RB) & r0mask;
+        V[s] = shift_down64(v0);
+      }
+    }
+    normalize(b, V);
+  }
+
+  // x = x * y (both below 2N)
+  __device__ __forceinline__ void mul_into(u32 (&x)[K], const u32 (&y)[K]) {
+    stage(y);
+    wave_lds_sync();
+    mul_staged(x);
+    wave_lds_sync();
+  }
+
+  __device__ __forceinline__ void sqr(u32 (&x)[K]) {
+    stage(x);
+    wave_lds_sync();
+    mul_staged(x);
+    wave_lds_sync();
+  }
+
+  // 32-bit limbs (lane slice of K32, value < 2^(32 L32)) -> positions
+  __device__ __forceinline__ void from32(u32 (&x)[K], const u32 (&w)[K32]) {
+    const int g = grp::gid(), r = grp::rank();
+#pragma unroll
+    for (int k = 0; k < K32; ++k) lds[(r * K32 + k) * G + g] = w[k];
+    wave_lds_sync();
+#pragma unroll
+    for (int p = 0; p < K; ++p) {
+      const int bit = RB * (r * K + p);
+      const int j = bit >> 5, off = bit & 31;
+      const u32 lo = j < L32 ? lds[j * G + g] : 0u;
+      const u32 hi = j + 1 < L32 ? lds[(j + 1) * G + g] : 0u;
+      x[p] = (u32)(make64(lo, hi) >> off) & MASK;
+    }
+    wave_lds_sync();
+  }
+
+  // positions (normalized, value < 2^(32 L32)) -> 32-bit limbs (lane slice)
+  __device__ __forceinline__ void to32(u32 (&w)[K32], const u32 (&x)[K]) {
+    const int g = grp::gid(), r = grp::rank();
+    stage(x);
+    wave_lds_sync();
+#pragma unroll
+    for (int k = 0; k < K32; ++k) {
+      const int bit = 32 * (r * K32 + k);
+      const int q = bit / RB, off = bit % RB;
+      const u32 p0 = lds[q * G + g];
+      const u32 p1 = q + 1 < LP ? lds[(q + 1) * G + g] : 0u;
+      const u32 p2 = q + 2 < LP ? lds[(q + 2) * G + g] : 0u;
+      // 32 bits starting at `off` inside position q span at most three positions
+      u64 v = (u64)p0 >> off;
+      v |= (u64)p1 << (RB - off);
+      if (2 * RB - off < 64) v |= (u64)p2 << (2 * RB - off);
+      w[k] = (u32)v;
+    }
+    wave_lds_sync();
+  }
+};
+
+// Constants of one modulus in the lazy radix (device layout).
+constexpr int LPMAX = 152;
+struct LazyDev {
+  u32 n0;           // -N^-1 mod 2^RB
+  u32 rb, lp, l32;  // radix bits, positions, 32-bit limbs (checked by the host)
+  u32 N[LPMAX];     // modulus, positions
+  u32 R2[LPMAX];    // R'^2 mod N, positions
+  u32 N32[LMAX];    // modulus, 32-bit limbs (final canonicalisation)
+};
+
+// x^e mod N for a batch; e shared (host-built sliding-window schedule).
+// blockIdx.y selects one of two (modulus, schedule, input, output) sets so the
+// CRT halves run in one launch. Input: 32-bit limbs (any value < 2^(32 L32));
+// output: canonical 32-bit limbs. Window table: positions, per item, in global
+// scratch ([entry][position][item]).
+struct LazyJob {
+  const LazyDev* md;
+  const u32* sched;
+  int nsched;
+  const u32* in;
+  u32* out;
+  u32* table;
+};
+
+template <int RB, int LP, int TPI, int L32>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_modexp(LazyJob j0, LazyJob j1, int count, int stride,
+                                                               int tentries, int tstride) {
+  extern __shared__ uint4 smem4[];
+  typedef Lazy<RB, LP, TPI, L32> E;
+  constexpr int K = E::K, K32 = E::K32;
+  const LazyJob jb = blockIdx.y == 0 ? j0 : j1;
+  u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * (E::SLOTS * E::G);
+  E M;
+  M.init(area, jb.md->N, jb.md->n0);
+  const int item = group_item<TPI>();
+  const int it = item < count ? item : count - 1;
+  const int r = Grp<TPI>::rank();
+
+  u32 w[K32];
+#pragma unroll
+  for (int k = 0; k < K32; ++k) w[k] = jb.in[(size_t)(r * K32 + k) * stride + it];
+  u32 x[K];
+  M.from32(x, w);
+  {
+    u32 r2[K];
+#pragma unroll
+    for (int p = 0; p < K; ++p) r2[p] = jb.md->R2[r * K + p];
+    M.mul_into(x, r2);  // x * R' (below 2N)
+  }
+  // odd powers x, x^3, ..., x^(2^w - 1)
+  u32* tab = jb.table;
+  auto tstore = [&](int e, const u32 (&v)[K]) {
+#pragma unroll
+    for (int p = 0; p < K; ++p) tab[((size_t)e * LP + r * K + p) * tstride + item] = v[p];
+  };
+  auto tload = [&](int e, u32 (&v)[K]) {
+#pragma unroll
+    for (int p = 0; p < K; ++p) v[p] = tab[((size_t)e * LP + r * K + p) * tstride + item];
+  };
+  tstore(0, x);
+  if (tentries > 1) {
+    // x^2 stays staged in LDS while the odd powers are built in place
+    u32 cur[K];
+#pragma unroll
+    for (int p = 0; p < K; ++p) cur[p] = x[p];
+    M.sqr(cur);
+    M.stage(cur);
+#pragma unroll
+    for (int p = 0; p < K; ++p) cur[p] = x[p];
+    wave_lds_sync();
+    for (int e = 1; e < tentries; ++e) {
+      M.mul_staged(cur);
+      tstore(e, cur);
+    }
+    wave_lds_sync();
+  }
+  tload((int)jb.sched[0], x);
+  for (int s = 1; s < jb.nsched; ++s) {
+    const u32 op = jb.sched[s];
+    const int sq = (int)(op >> 16);
+    const int idx = (int)(op & 0xFFFFu);
+    for (int i = 0; i < sq; ++i) M.sqr(x);
+    if (idx) {
+      u32 y[K];
+      tload(idx - 1, y);
+      M.mul_into(x, y);
+    }
+  }
+  // out of Montgomery form: MontMul(x, 1) <= N, then canonical
+  {
+    u32 one[K];
+#pragma unroll
+    for (int p = 0; p < K; ++p) one[p] = 0;
+    one[0] = r == 0 ? 1u : 0u;
+    M.mul_into(x, one);
+  }
+  M.to32(w, x);
+  u32 n32[K32];
+#pragma unroll
+  for (int k = 0; k < K32; ++k) n32[k] = jb.md->N32[r * K32 + k];
+  cond_sub<K32, TPI>(w, 0u, n32);
+  if (item < count) {
+#pragma unroll
+    for (int k = 0; k < K32; ++k) jb.out[(size_t)(r * K32 + k) * stride + item] = w[k];
+  }
+}
+
+}  // namespace chx

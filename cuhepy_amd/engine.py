@@ -94,9 +94,10 @@ class Context:
         N.check(N.lib.chx_ctx_set_tuning(self.handle, op, limbs, tpi, window))
 
     @staticmethod
-    def supported_tpi(limbs: int) -> list[int]:
+    def supported_tpi(limbs: int, op: int = 1) -> list[int]:
+        """Lanes per bignum built for `op` at `limbs` (op 0 = exponentiation)."""
         buf = (ctypes.c_int * 16)()
-        n = N.lib.chx_supported_tpi(limbs, buf, 16)
+        n = N.lib.chx_supported_tpi_op(op, limbs, buf, 16)
         return list(buf[:n])
 
     def scratch_bytes(self) -> int:

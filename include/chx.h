@@ -69,8 +69,11 @@ void* chx_ctx_get_stream(chx_ctx* ctx);
 int chx_ctx_sync(chx_ctx* ctx);
 /* lanes per bignum (tpi) and window width for `op` at `limbs`; 0 = automatic */
 int chx_ctx_set_tuning(chx_ctx* ctx, int op, int limbs, int tpi, int window);
-/* lanes-per-bignum values built for `limbs` (returns count, fills up to cap) */
+/* lanes-per-bignum values built for `limbs` (returns count, fills up to cap);
+ * op CHX_OP_MODEXP lists the exponentiation engine's values, other ops the
+ * canonical engine's */
 int chx_supported_tpi(int limbs, int* tpis, int cap);
+int chx_supported_tpi_op(int op, int limbs, int* tpis, int cap);
 /* bytes of device scratch currently held by the context */
 size_t chx_ctx_scratch_bytes(chx_ctx* ctx);
 

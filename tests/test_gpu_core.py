@@ -62,7 +62,8 @@ def test_modexp_every_variant(ctx, limbs):
     from cuhepy_amd.engine import Modulus
     from cuhepy_amd.limbs import ints_to_limbs, limbs_to_ints
     rng = random.Random(200 + limbs)
-    for tpi in _variants(limbs):
+    from cuhepy_amd.engine import Context
+    for tpi in Context.supported_tpi(limbs, op=OP_MODEXP):
         for window in (0, 2):
             ctx.set_tuning(OP_MODEXP, limbs, tpi, window)
             n = _mod(rng, limbs)

@@ -54,19 +54,26 @@ def test_seeded_batches_every_variant(ctx, key_len):
     ms = [rng.randrange(n) for _ in range(count)]
     rs = [O.random_r(n, rng) for _ in range(count)]
     want_c = [O.encrypt(m, keys["pk"], r) for m, r in zip(ms, rs)]
-    for tpi in Context.supported_tpi(key.width):
-        ctx.set_tuning(OP_MODEXP, key.width, tpi, 0)
-        assert limbs_to_ints(key.encrypt(ms, rs)) == want_c, f"encrypt tpi={tpi}"
-        assert limbs_to_ints(key.decrypt(want_c)) == ms, f"decrypt tpi={tpi}"
+    exp_tpis = Context.supported_tpi(key.width, op=OP_MODEXP)
+    fin_tpis = Context.supported_tpi(key.width, op=OP_MULMOD)
+    for i in range(max(len(exp_tpis), len(fin_tpis))):
+        ctx.set_tuning(OP_MODEXP, key.width, exp_tpis[i % len(exp_tpis)], 0)
+        ctx.set_tuning(OP_MULMOD, key.width, fin_tpis[i % len(fin_tpis)], 0)
+        tag = f"exp tpi={exp_tpis[i % len(exp_tpis)]} finish tpi={fin_tpis[i % len(fin_tpis)]}"
+        assert limbs_to_ints(key.encrypt(ms, rs)) == want_c, f"encrypt {tag}"
+        assert limbs_to_ints(key.decrypt(want_c)) == ms, f"decrypt {tag}"
     ctx.set_tuning(OP_MODEXP, key.width, 0, 0)
+    ctx.set_tuning(OP_MULMOD, key.width, 0, 0)
     if not key.crt:  # 16-limb halves are not built: CRT is refused, plain decrypt covers the key
         with pytest.raises(ValueError):
             key.decrypt_crt(want_c)
         return
-    for tpi in Context.supported_tpi(key.half):
-        ctx.set_tuning(OP_MODEXP, key.half, tpi, 0)
-        ctx.set_tuning(OP_MULMOD, key.half, tpi, 0)
-        assert limbs_to_ints(key.decrypt_crt(want_c)) == ms, f"crt tpi={tpi}"
+    exp_tpis = Context.supported_tpi(key.half, op=OP_MODEXP)
+    fin_tpis = Context.supported_tpi(key.half, op=OP_MULMOD)
+    for i in range(max(len(exp_tpis), len(fin_tpis))):
+        ctx.set_tuning(OP_MODEXP, key.half, exp_tpis[i % len(exp_tpis)], 0)
+        ctx.set_tuning(OP_MULMOD, key.half, fin_tpis[i % len(fin_tpis)], 0)
+        assert limbs_to_ints(key.decrypt_crt(want_c)) == ms, f"crt variant {i}"
 
 
 def test_decode_hamming_fused(ctx, golden):
@@ -141,9 +148,11 @@ def test_lookup_seeded_every_variant(ctx):
     want = [O.lookup_encrypt(m, keys["pk"], table, keys["noise_table"], keys["message_chunks"], ix.tolist())
             for m, ix in zip(ms, idx)]
     digits = plaintexts_to_digits(ints_to_limbs(ms, key.width), key.chunks)
-    for tpi in Context.supported_tpi(key.width):
+    exp_tpis = Context.supported_tpi(key.width, op=OP_MODEXP)
+    for i, tpi in enumerate(Context.supported_tpi(key.width, op=OP_LOOKUP)):
         ctx.set_tuning(OP_LOOKUP, key.width, tpi, 0)
-        ctx.set_tuning(OP_MODEXP, key.width, tpi, 0)
+        ctx.set_tuning(OP_MULMOD, key.width, tpi, 0)
+        ctx.set_tuning(OP_MODEXP, key.width, exp_tpis[i % len(exp_tpis)], 0)
         got = limbs_to_ints(key.encrypt_digits(digits, idx))
         assert got == want, f"tpi={tpi}"
         assert limbs_to_ints(key.decrypt(got)) == ms, f"decrypt tpi={tpi}"
