@@ -90,6 +90,29 @@ def cpu_baseline(cp: dict, cs: list[int], want: list[int], cores: int) -> dict:
             "single_n": single_n}
 
 
+def lazy_tag(key) -> str:
+    """Template arguments of the exponentiation kernel the CRT halves use."""
+    return {32: "28, 38, 1, 32", 64: "28, 76, 4, 64", 128: "27, 152, 8, 128"}.get(key.half, "?")
+
+
+def committed_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the rocprofv3 counter summary
+    committed under profiles/ (FETCH_SIZE and WRITE_SIZE runs, each scaled by
+    what the counter reads for a device copy of known size)."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        text = open(path).read()
+        if "```json" not in text:
+            continue
+        try:
+            blob = json.loads(text.split("```json", 1)[1].split("```", 1)[0])
+        except ValueError:
+            continue
+        if blob.get("kernel") == kernel:
+            return blob.get("traffic_bytes"), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +127,8 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--tpi", type=int, default=0, help="force lanes per bignum (0 = automatic)")
     ap.add_argument("--window", type=int, default=0, help="force the window width (0 = automatic)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run a plain device copy of MIB MiB (counter calibration for profiles)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -219,6 +244,8 @@ def main() -> int:
     log(f"lazy_modexp avg {avg_launch_s * 1e3:.2f} ms over {half_n} launches -> {achieved:.3f} TMAC/s; "
         f"v_mad_u64_u32 probe {peak_probe:.3f} TMAC/s on {cus} CUs")
 
+    traffic, traffic_src = committed_traffic(f"k_lazy_modexp<{lazy_tag(key)}>")
+
     # plain per-call GPU form on the same batch (host buffers, synchronous calls)
     plain = None
     if not args.no_plain:
@@ -279,7 +306,8 @@ def main() -> int:
             "peak": round(peak_probe, 4),
             "unit": "TMAC/s (32x32->64 multiply-accumulates)",
             "frac": round(achieved / peak_probe, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "kernel": "k_lazy_modexp (CRT halves, one launch)",
             "launch_ms": round(avg_launch_s * 1e3, 3),
             "macs_per_launch": macs_per_launch,
@@ -300,6 +328,13 @@ def main() -> int:
                       f"{cpu['single_n']} of them",
             "single_thread": round(cpu["single"], 1),
         }
+    if args.copy_calibration:
+        src = torch.ones(args.copy_calibration * (1 << 20) // 4, dtype=torch.int32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 3 device copies of {args.copy_calibration} MiB")
     print(json.dumps(line), flush=True)
     return 0
 

@@ -53,6 +53,8 @@ SIGNATURES = {
     "chx_mulmod_bcast_dev": (c_int, [vp, vp, vp, u32p, vp, c_size_t, c_size_t]),
     "chx_modexp": (c_int, [vp, vp, u32p, c_size_t, u32p, c_int, u32p]),
     "chx_modexp_dev": (c_int, [vp, vp, vp, vp, c_size_t, c_size_t, u32p, c_int]),
+    "chx_modexp_each": (c_int, [vp, vp, u32p, c_size_t, u32p, c_int, u32p]),
+    "chx_modexp_each_dev": (c_int, [vp, vp, vp, vp, c_int, c_int, vp, c_size_t, c_size_t]),
     "chx_paillier_key_create": (c_int, [vp, u32p, c_int, u32p, u32p, u32p, u32p, c_int, u32p, u32p, u32p,
                                         POINTER(vp)]),
     "chx_paillier_key_destroy": (c_int, [vp]),

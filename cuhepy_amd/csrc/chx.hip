@@ -422,7 +422,9 @@ LazySet make_lazy_set(const host::MontConsts& c, hipStream_t st) {
     to_positions(c.N, rb, lp, ld->N);
     BN one(c.L, 0u);
     one[0] = 1;
-    BN r2 = host::bn_shl_mod(one, 2 * rb * lp, c.N);  // R'^2 mod N
+    BN r1 = host::bn_shl_mod(one, rb * lp, c.N);  // R' mod N
+    BN r2 = host::bn_shl_mod(r1, rb * lp, c.N);   // R'^2 mod N
+    to_positions(r1, rb, lp, ld->R1);
     to_positions(r2, rb, lp, ld->R2);
     for (int k = 0; k < c.L; ++k) ld->N32[k] = c.N[k];
     ls.d[i] = dev_upload(ld.get(), 1, st);
@@ -436,11 +438,11 @@ void free_lazy_set(LazySet& ls) {
 }
 
 // pick the lazy configuration for a batch (tuning override, else fill the chip)
-int pick_lazy(chx_ctx* ctx, int L32, size_t count) {
+int pick_lazy(chx_ctx* ctx, int L32, size_t count, int op = CHX_OP_MODEXP) {
   const int li = lidx(L32);
   if (li < 0) fail(CHX_EUNSUPPORTED, "width of %d limbs is not built", L32);
   const auto& cfgs = lazy_cfgs(L32);
-  const int forced = ctx->tune_tpi[CHX_OP_MODEXP][li];
+  const int forced = ctx->tune_tpi[op][li];
   if (forced) {
     for (size_t i = 0; i < cfgs.size(); ++i)
       if (cfgs[i].tpi == forced) return (int)i;
@@ -519,6 +521,28 @@ void launch_modexp(chx_ctx* ctx, const chx_mod* m, const uint32_t* base, uint32_
                    size_t stride, const ExpSched& s) {
   const ExpSched* sp = &s;
   run_lazy(ctx, m->hc.L, &m->lz, &sp, &base, &out, 1, count, stride);
+}
+
+// per-item exponents (fixed window); exps interleaved, e_limbs limbs per item
+void launch_modexp_each(chx_ctx* ctx, const LazySet& ls, const uint32_t* base, const uint32_t* exps, int e_limbs,
+                        int ebits, uint32_t* out, size_t count, size_t stride) {
+  const int L = ls.L32;
+  const int ci = pick_lazy(ctx, L, count, CHX_OP_RAGGED);
+  const LazyCfg cfg = lazy_cfgs(L)[ci];
+  const int li = lidx(L);
+  int w = ctx->tune_window[CHX_OP_RAGGED][li];
+  if (!w) w = ebits >= 512 ? 5 : ebits >= 64 ? 4 : 2;
+  const int nwin = std::max(1, (ebits + w - 1) / w);
+  const size_t tstride = round_up(count, BLOCK_THREADS);
+  ctx->slot[S_TAB].ensure(((size_t)1 << w) * cfg.lp * tstride * sizeof(uint32_t));
+  record_start(ctx, "lazy_modexp_each");
+  CHX_DISPATCH_LAZY(L, ci, {
+    hipLaunchKernelGGL((k_lazy_modexp_each<RB, LP, TPI, L32>), grid_for(count, TPI), dim3(BLOCK_THREADS),
+                       lazy_lds_bytes(LP, L32, TPI), ctx->stream, ls.d[ci], base, exps, e_limbs, nwin, w, out,
+                       (int)count, (int)stride, ctx->slot[S_TAB].as<uint32_t>(), (int)tstride);
+  });
+  record_stop(ctx);
+  check_launch("k_lazy_modexp_each");
 }
 
 // m = L(c^e mod n^2) * cR mod n (+ optional Hamming popcount): lazy
@@ -608,16 +632,42 @@ const ExpSched& get_sched_l(chx_lkey* k, int window) {
 void launch_lookup_encrypt(chx_lkey* k, const uint8_t* digits, const uint8_t* nidx, int nmult, uint32_t* c,
                            size_t count, size_t stride) {
   chx_ctx* ctx = k->ctx;
+  if (!k->d_gtab || !k->d_ntab) fail(CHX_EINVAL, "lookup key was created without tables (decrypt-only)");
   const int L = k->W;
-  const int tpi = pick_tpi(ctx, CHX_OP_LOOKUP, L, count);
+  const int ci = pick_lazy(ctx, L, count, CHX_OP_LOOKUP);
   record_start(ctx, "lookup_encrypt");
-  CHX_DISPATCH(L, tpi, {
-    hipLaunchKernelGGL((k_lookup_encrypt<L, TPI>), grid_for(count, TPI), dim3(BLOCK_THREADS),
-                       lds_bytes(L, TPI), ctx->stream, k->d_n2, k->d_gtab, k->d_ntab, digits, k->chunks, nidx,
-                       nmult, k->noise, c, (int)count, (int)stride);
+  CHX_DISPATCH_LAZY(L, ci, {
+    hipLaunchKernelGGL((k_lazy_lookup_encrypt<RB, LP, TPI, L32>), grid_for(count, TPI), dim3(BLOCK_THREADS),
+                       lazy_lds_bytes(LP, L32, TPI), ctx->stream, k->lz_n2.d[ci], k->d_gtab, k->d_ntab, digits,
+                       k->chunks, nidx, nmult, k->noise, c, (int)count, (int)stride);
   });
   record_stop(ctx);
-  check_launch("k_lookup_encrypt");
+  check_launch("k_lazy_lookup_encrypt");
+}
+
+// convert an AoS table of plain residues (entries x L32 limbs, host) into
+// lazy Montgomery positions on the device (entries x LP); returns the buffer
+uint32_t* lazy_table_upload(chx_ctx* ctx, const LazySet& ls, const uint32_t* host, size_t entries) {
+  const int L = ls.L32;
+  const LazyCfg cfg = lazy_cfgs(L)[0];
+  const int ci = 0;
+  uint32_t* src = nullptr;
+  uint32_t* dst = nullptr;
+  hip_check(hipMalloc(&src, entries * L * sizeof(uint32_t)), "hipMalloc");
+  hipError_t e = hipMalloc(&dst, entries * cfg.lp * sizeof(uint32_t));
+  if (e != hipSuccess) {
+    (void)hipFree(src);
+    fail(CHX_ENOMEM, "hipMalloc: %s", hipGetErrorString(e));
+  }
+  hip_check(hipMemcpyAsync(src, host, entries * L * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream), "H2D");
+  CHX_DISPATCH_LAZY(L, ci, {
+    hipLaunchKernelGGL((k_lazy_table<RB, LP, TPI, L32>), grid_for(entries, TPI), dim3(BLOCK_THREADS),
+                       lazy_lds_bytes(LP, L32, TPI), ctx->stream, ls.d[ci], src, dst, (int)entries);
+  });
+  check_launch("k_lazy_table");
+  hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  (void)hipFree(src);
+  return dst;
 }
 
 template <class F>
@@ -699,7 +749,7 @@ int chx_ctx_set_tuning(chx_ctx* ctx, int op, int limbs, int tpi, int window) {
     if (op < 0 || op > 3 || li < 0) fail(CHX_EINVAL, "bad op %d or limbs %d", op, limbs);
     if (tpi) {
       bool ok = false;
-      if (op == CHX_OP_MODEXP) {
+      if (op == CHX_OP_MODEXP || op == CHX_OP_LOOKUP || op == CHX_OP_RAGGED) {
         for (const auto& c : lazy_cfgs(limbs)) ok = ok || c.tpi == tpi;
       } else {
         const auto& opts = built_tpi(limbs);
@@ -720,7 +770,7 @@ int chx_supported_tpi(int limbs, int* tpis, int cap) {
 }
 
 int chx_supported_tpi_op(int op, int limbs, int* tpis, int cap) {
-  if (op != CHX_OP_MODEXP) return chx_supported_tpi(limbs, tpis, cap);
+  if (op == CHX_OP_MULMOD) return chx_supported_tpi(limbs, tpis, cap);
   const auto& v = lazy_cfgs(limbs);
   for (int i = 0; i < (int)v.size() && i < cap; ++i) tpis[i] = v[i].tpi;
   return (int)v.size();
@@ -863,6 +913,37 @@ int chx_modexp_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* base, uint32_
     s.d = ctx->slot[S_AUX].as<uint32_t>();
     launch_modexp(ctx, m, base, out, count, stride, s);
     s.d = nullptr;
+  });
+}
+
+int chx_modexp_each(chx_ctx* ctx, const chx_mod* m, const uint32_t* base, size_t count, const uint32_t* exps,
+                    int e_limbs, uint32_t* out) {
+  return with_ctx(ctx, [&] {
+    if (!m || !base || !out || !exps || e_limbs <= 0) fail(CHX_EINVAL, "null argument");
+    if (count == 0) return;
+    const int L = m->hc.L;
+    check_below(base, count, L, m->hc.N, "base");
+    int ebits = 0;
+    for (size_t i = 0; i < count * (size_t)e_limbs; ++i)
+      if (exps[i]) ebits = std::max(ebits, 32 * (int)(i % e_limbs) + 32 - __builtin_clz(exps[i]));
+    const size_t stride = round_up(count, 64);
+    uint32_t* db = upload_il(ctx, S_IN0, base, count, L, stride);
+    uint32_t* de = upload_il(ctx, S_IN1, exps, count, e_limbs, stride);
+    ctx->slot[S_OUT].ensure(stride * L * 4);
+    launch_modexp_each(ctx, m->lz, db, de, e_limbs, std::max(ebits, 1), ctx->slot[S_OUT].as<uint32_t>(), count,
+                       stride);
+    download_il(ctx, ctx->slot[S_OUT].as<uint32_t>(), out, count, L, stride);
+  });
+}
+
+int chx_modexp_each_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* base, const uint32_t* exps, int e_limbs,
+                        int e_bits, uint32_t* out, size_t count, size_t stride) {
+  return with_ctx(ctx, [&] {
+    if (!m || !base || !out || !exps || e_limbs <= 0 || e_bits <= 0 || e_bits > 32 * e_limbs)
+      fail(CHX_EINVAL, "bad argument");
+    if (count == 0) return;
+    if (stride < round_up(count, 64)) fail(CHX_EINVAL, "stride %zu below count rounded to 64", stride);
+    launch_modexp_each(ctx, m->lz, base, exps, e_limbs, e_bits, out, count, stride);
   });
 }
 
@@ -1070,8 +1151,10 @@ int chx_lookup_key_create(chx_ctx* ctx, const uint32_t* n, int n_limbs, const ui
                           const uint32_t* g_a_inv, const uint32_t* g_table, int chunks,
                           const uint32_t* noise_table, int noise_entries, chx_lkey** out) {
   return with_ctx(ctx, [&] {
-    if (!n || !a || !g_a_inv || !g_table || !noise_table || !out) fail(CHX_EINVAL, "null argument");
-    if (n_limbs <= 0 || n_limbs > 64 || chunks <= 0 || noise_entries <= 0 || noise_entries > 256)
+    if (!n || !a || !g_a_inv || !out) fail(CHX_EINVAL, "null argument");
+    const bool tables = chunks > 0;
+    if (n_limbs <= 0 || n_limbs > 64 || chunks < 0 || (tables && (!g_table || !noise_table)) ||
+        (tables && (noise_entries <= 0 || noise_entries > 256)))
       fail(CHX_EINVAL, "bad sizes");
     BN nb = host::bn_from(n, n_limbs, n_limbs);
     const int W = pick_width(2 * host::bn_bits(nb));
@@ -1093,21 +1176,11 @@ int chx_lookup_key_create(chx_ctx* ctx, const uint32_t* n, int n_limbs, const ui
     k->lz_n2 = make_lazy_set(k->n2, ctx->stream);
     BN gar = host::to_mont_host(host::bn_from(g_a_inv, n_limbs, W), k->n);
     k->d_gar = dev_upload(gar.data(), W, ctx->stream);
-    // tables to Montgomery form (host shift-and-reduce; once per key)
-    const size_t gentries = (size_t)chunks * 256;
-    std::vector<uint32_t> gm(gentries * W), nm((size_t)noise_entries * W);
-    for (size_t e = 0; e < gentries; ++e) {
-      BN v = host::bn_from(g_table + e * W, W, W);
-      BN vm = host::to_mont_host(v, k->n2);
-      std::copy(vm.begin(), vm.end(), gm.begin() + e * W);
+    if (tables) {
+      // tables to the lazy Montgomery domain on the device (once per key)
+      k->d_gtab = lazy_table_upload(ctx, k->lz_n2, g_table, (size_t)chunks * 256);
+      k->d_ntab = lazy_table_upload(ctx, k->lz_n2, noise_table, (size_t)noise_entries);
     }
-    for (int e = 0; e < noise_entries; ++e) {
-      BN v = host::bn_from(noise_table + (size_t)e * W, W, W);
-      BN vm = host::to_mont_host(v, k->n2);
-      std::copy(vm.begin(), vm.end(), nm.begin() + (size_t)e * W);
-    }
-    k->d_gtab = dev_upload(gm.data(), gm.size(), ctx->stream);
-    k->d_ntab = dev_upload(nm.data(), nm.size(), ctx->stream);
     *out = k.release();
   });
 }

@@ -372,52 +372,6 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_crt_finish(const PaillierDev*
   if (item < count) store_dist<H, TPI>(out, m, item, stride);
 }
 
-// Lookup-table encryption (lookup.py:213-252, lcu:839-948):
-//   c = prod_i g_table[i][d_i] * prod_k noise[idx_k] mod n^2
-// Tables are in Montgomery form (entry*R mod n^2), AoS: entry e at [e*L].
-// digits: [count][chunks] bytes, noise_idx: [count][14] bytes.
-// Rows where every item of the wave has digit 0 are skipped (g_table[i][0] = 1);
-// otherwise items with digit 0 multiply by the j = 0 entry (R mod n^2), so the
-// wave stays convergent and the product is unchanged.
-template <int L, int TPI>
-__global__ __launch_bounds__(BLOCK_THREADS) void k_lookup_encrypt(const ModDev* __restrict__ md,
-                                                                  const u32* __restrict__ gtab,
-                                                                  const u32* __restrict__ ntab,
-                                                                  const unsigned char* __restrict__ digits,
-                                                                  int chunks, const unsigned char* __restrict__ nidx,
-                                                                  int nmult, int nnoise, u32* __restrict__ out,
-                                                                  int count, int stride) {
-  extern __shared__ uint4 smem[];
-  constexpr int K = L / TPI;
-  Mont<L, TPI> M;
-  mont_init<L, TPI>(M, md, smem);
-  const int item = group_item<TPI>();
-  const int it = item < count ? item : count - 1;
-  const int r = Grp<TPI>::rank();
-  u32 acc[K];
-  load_const<L, TPI>(acc, md->R1);
-  const unsigned char* dg = digits + (size_t)it * chunks;
-  for (int i = 0; i < chunks; ++i) {
-    const int d = dg[i];
-    if (__ballot(d != 0) == 0ull) continue;  // uniform skip
-    u32 y[K];
-    const u32* e = gtab + ((size_t)i * 256 + d) * L + r * K;
-#pragma unroll
-    for (int k = 0; k < K; ++k) y[k] = e[k];
-    M.mul_into(acc, y);
-  }
-  const unsigned char* ni = nidx + (size_t)it * nmult;
-  for (int j = 0; j < nmult; ++j) {
-    u32 y[K];
-    const u32* e = ntab + (size_t)(ni[j] % nnoise) * L + r * K;  // index mod table size, as lcu:933-934
-#pragma unroll
-    for (int k = 0; k < K; ++k) y[k] = e[k];
-    M.mul_into(acc, y);
-  }
-  from_mont<L, TPI>(M, acc);
-  if (item < count) store_dist<L, TPI>(out, acc, item, stride);
-}
-
 // ---------------------------------------------------------------------------
 // Layout conversion: AoS [count][L] <-> interleaved [L][stride]
 // ---------------------------------------------------------------------------

@@ -184,6 +184,7 @@ struct LazyDev {
   u32 rb, lp, l32;  // radix bits, positions, 32-bit limbs (checked by the host)
   u32 N[LPMAX];     // modulus, positions
   u32 R2[LPMAX];    // R'^2 mod N, positions
+  u32 R1[LPMAX];    // R' mod N, positions (Montgomery form of 1)
   u32 N32[LMAX];    // modulus, 32-bit limbs (final canonicalisation)
 };
 
@@ -281,6 +282,191 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_modexp(LazyJob j0, LazyJ
   if (item < count) {
 #pragma unroll
     for (int k = 0; k < K32; ++k) jb.out[(size_t)(r * K32 + k) * stride + item] = w[k];
+  }
+}
+
+// Lookup tables to the lazy Montgomery domain: out[e] = in[e] * R' mod N
+// (below 2N) as positions; in: AoS 32-bit limbs (L32 per entry), out: AoS
+// positions (LP per entry). One group per entry.
+template <int RB, int LP, int TPI, int L32>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_table(const LazyDev* __restrict__ md,
+                                                              const u32* __restrict__ in, u32* __restrict__ out,
+                                                              int entries) {
+  extern __shared__ uint4 smem4[];
+  typedef Lazy<RB, LP, TPI, L32> E;
+  constexpr int K = E::K, K32 = E::K32;
+  u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * (E::SLOTS * E::G);
+  E M;
+  M.init(area, md->N, md->n0);
+  const int item = group_item<TPI>();
+  const int it = item < entries ? item : entries - 1;
+  const int r = Grp<TPI>::rank();
+  u32 w[K32], x[K], r2[K];
+#pragma unroll
+  for (int k = 0; k < K32; ++k) w[k] = in[(size_t)it * L32 + r * K32 + k];
+  M.from32(x, w);
+#pragma unroll
+  for (int p = 0; p < K; ++p) r2[p] = md->R2[r * K + p];
+  M.mul_into(x, r2);
+  if (item < entries) {
+#pragma unroll
+    for (int p = 0; p < K; ++p) out[(size_t)item * LP + r * K + p] = x[p];
+  }
+}
+
+// Lookup-table encryption (lookup.py:213-252, lcu:839-948) in the lazy domain:
+//   c = prod_i g_table[i][d_i] * prod_k noise[idx_k] mod n^2
+// Tables are AoS positions in lazy Montgomery form. Rows where every item of
+// the wave has digit 0 are skipped (g_table[i][0] = 1); other items with
+// digit 0 multiply by their row's j = 0 entry (the Montgomery one), so the
+// wave stays convergent and the product is unchanged.
+template <int RB, int LP, int TPI, int L32>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_lookup_encrypt(const LazyDev* __restrict__ md,
+                                                                       const u32* __restrict__ gtab,
+                                                                       const u32* __restrict__ ntab,
+                                                                       const unsigned char* __restrict__ digits,
+                                                                       int chunks,
+                                                                       const unsigned char* __restrict__ nidx,
+                                                                       int nmult, int nnoise, u32* __restrict__ out,
+                                                                       int count, int stride) {
+  extern __shared__ uint4 smem4[];
+  typedef Lazy<RB, LP, TPI, L32> E;
+  constexpr int K = E::K, K32 = E::K32;
+  u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * (E::SLOTS * E::G);
+  E M;
+  M.init(area, md->N, md->n0);
+  const int item = group_item<TPI>();
+  const int it = item < count ? item : count - 1;
+  const int r = Grp<TPI>::rank();
+  u32 acc[K];
+#pragma unroll
+  for (int p = 0; p < K; ++p) acc[p] = md->R1[r * K + p];
+  const unsigned char* dg = digits + (size_t)it * chunks;
+  for (int i = 0; i < chunks; ++i) {
+    const int d = dg[i];
+    if (__ballot(d != 0) == 0ull) continue;  // uniform skip
+    u32 y[K];
+    const u32* e = gtab + ((size_t)i * 256 + d) * LP + r * K;
+#pragma unroll
+    for (int p = 0; p < K; ++p) y[p] = e[p];
+    M.mul_into(acc, y);
+  }
+  const unsigned char* ni = nidx + (size_t)it * nmult;
+  for (int j = 0; j < nmult; ++j) {
+    u32 y[K];
+    const u32* e = ntab + (size_t)(ni[j] % nnoise) * LP + r * K;  // index mod table size, as lcu:933-934
+#pragma unroll
+    for (int p = 0; p < K; ++p) y[p] = e[p];
+    M.mul_into(acc, y);
+  }
+  {
+    u32 one[K];
+#pragma unroll
+    for (int p = 0; p < K; ++p) one[p] = 0;
+    one[0] = r == 0 ? 1u : 0u;
+    M.mul_into(acc, one);
+  }
+  u32 w[K32];
+  M.to32(w, acc);
+  u32 n32[K32];
+#pragma unroll
+  for (int k = 0; k < K32; ++k) n32[k] = md->N32[r * K32 + k];
+  cond_sub<K32, TPI>(w, 0u, n32);
+  if (item < count) {
+#pragma unroll
+    for (int k = 0; k < K32; ++k) out[(size_t)(r * K32 + k) * stride + item] = w[k];
+  }
+}
+
+// base_i^(e_i) mod N with a per-item exponent (fixed-window method, HAC
+// Alg. 14.82): every item runs the same sequence -- `w` squarings then one
+// table multiply per window, `nwin` windows from the top -- so a wave never
+// diverges; the table holds x^0 .. x^(2^w - 1) per item. Exponents: e_limbs
+// 32-bit limbs per item, interleaved like the other batch buffers; nwin
+// covers the widest exponent of the batch.
+template <int RB, int LP, int TPI, int L32>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_modexp_each(const LazyDev* __restrict__ md,
+                                                                    const u32* __restrict__ in,
+                                                                    const u32* __restrict__ exps, int e_limbs,
+                                                                    int nwin, int w, u32* __restrict__ out,
+                                                                    int count, int stride, u32* __restrict__ table,
+                                                                    int tstride) {
+  extern __shared__ uint4 smem4[];
+  typedef Lazy<RB, LP, TPI, L32> E;
+  constexpr int K = E::K, K32 = E::K32;
+  u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * (E::SLOTS * E::G);
+  E M;
+  M.init(area, md->N, md->n0);
+  const int item = group_item<TPI>();
+  const int it = item < count ? item : count - 1;
+  const int r = Grp<TPI>::rank();
+  auto tstore = [&](int e, const u32 (&v)[K]) {
+#pragma unroll
+    for (int p = 0; p < K; ++p) table[((size_t)e * LP + r * K + p) * tstride + item] = v[p];
+  };
+  auto tload = [&](int e, u32 (&v)[K]) {
+#pragma unroll
+    for (int p = 0; p < K; ++p) v[p] = table[((size_t)e * LP + r * K + p) * tstride + item];
+  };
+  auto digit = [&](int win) -> int {
+    const int bit = win * w;
+    const int j = bit >> 5, off = bit & 31;
+    const u32 lo = j < e_limbs ? exps[(size_t)j * stride + it] : 0u;
+    const u32 hi = j + 1 < e_limbs ? exps[(size_t)(j + 1) * stride + it] : 0u;
+    return (int)((u32)(make64(lo, hi) >> off) & ((1u << w) - 1u));
+  };
+  u32 x[K];
+  {
+    u32 wv[K32];
+#pragma unroll
+    for (int k = 0; k < K32; ++k) wv[k] = in[(size_t)(r * K32 + k) * stride + it];
+    M.from32(x, wv);
+    u32 r2[K];
+#pragma unroll
+    for (int p = 0; p < K; ++p) r2[p] = md->R2[r * K + p];
+    M.mul_into(x, r2);
+  }
+  {
+    u32 cur[K];
+#pragma unroll
+    for (int p = 0; p < K; ++p) cur[p] = md->R1[r * K + p];
+    tstore(0, cur);
+    tstore(1, x);
+    M.stage(x);
+#pragma unroll
+    for (int p = 0; p < K; ++p) cur[p] = x[p];
+    wave_lds_sync();
+    for (int e = 2; e < (1 << w); ++e) {
+      M.mul_staged(cur);
+      tstore(e, cur);
+    }
+    wave_lds_sync();
+  }
+  tload(digit(nwin - 1), x);
+  for (int win = nwin - 2; win >= 0; --win) {
+    for (int i = 0; i < w; ++i) M.sqr(x);
+    const int d = digit(win);
+    if (__ballot(d != 0) == 0ull) continue;  // whole wave multiplies by one: skip
+    u32 y[K];
+    tload(d, y);
+    M.mul_into(x, y);
+  }
+  {
+    u32 one[K];
+#pragma unroll
+    for (int p = 0; p < K; ++p) one[p] = 0;
+    one[0] = r == 0 ? 1u : 0u;
+    M.mul_into(x, one);
+  }
+  u32 wv[K32];
+  M.to32(wv, x);
+  u32 n32[K32];
+#pragma unroll
+  for (int k = 0; k < K32; ++k) n32[k] = md->N32[r * K32 + k];
+  cond_sub<K32, TPI>(wv, 0u, n32);
+  if (item < count) {
+#pragma unroll
+    for (int k = 0; k < K32; ++k) out[(size_t)(r * K32 + k) * stride + item] = wv[k];
   }
 }
 

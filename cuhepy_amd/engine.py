@@ -174,6 +174,25 @@ class Modulus:
                                      _p32(out)))
         return out
 
+    def modexp_each(self, bases, exps) -> np.ndarray:
+        """bases[i] ** exps[i] mod n (per-item exponents)."""
+        B = _batch(bases, self.limbs)
+        ex = [int(x) for x in exps]
+        if len(ex) != len(B):
+            raise ValueError("bases and exponents must have the same length")
+        if any(x < 0 for x in ex):
+            raise ValueError("negative exponent")
+        el = max([limb_count(x) for x in ex] + [1])
+        E = ints_to_limbs(ex, el)
+        out = np.empty_like(B)
+        if len(B):
+            N.check(N.lib.chx_modexp_each(self.ctx.handle, self.handle, _p32(B), len(B), _p32(E), el, _p32(out)))
+        return out
+
+    def modexp_each_dev(self, base, exps, e_limbs: int, e_bits: int, out, count: int, stride: int) -> None:
+        N.check(N.lib.chx_modexp_each_dev(self.ctx.handle, self.handle, _dptr(base), _dptr(exps), e_limbs, e_bits,
+                                          _dptr(out), count, stride))
+
     # device-resident forms
     def mulmod_dev(self, a, b, out, count: int, stride: int) -> None:
         N.check(N.lib.chx_mulmod_dev(self.ctx.handle, self.handle, _dptr(a), _dptr(b), _dptr(out), count, stride))
@@ -288,7 +307,10 @@ class PaillierKey:
 
 
 class LookupKey:
-    """Device-resident Paillier-lookup key: tables in HBM (Montgomery form)."""
+    """Device-resident Paillier-lookup key: tables in HBM (Montgomery form).
+
+    Empty tables give a decrypt-only key.
+    """
 
     def __init__(self, ctx: Context, n: int, a: int, g_a_inv: int, g_table: Sequence[Sequence[int]],
                  noise_table: Sequence[int]):
@@ -297,18 +319,27 @@ class LookupKey:
         self.n2 = n * n
         nl = limb_count(n)
         self.width = width_for_bits(self.n2.bit_length())
-        self.chunks = len(g_table)
-        flat = [v for row in g_table for v in row]
-        if any(len(row) != 256 for row in g_table):
-            raise ValueError("every g_table row must have 256 entries")
-        G = ints_to_limbs(flat, self.width)
-        NT = ints_to_limbs(list(noise_table), self.width)
+        if isinstance(g_table, np.ndarray):  # (chunks, 256, width) limb array
+            if g_table.ndim != 3 or g_table.shape[1:] != (256, self.width):
+                raise ValueError(f"g_table array must be (chunks, 256, {self.width})")
+            self.chunks = g_table.shape[0]
+            G = np.ascontiguousarray(g_table.reshape(-1, self.width), dtype=np.uint32)
+        else:
+            self.chunks = len(g_table)
+            if any(len(row) != 256 for row in g_table):
+                raise ValueError("every g_table row must have 256 entries")
+            G = ints_to_limbs([v for row in g_table for v in row], self.width)
+        NT = _batch(noise_table, self.width) if isinstance(noise_table, np.ndarray) else \
+            ints_to_limbs(list(noise_table), self.width)
+        if (self.chunks == 0) != (len(NT) == 0):
+            raise ValueError("give both tables or neither (decrypt-only key)")
         nn = ints_to_limbs([n], nl)
         A = ints_to_limbs([a], limb_count(a))
         GA = ints_to_limbs([g_a_inv], nl)
         h = ctypes.c_void_p()
-        N.check(N.lib.chx_lookup_key_create(ctx.handle, _p32(nn), nl, _p32(A), A.shape[1], _p32(GA), _p32(G),
-                                            self.chunks, _p32(NT), len(NT), ctypes.byref(h)))
+        N.check(N.lib.chx_lookup_key_create(ctx.handle, _p32(nn), nl, _p32(A), A.shape[1], _p32(GA),
+                                            _p32(G) if self.chunks else None, self.chunks,
+                                            _p32(NT) if len(NT) else None, len(NT), ctypes.byref(h)))
         self.handle = h
         self.noise_entries = len(NT)
 

@@ -101,6 +101,15 @@ int chx_modexp(chx_ctx* ctx, const chx_mod* m, const uint32_t* base, size_t coun
 int chx_modexp_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* base, uint32_t* out, size_t count,
                    size_t stride, const uint32_t* e, int e_limbs);
 
+/* out[i] = base[i]^e[i] mod n with a per-item exponent (fixed window); exps:
+ * count x e_limbs (AoS). No reference counterpart (config 4's ragged modexp;
+ * the lookup noise table g_n^r_k, lookup.py:113-117, lcu:592-626).
+ * _dev: exps interleaved; e_bits = bit length of the widest exponent. */
+int chx_modexp_each(chx_ctx* ctx, const chx_mod* m, const uint32_t* base, size_t count, const uint32_t* exps,
+                    int e_limbs, uint32_t* out);
+int chx_modexp_each_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* base, const uint32_t* exps, int e_limbs,
+                        int e_bits, uint32_t* out, size_t count, size_t stride);
+
 /* ---- Paillier (g = n + 1) ----
  * n_limbs: limbs of n (<= 64); the working width is the smallest of 32/64/128
  * limbs that holds n^2. inv = phi^-1 mod n. When p != NULL, CRT decryption is
@@ -141,7 +150,8 @@ int chx_paillier_decode_hamming(chx_ctx* ctx, const chx_pkey* k, const uint32_t*
  * (a_limbs), g_a_inv < n. Tables are plain residues mod n^2, `width` limbs per
  * entry: g_table[chunks][256] (lookup.py:104-110, lcu:550-588) and
  * noise_table[256] (lookup.py:113-117); they are converted to Montgomery form
- * and kept resident in HBM (ensure_device_tables_, lcu:2625-2652). */
+ * on the device and kept resident in HBM (ensure_device_tables_, lcu:2625-2652).
+ * chunks = 0 (tables NULL) creates a decrypt-only key. */
 int chx_lookup_key_create(chx_ctx* ctx, const uint32_t* n, int n_limbs, const uint32_t* a, int a_limbs,
                           const uint32_t* g_a_inv, const uint32_t* g_table, int chunks,
                           const uint32_t* noise_table, int noise_entries, chx_lkey** out);

@@ -149,9 +149,12 @@ def test_lookup_seeded_every_variant(ctx):
             for m, ix in zip(ms, idx)]
     digits = plaintexts_to_digits(ints_to_limbs(ms, key.width), key.chunks)
     exp_tpis = Context.supported_tpi(key.width, op=OP_MODEXP)
-    for i, tpi in enumerate(Context.supported_tpi(key.width, op=OP_LOOKUP)):
+    lk_tpis = Context.supported_tpi(key.width, op=OP_LOOKUP)
+    mul_tpis = Context.supported_tpi(key.width, op=OP_MULMOD)
+    for i in range(max(len(exp_tpis), len(lk_tpis), len(mul_tpis))):
+        tpi = lk_tpis[i % len(lk_tpis)]
         ctx.set_tuning(OP_LOOKUP, key.width, tpi, 0)
-        ctx.set_tuning(OP_MULMOD, key.width, tpi, 0)
+        ctx.set_tuning(OP_MULMOD, key.width, mul_tpis[i % len(mul_tpis)], 0)
         ctx.set_tuning(OP_MODEXP, key.width, exp_tpis[i % len(exp_tpis)], 0)
         got = limbs_to_ints(key.encrypt_digits(digits, idx))
         assert got == want, f"tpi={tpi}"
