@@ -122,3 +122,24 @@ def test_invalid_inputs_raise(ctx):
         Modulus(ctx, 1 << 100, 32)  # even modulus
     with pytest.raises(ValueError):
         mod.mulmod([1 << 1024], [1])  # does not fit the width
+
+
+@pytest.mark.parametrize("limbs", [32, 64, 128])
+def test_modexp_each_every_variant(ctx, limbs):
+    """Per-item exponents (fixed window): zero, one, small and full-width."""
+    from cuhepy_amd.engine import Context, Modulus
+    from cuhepy_amd.limbs import limbs_to_ints
+    rng = random.Random(300 + limbs)
+    OP_RAGGED = 3
+    for tpi in Context.supported_tpi(limbs, op=OP_RAGGED):
+        for window in (0, 3):
+            ctx.set_tuning(OP_RAGGED, limbs, tpi, window)
+            n = _mod(rng, limbs)
+            mod = Modulus(ctx, n, limbs)
+            count = 67
+            bases = [0, 1, n - 1, rng.randrange(n)] + [rng.randrange(n) for _ in range(count - 4)]
+            exps = [5, 0, 1, 2] + [rng.getrandbits(rng.choice([1, 17, 300, 32 * limbs])) for _ in range(count - 4)]
+            got = limbs_to_ints(mod.modexp_each(bases, exps))
+            want = [pow(b, e, n) for b, e in zip(bases, exps)]
+            bad = [i for i, (g, w) in enumerate(zip(got, want)) if g != w]
+            assert not bad, f"tpi={tpi} window={window} first bad {bad[:5]}"
