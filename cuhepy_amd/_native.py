@@ -20,6 +20,17 @@ if not os.path.exists(LIB_PATH):
         "`python -c 'import __graft_entry__ as g; g.build()'` or `python -m cuhepy_amd._build`."
     )
 
+# One HIP runtime per process. PyTorch bundles its own libamdhip64 /
+# libhsa-runtime64 with the same sonames as the system ones; whichever is
+# loaded first is the one both sides bind to. If the system runtime were loaded
+# first, PyTorch would still load its own copy and the second runtime to open
+# the device finds no GPU. So when PyTorch is installed it is imported first
+# and the library binds to PyTorch's runtime.
+try:
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover - plain C/ctypes use without PyTorch
+    pass
+
 try:
     lib = ctypes.CDLL(LIB_PATH)
 except OSError as e:  # pragma: no cover - depends on the ROCm runtime
@@ -31,6 +42,14 @@ vp = c_void_p
 
 CHX_OK, CHX_EINVAL, CHX_EDEVICE, CHX_ENOMEM, CHX_EUNSUPPORTED = 0, 1, 2, 3, 4
 OP_MODEXP, OP_MULMOD, OP_LOOKUP, OP_RAGGED = 0, 1, 2, 3
+
+
+
+class RaggedSubStruct(ctypes.Structure):
+    """chx_ragged_sub"""
+    _fields_ = [("mod", c_void_p), ("base", c_void_p), ("exps", c_void_p), ("out", c_void_p),
+                ("count", c_uint32), ("stride", c_uint32), ("e_limbs", c_uint32), ("e_bits", c_uint32)]
+
 
 # name -> (restype, argtypes); the list is the exported surface of chx.h
 SIGNATURES = {
@@ -55,6 +74,10 @@ SIGNATURES = {
     "chx_modexp_dev": (c_int, [vp, vp, vp, vp, c_size_t, c_size_t, u32p, c_int]),
     "chx_modexp_each": (c_int, [vp, vp, u32p, c_size_t, u32p, c_int, u32p]),
     "chx_modexp_each_dev": (c_int, [vp, vp, vp, vp, c_int, c_int, vp, c_size_t, c_size_t]),
+    "chx_ragged_create": (c_int, [vp, POINTER(RaggedSubStruct), c_int, c_int, c_int, c_int, POINTER(vp)]),
+    "chx_ragged_run": (c_int, [vp, c_int]),
+    "chx_ragged_run_plain": (c_int, [vp]),
+    "chx_ragged_destroy": (c_int, [vp]),
     "chx_paillier_key_create": (c_int, [vp, u32p, c_int, u32p, u32p, u32p, u32p, c_int, u32p, u32p, u32p,
                                         POINTER(vp)]),
     "chx_paillier_key_destroy": (c_int, [vp]),

@@ -389,8 +389,8 @@ const std::vector<LazyCfg>& lazy_cfgs(int L32) {
   // variants whose accumulators fit in VGPRs without spilling (checked with
   // -Rpass-analysis=kernel-resource-usage); index = dispatch slot
   static const std::vector<LazyCfg> c32 = {{28, 38, 1}, {28, 38, 2}};
-  static const std::vector<LazyCfg> c64 = {{28, 76, 4}};
-  static const std::vector<LazyCfg> c128 = {{27, 152, 8}};
+  static const std::vector<LazyCfg> c64 = {{28, 76, 4}, {28, 80, 8}};
+  static const std::vector<LazyCfg> c128 = {{27, 152, 8}, {27, 160, 16}};
   static const std::vector<LazyCfg> none = {};
   return L32 == 32 ? c32 : L32 == 64 ? c64 : L32 == 128 ? c128 : none;
 }
@@ -458,13 +458,15 @@ size_t lazy_lds_bytes(int lp, int l32, int tpi) {
   return (size_t)WAVES_PER_BLOCK * (size_t)std::max(lp, l32) * (64 / tpi) * 4;
 }
 
-#define CHX_DISPATCH_LAZY(L32_, IDX_, CALL)                                                     \
+#define CHX_DISPATCH_LAZY(L32_, IDX_, ...)                                                        \
   do {                                                                                          \
     const int l__ = (L32_), i__ = (IDX_);                                                       \
-    if (l__ == 32 && i__ == 0) { constexpr int RB = 28, LP = 38, TPI = 1, L32 = 32; CALL; }      \
-    else if (l__ == 32 && i__ == 1) { constexpr int RB = 28, LP = 38, TPI = 2, L32 = 32; CALL; } \
-    else if (l__ == 64 && i__ == 0) { constexpr int RB = 28, LP = 76, TPI = 4, L32 = 64; CALL; }  \
-    else if (l__ == 128 && i__ == 0) { constexpr int RB = 27, LP = 152, TPI = 8, L32 = 128; CALL; } \
+    if (l__ == 32 && i__ == 0) { constexpr int RB = 28, LP = 38, TPI = 1, L32 = 32; __VA_ARGS__; }      \
+    else if (l__ == 32 && i__ == 1) { constexpr int RB = 28, LP = 38, TPI = 2, L32 = 32; __VA_ARGS__; } \
+    else if (l__ == 64 && i__ == 0) { constexpr int RB = 28, LP = 76, TPI = 4, L32 = 64; __VA_ARGS__; }  \
+    else if (l__ == 64 && i__ == 1) { constexpr int RB = 28, LP = 80, TPI = 8, L32 = 64; __VA_ARGS__; }  \
+    else if (l__ == 128 && i__ == 0) { constexpr int RB = 27, LP = 152, TPI = 8, L32 = 128; __VA_ARGS__; } \
+    else if (l__ == 128 && i__ == 1) { constexpr int RB = 27, LP = 160, TPI = 16, L32 = 128; __VA_ARGS__; } \
     else fail(CHX_EUNSUPPORTED, "no lazy kernel for %d limbs (configuration %d)", l__, i__);   \
   } while (0)
 
@@ -945,6 +947,208 @@ int chx_modexp_each_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* base, co
     if (stride < round_up(count, 64)) fail(CHX_EINVAL, "stride %zu below count rounded to 64", stride);
     launch_modexp_each(ctx, m->lz, base, exps, e_limbs, e_bits, out, count, stride);
   });
+}
+
+// ---------------------------------------------------------------------------
+// ragged batches
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+struct chx_ragged {
+  chx_ctx* ctx = nullptr;
+  int variant = 0, window = 5, nsubs = 0, total_tiles = 0, blocks = 0;
+  std::vector<chx_ragged_sub> subs;
+  std::vector<RaggedSub> hsubs;  // plain form (one entry per sub-batch, input order)
+  // persistent form: [0] all sub-batches (mixed kernel) or [0] 64-limb and
+  // [1] 128-limb sub-batches (one persistent kernel per width)
+  int nq = 0;
+  int q_nsubs[2] = {0, 0}, q_tiles[2] = {0, 0};
+  RaggedSub* d_q[2] = {nullptr, nullptr};
+  uint32_t* d_counter = nullptr;
+  uint32_t* d_scratch = nullptr;
+  uint32_t* d_plain_tab = nullptr;
+  size_t wave_words = 0, plain_tab_words = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+// variant 0: per-width persistent kernels (4 / 8 lanes), variant 1: per-width
+// (8 / 16 lanes), variant 2: one mixed persistent kernel (8 / 16 lanes)
+const int kRaggedCfg[3] = {0, 1, 1};  // lazy configuration index used by each variant
+
+void ragged_enqueue(chx_ragged* rg) {
+  chx_ctx* ctx = rg->ctx;
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, rg->d_counter, 0u, 2);
+  if (rg->variant == 2) {
+    constexpr int RB1 = 28, LP1 = 80, T1 = 8, RB2 = 27, LP2 = 160, T2 = 16;
+    typedef Lazy<RB1, LP1, T1, 64> E1;
+    typedef Lazy<RB2, LP2, T2, 128> E2;
+    constexpr int AREA = (E1::SLOTS * E1::G > E2::SLOTS * E2::G) ? E1::SLOTS * E1::G : E2::SLOTS * E2::G;
+    hipLaunchKernelGGL((k_ragged2<RB1, LP1, T1, RB2, LP2, T2>), dim3(rg->blocks), dim3(BLOCK_THREADS),
+                       (size_t)WAVES_PER_BLOCK * AREA * 4, ctx->stream, rg->d_q[0], rg->q_nsubs[0], rg->q_tiles[0],
+                       rg->d_counter, rg->window, rg->d_scratch, rg->wave_words);
+    return;
+  }
+  for (int q = 0; q < 2; ++q) {
+    if (!rg->q_nsubs[q]) continue;
+    const int L = q == 0 ? 64 : 128;
+    const int ci = kRaggedCfg[rg->variant];
+    CHX_DISPATCH_LAZY(L, ci, {
+      typedef Lazy<RB, LP, TPI, L32> E;
+      hipLaunchKernelGGL((k_ragged1<RB, LP, TPI, L32>), dim3(rg->blocks), dim3(BLOCK_THREADS),
+                         (size_t)WAVES_PER_BLOCK * E::SLOTS * E::G * 4, ctx->stream, rg->d_q[q], rg->q_nsubs[q],
+                         rg->q_tiles[q], rg->d_counter + q, rg->window, rg->d_scratch, rg->wave_words);
+    });
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int chx_ragged_create(chx_ctx* ctx, const chx_ragged_sub* subs, int nsubs, int variant, int window,
+                      int waves_per_cu, chx_ragged** out) {
+  return with_ctx(ctx, [&] {
+    if (!subs || nsubs <= 0 || !out) fail(CHX_EINVAL, "bad argument");
+    if (variant < 0 || variant > 2) fail(CHX_EUNSUPPORTED, "ragged variant %d not built", variant);
+    if (window == 0) window = 5;
+    if (window < 2 || window > 6) fail(CHX_EINVAL, "window %d out of range [2, 6]", window);
+    if (waves_per_cu == 0) waves_per_cu = 8;
+    if (waves_per_cu < 1 || waves_per_cu > 16) fail(CHX_EINVAL, "waves_per_cu out of range");
+    std::unique_ptr<chx_ragged> rg(new chx_ragged());
+    rg->ctx = ctx;
+    rg->variant = variant;
+    rg->window = window;
+    rg->nsubs = nsubs;
+    rg->subs.assign(subs, subs + nsubs);
+    const int ci = kRaggedCfg[variant];
+    std::vector<RaggedSub> q[2];
+    int tiles = 0;
+    int lp_max_g = 0;
+    for (int s = 0; s < nsubs; ++s) {
+      const chx_ragged_sub& u = subs[s];
+      if (!u.mod || !u.base || !u.exps || !u.out) fail(CHX_EINVAL, "sub-batch %d: null pointer", s);
+      const int L = u.mod->hc.L;
+      if (L != 64 && L != 128) fail(CHX_EUNSUPPORTED, "sub-batch %d: width %d limbs (64 or 128 supported)", s, L);
+      if (u.count == 0 || u.stride < round_up(u.count, 64)) fail(CHX_EINVAL, "sub-batch %d: bad count/stride", s);
+      if (u.e_limbs == 0 || u.e_bits == 0 || u.e_bits > 32 * u.e_limbs)
+        fail(CHX_EINVAL, "sub-batch %d: bad exponent size", s);
+      const LazyCfg cfg = lazy_cfgs(L)[ci];
+      const int G = 64 / cfg.tpi;
+      RaggedSub r;
+      r.md = u.mod->lz.d[ci];
+      r.base = u.base;
+      r.exps = u.exps;
+      r.out = u.out;
+      r.width = L;
+      r.count = (int)u.count;
+      r.stride = (int)u.stride;
+      r.e_limbs = (int)u.e_limbs;
+      r.nwin = ((int)u.e_bits + window - 1) / window;
+      const int qi = variant == 2 ? 0 : (L == 64 ? 0 : 1);
+      r.tile0 = rg->q_tiles[qi];
+      rg->q_tiles[qi] += ((int)u.count + G - 1) / G;
+      tiles += ((int)u.count + G - 1) / G;
+      lp_max_g = std::max(lp_max_g, cfg.lp * G);
+      rg->hsubs.push_back(r);
+      q[qi].push_back(r);
+    }
+    for (int i = 0; i < 2; ++i) {
+      rg->q_nsubs[i] = (int)q[i].size();
+      if (!q[i].empty()) rg->d_q[i] = dev_upload(q[i].data(), q[i].size(), ctx->stream);
+    }
+    rg->total_tiles = tiles;
+    hipDeviceProp_t prop;
+    hip_check(hipGetDeviceProperties(&prop, ctx->device), "hipGetDeviceProperties");
+    const int waves = prop.multiProcessorCount * waves_per_cu;
+    rg->blocks = std::max(1, waves / WAVES_PER_BLOCK);
+    rg->wave_words = ((size_t)1 << window) * lp_max_g;
+    hip_check(hipMalloc(&rg->d_counter, 64), "hipMalloc");
+    hip_check(hipMalloc(&rg->d_scratch, (size_t)rg->blocks * WAVES_PER_BLOCK * rg->wave_words * 4), "hipMalloc");
+    *out = rg.release();
+  });
+}
+
+int chx_ragged_run(chx_ragged* rg, int use_graph) {
+  if (!rg) {
+    g_last_error = "null plan";
+    return CHX_EINVAL;
+  }
+  chx_ctx* ctx = rg->ctx;
+  return with_ctx(ctx, [&] {
+    if (!use_graph) {
+      record_start(ctx, "ragged");
+      ragged_enqueue(rg);
+      record_stop(ctx);
+      check_launch("k_ragged1/k_ragged2");
+      return;
+    }
+    if (!rg->exec) {
+      hip_check(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+      ragged_enqueue(rg);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+      hip_check(e, "hipStreamEndCapture");
+      rg->graph = g;
+      hip_check(hipGraphInstantiate(&rg->exec, rg->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    }
+    record_start(ctx, "ragged");
+    hip_check(hipGraphLaunch(rg->exec, ctx->stream), "hipGraphLaunch");
+    record_stop(ctx);
+  });
+}
+
+int chx_ragged_run_plain(chx_ragged* rg) {
+  if (!rg) {
+    g_last_error = "null plan";
+    return CHX_EINVAL;
+  }
+  chx_ctx* ctx = rg->ctx;
+  return with_ctx(ctx, [&] {
+    size_t need = 0;
+    for (const auto& u : rg->subs) {
+      const LazyCfg cfg = lazy_cfgs(u.mod->hc.L)[kRaggedCfg[rg->variant]];
+      need = std::max(need, ((size_t)1 << rg->window) * cfg.lp * round_up(u.count, BLOCK_THREADS));
+    }
+    if (need > rg->plain_tab_words) {
+      if (rg->d_plain_tab) (void)hipFree(rg->d_plain_tab);
+      rg->d_plain_tab = nullptr;
+      hip_check(hipMalloc(&rg->d_plain_tab, need * 4), "hipMalloc");
+      rg->plain_tab_words = need;
+    }
+    record_start(ctx, "ragged_plain");
+    for (size_t s = 0; s < rg->subs.size(); ++s) {
+      const chx_ragged_sub& u = rg->subs[s];
+      const int L = u.mod->hc.L;
+      const int ci = kRaggedCfg[rg->variant];
+      const size_t tstride = round_up(u.count, BLOCK_THREADS);
+      CHX_DISPATCH_LAZY(L, ci, {
+        hipLaunchKernelGGL((k_lazy_modexp_each<RB, LP, TPI, L32>), grid_for(u.count, TPI), dim3(BLOCK_THREADS),
+                           lazy_lds_bytes(LP, L32, TPI), ctx->stream, rg->hsubs[s].md, u.base, u.exps,
+                           (int)u.e_limbs, rg->hsubs[s].nwin, rg->window, u.out, (int)u.count, (int)u.stride,
+                           rg->d_plain_tab, (int)tstride);
+      });
+    }
+    record_stop(ctx);
+    check_launch("k_lazy_modexp_each");
+  });
+}
+
+int chx_ragged_destroy(chx_ragged* rg) {
+  if (!rg) return CHX_OK;
+  int rc = guarded([&] {
+    hip_check(hipSetDevice(rg->ctx->device), "hipSetDevice");
+    (void)hipStreamSynchronize(rg->ctx->stream);
+    if (rg->exec) (void)hipGraphExecDestroy(rg->exec);
+    if (rg->graph) (void)hipGraphDestroy(rg->graph);
+    for (void* p : {(void*)rg->d_q[0], (void*)rg->d_q[1], (void*)rg->d_counter, (void*)rg->d_scratch,
+                    (void*)rg->d_plain_tab})
+      if (p) (void)hipFree(p);
+  });
+  delete rg;
+  return rc;
 }
 
 // ---------------------------------------------------------------------------

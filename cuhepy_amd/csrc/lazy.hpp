@@ -178,7 +178,7 @@ struct Lazy {
 };
 
 // Constants of one modulus in the lazy radix (device layout).
-constexpr int LPMAX = 152;
+constexpr int LPMAX = 160;
 struct LazyDev {
   u32 n0;           // -N^-1 mod 2^RB
   u32 rb, lp, l32;  // radix bits, positions, 32-bit limbs (checked by the host)
@@ -384,29 +384,26 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_lookup_encrypt(const Laz
 // diverges; the table holds x^0 .. x^(2^w - 1) per item. Exponents: e_limbs
 // 32-bit limbs per item, interleaved like the other batch buffers; nwin
 // covers the widest exponent of the batch.
+// One item per group: out[item] = in[item]^exps[item] mod N, fixed window
+// `w`, `nwin` windows. Table: 2^w entries of LP positions at
+// table[(e * LP + pos) * tstride + slot]. `it` is the clamped item used for
+// loads (groups past `count` compute a duplicate and skip the store).
 template <int RB, int LP, int TPI, int L32>
-__global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_modexp_each(const LazyDev* __restrict__ md,
-                                                                    const u32* __restrict__ in,
-                                                                    const u32* __restrict__ exps, int e_limbs,
-                                                                    int nwin, int w, u32* __restrict__ out,
-                                                                    int count, int stride, u32* __restrict__ table,
-                                                                    int tstride) {
-  extern __shared__ uint4 smem4[];
+__device__ __forceinline__ void modexp_each_item(Lazy<RB, LP, TPI, L32>& M, const LazyDev* __restrict__ md,
+                                                 const u32* __restrict__ in, const u32* __restrict__ exps,
+                                                 int e_limbs, int nwin, int w, u32* __restrict__ out, int count,
+                                                 int stride, u32* __restrict__ table, int tstride, int item, int it,
+                                                 int slot) {
   typedef Lazy<RB, LP, TPI, L32> E;
   constexpr int K = E::K, K32 = E::K32;
-  u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * (E::SLOTS * E::G);
-  E M;
-  M.init(area, md->N, md->n0);
-  const int item = group_item<TPI>();
-  const int it = item < count ? item : count - 1;
   const int r = Grp<TPI>::rank();
   auto tstore = [&](int e, const u32 (&v)[K]) {
 #pragma unroll
-    for (int p = 0; p < K; ++p) table[((size_t)e * LP + r * K + p) * tstride + item] = v[p];
+    for (int p = 0; p < K; ++p) table[((size_t)e * LP + r * K + p) * tstride + slot] = v[p];
   };
   auto tload = [&](int e, u32 (&v)[K]) {
 #pragma unroll
-    for (int p = 0; p < K; ++p) v[p] = table[((size_t)e * LP + r * K + p) * tstride + item];
+    for (int p = 0; p < K; ++p) v[p] = table[((size_t)e * LP + r * K + p) * tstride + slot];
   };
   auto digit = [&](int win) -> int {
     const int bit = win * w;
@@ -467,6 +464,116 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_modexp_each(const LazyDe
   if (item < count) {
 #pragma unroll
     for (int k = 0; k < K32; ++k) out[(size_t)(r * K32 + k) * stride + item] = wv[k];
+  }
+}
+
+// base_i^(e_i) mod N with a per-item exponent (fixed-window method, HAC
+// Alg. 14.82): every item runs the same sequence -- `w` squarings then one
+// table multiply per window, `nwin` windows from the top -- so a wave never
+// diverges; the table holds x^0 .. x^(2^w - 1) per item. Exponents: e_limbs
+// 32-bit limbs per item, interleaved like the other batch buffers; nwin
+// covers the widest exponent of the batch.
+template <int RB, int LP, int TPI, int L32>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_modexp_each(const LazyDev* __restrict__ md,
+                                                                    const u32* __restrict__ in,
+                                                                    const u32* __restrict__ exps, int e_limbs,
+                                                                    int nwin, int w, u32* __restrict__ out,
+                                                                    int count, int stride, u32* __restrict__ table,
+                                                                    int tstride) {
+  extern __shared__ uint4 smem4[];
+  typedef Lazy<RB, LP, TPI, L32> E;
+  u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * (E::SLOTS * E::G);
+  E M;
+  M.init(area, md->N, md->n0);
+  const int item = group_item<TPI>();
+  const int it = item < count ? item : count - 1;
+  modexp_each_item<RB, LP, TPI, L32>(M, md, in, exps, e_limbs, nwin, w, out, count, stride, table, tstride, item,
+                                     it, item);
+}
+
+// ---------------------------------------------------------------------------
+// Ragged batches: one persistent launch over many sub-batches of mixed width
+// ---------------------------------------------------------------------------
+struct RaggedSub {
+  const LazyDev* md;  // constants of this sub-batch's modulus (configuration of its width)
+  const u32* base;    // interleaved [L32][stride]
+  const u32* exps;    // interleaved [e_limbs][stride]
+  u32* out;           // interleaved [L32][stride]
+  int width;          // 32-bit limbs: 64 or 128
+  int count, stride, e_limbs, nwin;
+  int tile0;          // first queue tile of this sub-batch
+};
+
+__global__ void k_fill_u32(u32* __restrict__ p, u32 v, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Persistent work-queue loop of one wave: take tiles (one tile = the groups
+// of one wave) from `counter` until it passes `total_tiles`; map each tile to
+// its sub-batch by binary search over the tile prefix offsets. Every wave
+// reaches the exit test after every tile, so the grid drains.
+__device__ __forceinline__ int ragged_next(u32* counter, int total_tiles, const RaggedSub* __restrict__ subs,
+                                           int nsubs, int* local) {
+  u32 t = 0;
+  if (lane_id() == 0) t = atomicAdd(counter, 1u);
+  t = (u32)__builtin_amdgcn_readfirstlane((int)t);
+  if ((int)t >= total_tiles) return -1;
+  int lo = 0, hi = nsubs - 1;
+  while (lo < hi) {  // last sub with tile0 <= t
+    const int mid = (lo + hi + 1) >> 1;
+    if (subs[mid].tile0 <= (int)t) lo = mid; else hi = mid - 1;
+  }
+  *local = (int)t - subs[lo].tile0;
+  return lo;
+}
+
+template <int RB, int LP, int TPI, int L32>
+__device__ __forceinline__ void ragged_tile(const RaggedSub* __restrict__ sbp, int local, u32* area, u32* table,
+                                            int w) {
+  typedef Lazy<RB, LP, TPI, L32> E;
+  const RaggedSub sb = *sbp;
+  E M;
+  M.init(area, sb.md->N, sb.md->n0);
+  const int slot = Grp<TPI>::gid();
+  const int item = local * E::G + slot;
+  const int it = item < sb.count ? item : sb.count - 1;
+  modexp_each_item<RB, LP, TPI, L32>(M, sb.md, sb.base, sb.exps, sb.e_limbs, sb.nwin, w, sb.out, sb.count,
+                                     sb.stride, table, E::G, item, it, slot);
+}
+
+// Persistent kernel over sub-batches of one width.
+template <int RB, int LP, int TPI, int L32>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_ragged1(const RaggedSub* __restrict__ subs, int nsubs,
+                                                           int total_tiles, u32* __restrict__ counter, int w,
+                                                           u32* __restrict__ scratch, size_t wave_words) {
+  extern __shared__ uint4 smem4[];
+  typedef Lazy<RB, LP, TPI, L32> E;
+  u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * (E::SLOTS * E::G);
+  u32* table = scratch + ((size_t)blockIdx.x * (BLOCK_THREADS / 64) + threadIdx.x / 64) * wave_words;
+  int local = 0;
+  for (int s; (s = ragged_next(counter, total_tiles, subs, nsubs, &local)) >= 0;)
+    ragged_tile<RB, LP, TPI, L32>(subs + s, local, area, table, w);
+}
+
+// Persistent kernel over sub-batches of both widths (64- and 128-limb moduli):
+// the width of a tile's sub-batch selects the engine (a wave-uniform branch).
+template <int RB1, int LP1, int TPI1, int RB2, int LP2, int TPI2>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_ragged2(const RaggedSub* __restrict__ subs, int nsubs,
+                                                           int total_tiles, u32* __restrict__ counter, int w,
+                                                           u32* __restrict__ scratch, size_t wave_words) {
+  extern __shared__ uint4 smem4[];
+  typedef Lazy<RB1, LP1, TPI1, 64> E1;
+  typedef Lazy<RB2, LP2, TPI2, 128> E2;
+  constexpr int AREA = (E1::SLOTS * E1::G > E2::SLOTS * E2::G) ? E1::SLOTS * E1::G : E2::SLOTS * E2::G;
+  u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * AREA;
+  u32* table = scratch + ((size_t)blockIdx.x * (BLOCK_THREADS / 64) + threadIdx.x / 64) * wave_words;
+  int local = 0;
+  for (int s; (s = ragged_next(counter, total_tiles, subs, nsubs, &local)) >= 0;) {
+    if (subs[s].width == 64)
+      ragged_tile<RB1, LP1, TPI1, 64>(subs + s, local, area, table, w);
+    else
+      ragged_tile<RB2, LP2, TPI2, 128>(subs + s, local, area, table, w);
   }
 }
 

@@ -383,3 +383,47 @@ class LookupKey:
             N.check(N.lib.chx_lookup_decode_hamming(self.ctx.handle, self.handle, _p32(C), _p32(Mk), _p32(out),
                                                     len(C)))
         return out
+
+
+class RaggedPlan:
+    """Mixed-width, per-item-exponent modexp over many sub-batches in one
+    persistent launch captured into a hipGraph (chx_ragged_*).
+
+    `subs`: dicts with keys mod (Modulus of 64 or 128 limbs), base, exps, out
+    (device tensors, interleaved), count, stride, e_limbs, e_bits. The tensors
+    must stay alive while the plan exists (the plan keeps references).
+    """
+
+    def __init__(self, ctx: Context, subs: list, variant: int = 0, window: int = 0, waves_per_cu: int = 0):
+        self.ctx = ctx
+        self._keep = subs
+        arr = (N.RaggedSubStruct * len(subs))()
+        for i, s in enumerate(subs):
+            arr[i].mod = s["mod"].handle.value
+            arr[i].base = int(s["base"].data_ptr())
+            arr[i].exps = int(s["exps"].data_ptr())
+            arr[i].out = int(s["out"].data_ptr())
+            arr[i].count, arr[i].stride = s["count"], s["stride"]
+            arr[i].e_limbs, arr[i].e_bits = s["e_limbs"], s["e_bits"]
+        h = ctypes.c_void_p()
+        N.check(N.lib.chx_ragged_create(ctx.handle, arr, len(subs), variant, window, waves_per_cu,
+                                        ctypes.byref(h)))
+        self.handle = h
+        self.variant, self.window, self.waves_per_cu = variant, window, waves_per_cu
+
+    def run(self, graph: bool = True) -> None:
+        N.check(N.lib.chx_ragged_run(self.handle, int(graph)))
+
+    def run_plain(self) -> None:
+        N.check(N.lib.chx_ragged_run_plain(self.handle))
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            N.lib.chx_ragged_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -110,6 +110,32 @@ int chx_modexp_each(chx_ctx* ctx, const chx_mod* m, const uint32_t* base, size_t
 int chx_modexp_each_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* base, const uint32_t* exps, int e_limbs,
                         int e_bits, uint32_t* out, size_t count, size_t stride);
 
+/* ---- ragged batches (config 4; no reference counterpart) ----
+ * Many sub-batches of mixed width (64 or 128 limbs), each with its own modulus
+ * and per-item exponents, in ONE persistent launch: waves pull tiles from an
+ * atomic work queue. The launch (queue reset by a fill kernel + the persistent
+ * kernel) is captured once into a hipGraph and replayed. Device buffers are
+ * interleaved; they must stay valid while the plan exists. */
+typedef struct chx_ragged chx_ragged;
+typedef struct {
+  const chx_mod* mod;   /* modulus of the sub-batch: 64 or 128 limbs */
+  const uint32_t* base; /* device [limbs][stride] */
+  const uint32_t* exps; /* device [e_limbs][stride] */
+  uint32_t* out;        /* device [limbs][stride] */
+  uint32_t count, stride, e_limbs, e_bits;
+} chx_ragged_sub;
+/* variant 0: one persistent kernel per width (4 lanes per 2048-bit, 8 per
+ * 4096-bit modulus), both in the one graph; variant 1: the same with 8 and 16
+ * lanes; variant 2: a single persistent kernel over both widths (8 and 16
+ * lanes). window 2..6 (0 = 5); waves_per_cu: persistent waves per CU (0 = 8). */
+int chx_ragged_create(chx_ctx* ctx, const chx_ragged_sub* subs, int nsubs, int variant, int window,
+                      int waves_per_cu, chx_ragged** out);
+/* persistent launch; use_graph != 0 replays the captured hipGraph */
+int chx_ragged_run(chx_ragged* rg, int use_graph);
+/* the plain form: one non-persistent launch per sub-batch, no graph */
+int chx_ragged_run_plain(chx_ragged* rg);
+int chx_ragged_destroy(chx_ragged* rg);
+
 /* ---- Paillier (g = n + 1) ----
  * n_limbs: limbs of n (<= 64); the working width is the smallest of 32/64/128
  * limbs that holds n^2. inv = phi^-1 mod n. When p != NULL, CRT decryption is
