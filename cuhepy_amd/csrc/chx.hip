@@ -438,6 +438,15 @@ void free_lazy_set(LazySet& ls) {
 }
 
 // pick the lazy configuration for a batch (tuning override, else fill the chip)
+// Lookup tables (g table, noise table) are stored once, in the positions of
+// configuration 0; only configurations with the same radix and position count
+// can read them.
+bool lazy_usable(int op, int L32, size_t ci) {
+  const auto& cfgs = lazy_cfgs(L32);
+  if (op != CHX_OP_LOOKUP) return true;
+  return cfgs[ci].rb == cfgs[0].rb && cfgs[ci].lp == cfgs[0].lp;
+}
+
 int pick_lazy(chx_ctx* ctx, int L32, size_t count, int op = CHX_OP_MODEXP) {
   const int li = lidx(L32);
   if (li < 0) fail(CHX_EUNSUPPORTED, "width of %d limbs is not built", L32);
@@ -445,13 +454,17 @@ int pick_lazy(chx_ctx* ctx, int L32, size_t count, int op = CHX_OP_MODEXP) {
   const int forced = ctx->tune_tpi[op][li];
   if (forced) {
     for (size_t i = 0; i < cfgs.size(); ++i)
-      if (cfgs[i].tpi == forced) return (int)i;
-    fail(CHX_EUNSUPPORTED, "tpi=%d is not built for %d-limb exponentiation", forced, L32);
+      if (cfgs[i].tpi == forced && lazy_usable(op, L32, i)) return (int)i;
+    fail(CHX_EUNSUPPORTED, "tpi=%d is not built for %d-limb op %d", forced, L32, op);
   }
   const size_t want = target_lanes(ctx->device);
-  for (size_t i = 0; i < cfgs.size(); ++i)
+  int last = 0;
+  for (size_t i = 0; i < cfgs.size(); ++i) {
+    if (!lazy_usable(op, L32, i)) continue;
+    last = (int)i;
     if (count * (size_t)cfgs[i].tpi >= want) return (int)i;
-  return (int)cfgs.size() - 1;
+  }
+  return last;
 }
 
 size_t lazy_lds_bytes(int lp, int l32, int tpi) {
@@ -752,7 +765,8 @@ int chx_ctx_set_tuning(chx_ctx* ctx, int op, int limbs, int tpi, int window) {
     if (tpi) {
       bool ok = false;
       if (op == CHX_OP_MODEXP || op == CHX_OP_LOOKUP || op == CHX_OP_RAGGED) {
-        for (const auto& c : lazy_cfgs(limbs)) ok = ok || c.tpi == tpi;
+        const auto& cfgs = lazy_cfgs(limbs);
+        for (size_t i = 0; i < cfgs.size(); ++i) ok = ok || (cfgs[i].tpi == tpi && lazy_usable(op, limbs, i));
       } else {
         const auto& opts = built_tpi(limbs);
         ok = std::find(opts.begin(), opts.end(), tpi) != opts.end();
@@ -774,8 +788,13 @@ int chx_supported_tpi(int limbs, int* tpis, int cap) {
 int chx_supported_tpi_op(int op, int limbs, int* tpis, int cap) {
   if (op == CHX_OP_MULMOD) return chx_supported_tpi(limbs, tpis, cap);
   const auto& v = lazy_cfgs(limbs);
-  for (int i = 0; i < (int)v.size() && i < cap; ++i) tpis[i] = v[i].tpi;
-  return (int)v.size();
+  int k = 0;
+  for (size_t i = 0; i < v.size(); ++i) {
+    if (!lazy_usable(op, limbs, i)) continue;
+    if (k < cap) tpis[k] = v[i].tpi;
+    ++k;
+  }
+  return k;
 }
 
 size_t chx_ctx_scratch_bytes(chx_ctx* ctx) {
