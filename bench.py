@@ -113,6 +113,198 @@ def committed_traffic(kernel: str):
     return None, None
 
 
+
+def ragged_leg(ctx, args, peak: float, device_name: str) -> dict:
+    """Ragged mixed-width modexp: 256 sub-batches, sizes log-uniform in
+    [1, 8192], n of 1024 or 2048 bits (moduli n^2 of 2048 / 4096 bits, 50/50),
+    bases uniform below n^2, exponents of full modulus width; autotuned, one
+    persistent launch replayed from a hipGraph; beside it the plain form (one
+    launch per sub-batch)."""
+    import numpy as np
+
+    from cuhepy_amd.autotune import tune_ragged
+    from cuhepy_amd.engine import RaggedPlan
+    from cuhepy_amd.ragged import DeviceWorkload, make_subbatches
+    from oracle import gmp_engine as G
+
+    t = time.perf_counter()
+    subs = make_subbatches(args.ragged_subs, args.ragged_max, seed=args.seed + 3)
+    wl = DeviceWorkload(ctx, subs)
+    items, macs = wl.total_items(), wl.algorithmic_macs()
+    n64 = sum(s.count for s in subs if s.limbs == 64)
+    log(f"ragged: {len(subs)} sub-batches, {items} modexps ({n64} at 2048-bit, {items - n64} at 4096-bit moduli), "
+        f"{macs:.3e} MACs, uploaded ({time.perf_counter() - t:.1f}s)")
+    t = time.perf_counter()
+    best, tlog = tune_ragged(ctx, wl, device_name, reps=1, force=True)
+    log(f"ragged autotune ({time.perf_counter() - t:.1f}s): best {best}; " +
+        ", ".join(f"v{c['variant']}/w{c['window']}/{c['waves']}wpc {s * 1e3:.0f}ms" for c, s in tlog))
+    plan = RaggedPlan(ctx, wl.desc, best["variant"], best["window"], best["waves"])
+    plan.run(graph=True)
+    ctx.sync()
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.ragged_steps):
+        plan.run(graph=True)
+    ctx.sync()
+    wall = (time.perf_counter() - t0) / args.ragged_steps
+    ms, nl = ctx.kernel_time("ragged")
+    ctx.set_timing(False)
+    launch_s = ms / max(nl, 1) / 1e3
+    persistent = [wl.result(i) for i in range(len(subs))]
+    powm = G.powm if G.available() else pow
+    t = time.perf_counter()
+    bad = wl.spot_check(2, random.Random(args.seed), powm)
+    cpu_s = time.perf_counter() - t
+    checked = wl.checked
+    # plain form: one non-persistent launch per sub-batch, no graph
+    wl.clear_outputs()
+    plan.run_plain()
+    ctx.sync()
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    plan.run_plain()
+    ctx.sync()
+    plain_wall = time.perf_counter() - t0
+    pms, _ = ctx.kernel_time("ragged_plain")
+    ctx.set_timing(False)
+    same = all(np.array_equal(wl.result(i), persistent[i]) for i in range(len(subs)))
+    plan.close()
+    achieved = macs / launch_s / 1e12
+    log(f"ragged: persistent+graph {launch_s * 1e3:.1f} ms ({items / launch_s:,.0f} modexp/s, {achieved:.2f} TMAC/s, "
+        f"{achieved / peak:.1%}); plain {pms:.1f} ms; spot-check mismatches {len(bad)}; plain == persistent: {same}")
+    if bad or not same:
+        raise RuntimeError(f"ragged results wrong: {bad[:5]} same={same}")
+    return {
+        "workload": f"{len(subs)} sub-batches, sizes log-uniform in [1, {args.ragged_max}], n 1024/2048-bit 50/50, "
+                    "full-modulus-width exponents",
+        "modexps": items, "macs": macs,
+        "tuned": best,
+        "persistent_graph_ms": round(launch_s * 1e3, 3),
+        "modexp_per_s": round(items / launch_s, 1),
+        "achieved_tmac_s": round(achieved, 4),
+        "frac_of_peak": round(achieved / peak, 4),
+        "wall_ms_per_run": round(wall * 1e3, 3),
+        "plain_per_subbatch_ms": round(pms, 3),
+        "plain_wall_ms": round(plain_wall * 1e3, 3),
+        "persistent_over_plain": round(pms / (launch_s * 1e3), 3),
+        "bit_exact": {"vs_gmp_items": checked, "mismatches": 0, "plain_equals_persistent": same},
+        "cpu_gmp_single_thread_modexp_s": round(checked / cpu_s, 1),
+    }
+
+
+class _TableRows:
+    """g_table[i][j] as integers, converted on access from the (chunks, 256, W) limb array."""
+
+    def __init__(self, arr):
+        self.arr = arr
+
+    def __getitem__(self, i):
+        from cuhepy_amd.limbs import limbs_to_int
+        row = self.arr[i]
+
+        class _Row:
+            def __getitem__(self, j):
+                return limbs_to_int(row[j])
+
+        return _Row()
+
+
+def lookup_leg(ctx, args, peak: float) -> dict:
+    """Paillier-lookup table encryption: 100,000 binary vectors of 512 bits
+    (Bernoulli(0.5)), key_len 1024 (n 2048-bit, n^2 4096-bit), alpha 280,
+    one ciphertext per vector, 14 noise factors; digits and noise indices
+    resident in HBM when the timed region starts."""
+    import numpy as np
+    import torch
+
+    from cuhepy_amd.engine import stride_for
+    from cuhepy_amd.limbs import limbs_to_ints, width_for_bits
+    from cuhepy_amd.paillier._lookup_gpu_ext import PaillierGPULookupClient
+    from cuhepy_amd.paillier.packing import embeddings_to_plaintexts, plaintexts_to_digits
+    from oracle import paillier_oracle as O
+
+    dev = torch.device("cuda:0")
+    t = time.perf_counter()
+    client = PaillierGPULookupClient(embed_len=512, key_len=1024, rng=random.Random(args.seed + 5))
+    key = client._device_key()
+    k = client._keys
+    mc = client._message_chunks
+    log(f"lookup: key n={k['n'].bit_length()} bits, a={k['a'].bit_length()} bits, tables built on the GPU "
+        f"({time.perf_counter() - t:.1f}s)")
+    count = args.lookup_vectors
+    nrng = np.random.default_rng(args.seed + 6)
+    bits = nrng.integers(0, 2, size=(count, 512), dtype=np.uint8)
+    W = width_for_bits(k["n_squared"].bit_length())
+    m = embeddings_to_plaintexts(bits, 1024, W)
+    digits = plaintexts_to_digits(m, mc)
+    nidx = nrng.integers(0, 256, size=(count, 14), dtype=np.uint8)
+    stride = stride_for(count)
+    d_dig = torch.from_numpy(digits).to(dev)
+    d_idx = torch.from_numpy(nidx).to(dev)
+    out = torch.zeros((W, stride), dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    key.encrypt_digits_dev(d_dig, d_idx, 14, out, count, stride)  # warm
+    torch.cuda.synchronize()
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.lookup_steps):
+        key.encrypt_digits_dev(d_dig, d_idx, 14, out, count, stride)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.lookup_steps
+    ms, nl = ctx.kernel_time("lookup_encrypt")
+    ctx.set_timing(False)
+    launch_s = ms / max(nl, 1) / 1e3
+    nnz = int((digits != 0).sum())
+    mulmods = nnz + 13 * count
+    macs = mulmods * 2 * W * W
+    achieved = macs / launch_s / 1e12
+    # round trip: every ciphertext decrypts to its plaintext (GPU decrypt)
+    aos = torch.empty((count, W), dtype=torch.int32, device=dev)
+    ctx.il_to_aos(out, aos, count, W, stride)
+    torch.cuda.synchronize()
+    c_aos = aos.cpu().numpy().view(np.uint32)
+    back = key.decrypt(c_aos)
+    rt_bad = int((back != m).any(axis=1).sum())
+    # formula check against the oracle on a sample (injected noise indices)
+    g_tab, noise = client._ensure_tables()
+    nt = limbs_to_ints(noise)
+    rows = _TableRows(g_tab)
+    pk = {"n": k["n"], "n_squared": k["n_squared"]}
+    sample = list(range(0, count, max(1, count // args.lookup_cpu_sample)))[: args.lookup_cpu_sample]
+    ms_int = limbs_to_ints(m[sample])
+    t = time.perf_counter()
+    want = [O.lookup_encrypt(mi, pk, rows, nt, mc, nidx[i].tolist()) for mi, i in zip(ms_int, sample)]
+    cpu_s = time.perf_counter() - t
+    got = limbs_to_ints(c_aos[sample])
+    f_bad = sum(int(a != b) for a, b in zip(got, want))
+    log(f"lookup: {count} vectors in {launch_s * 1e3:.2f} ms ({count / launch_s:,.0f} vectors/s, "
+        f"{mulmods / launch_s:,.0f} mulmod/s, {achieved:.2f} TMAC/s, {achieved / peak:.1%}); round-trip "
+        f"mismatches {rt_bad}; oracle mismatches {f_bad}/{len(sample)}; CPU {len(sample) / cpu_s:.1f} vectors/s")
+    if rt_bad or f_bad:
+        raise RuntimeError(f"lookup encryption wrong: round trip {rt_bad}, oracle {f_bad}")
+    # the client path end to end (host packing, transfer, kernels, integer conversion) on a slice
+    sl = min(count, args.lookup_client_vectors)
+    t = time.perf_counter()
+    client.encrypt(bits[:sl].tolist())
+    client_s = time.perf_counter() - t
+    return {
+        "workload": f"{count} binary vectors x 512 bits (Bernoulli 0.5), key_len 1024 (n^2 4096-bit), alpha 280, "
+                    "14 noise factors per ciphertext",
+        "vectors": count, "mulmods": mulmods, "macs": macs,
+        "kernel_ms": round(launch_s * 1e3, 3),
+        "vectors_per_s": round(count / launch_s, 1),
+        "mulmod_per_s": round(mulmods / launch_s, 1),
+        "achieved_tmac_s": round(achieved, 4),
+        "frac_of_peak": round(achieved / peak, 4),
+        "wall_ms_per_batch": round(wall * 1e3, 3),
+        "client_path_vectors_per_s": round(sl / client_s, 1),
+        "bit_exact": {"round_trip_vectors": count, "oracle_vectors": len(sample), "mismatches": 0},
+        "cpu_port_single_thread_vectors_s": round(len(sample) / cpu_s, 2),
+    }
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +319,15 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--tpi", type=int, default=0, help="force lanes per bignum (0 = automatic)")
     ap.add_argument("--window", type=int, default=0, help="force the window width (0 = automatic)")
+    ap.add_argument("--legs", default="ragged,lookup",
+                    help="extra workloads reported in the same line: ragged, lookup (comma list, '' = none)")
+    ap.add_argument("--ragged-subs", type=int, default=256)
+    ap.add_argument("--ragged-max", type=int, default=8192)
+    ap.add_argument("--ragged-steps", type=int, default=3)
+    ap.add_argument("--lookup-vectors", type=int, default=100000)
+    ap.add_argument("--lookup-steps", type=int, default=5)
+    ap.add_argument("--lookup-cpu-sample", type=int, default=24)
+    ap.add_argument("--lookup-client-vectors", type=int, default=10000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (counter calibration for profiles)")
     args = ap.parse_args()
@@ -328,6 +529,11 @@ def main() -> int:
                       f"{cpu['single_n']} of them",
             "single_thread": round(cpu["single"], 1),
         }
+    legs = [x for x in args.legs.split(",") if x]
+    if "ragged" in legs:
+        line["ragged"] = ragged_leg(ctx, args, peak_probe, torch.cuda.get_device_name(0))
+    if "lookup" in legs:
+        line["lookup_encrypt"] = lookup_leg(ctx, args, peak_probe)
     if args.copy_calibration:
         src = torch.ones(args.copy_calibration * (1 << 20) // 4, dtype=torch.int32, device=dev)
         dst = torch.empty_like(src)

@@ -132,9 +132,11 @@ class DeviceWorkload:
         (sub-batch, item, count, limbs, got-is-zero)."""
         powm = powm or pow
         bad = []
+        self.checked = 0
         for i, s in enumerate(self.subs):
             got = self.result(i)
             picks = {0, s.count - 1} | set(rng.sample(range(s.count), min(per_sub, s.count)))
+            self.checked += len(picks)
             for j in sorted(picks):
                 b = limbs_to_ints(s.bases[j:j + 1])[0]
                 e = limbs_to_ints(s.exps[j:j + 1])[0]
