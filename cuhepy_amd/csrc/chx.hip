@@ -862,6 +862,56 @@ int chx_mulmod_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* a, const uint
   });
 }
 
+int chx_mulmod_operand_words(const chx_mod* m) {
+  if (!m) return 0;
+  const auto& cfgs = lazy_cfgs(m->hc.L);
+  return cfgs.empty() ? 0 : cfgs[0].lp;
+}
+
+int chx_mulmod_operands_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* y, uint32_t* ops, size_t count) {
+  return with_ctx(ctx, [&] {
+    if (!m || !y || !ops) fail(CHX_EINVAL, "null argument");
+    if (count == 0) return;
+    const int L = m->hc.L;
+    CHX_DISPATCH_LAZY(L, 0, {
+      hipLaunchKernelGGL((k_lazy_table<RB, LP, TPI, L32>), grid_for(count, TPI), dim3(BLOCK_THREADS),
+                         lazy_lds_bytes(LP, L32, TPI), ctx->stream, m->lz.d[0], y, ops, (int)count);
+    });
+    check_launch("k_lazy_table");
+  });
+}
+
+int chx_mulmod_multi_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* x, size_t stride, size_t begin,
+                         size_t count, int nq, const uint32_t* const* ops, uint32_t* const* outs,
+                         size_t out_stride) {
+  return with_ctx(ctx, [&] {
+    if (!m || !x || !ops || !outs) fail(CHX_EINVAL, "null argument");
+    if (nq < 1 || nq > MQ_MAX) fail(CHX_EINVAL, "nq=%d out of range [1, %d]", nq, MQ_MAX);
+    if (count == 0) return;
+    if (begin + count > stride) fail(CHX_EINVAL, "range [%zu, %zu) exceeds the index stride %zu", begin,
+                                     begin + count, stride);
+    if (begin + count > out_stride) fail(CHX_EINVAL, "range exceeds the output stride %zu", out_stride);
+    if (stride > (size_t)INT32_MAX / 2 || out_stride > (size_t)INT32_MAX / 2) fail(CHX_EINVAL, "stride too large");
+    MultiQuery mq;
+    memset(&mq, 0, sizeof(mq));
+    mq.nq = nq;
+    for (int q = 0; q < nq; ++q) {
+      if (!ops[q] || !outs[q]) fail(CHX_EINVAL, "query %d: null pointer", q);
+      mq.op[q] = ops[q];
+      mq.out[q] = outs[q];
+    }
+    const int L = m->hc.L;
+    record_start(ctx, "mulmod_multi");
+    CHX_DISPATCH_LAZY(L, 0, {
+      hipLaunchKernelGGL((k_lazy_mulmod_multi<RB, LP, TPI, L32>), grid_for(count, TPI), dim3(BLOCK_THREADS),
+                         lazy_lds_bytes(LP, L32, TPI), ctx->stream, m->lz.d[0], x, (int)begin, (int)count,
+                         (int)stride, mq, (int)out_stride);
+    });
+    record_stop(ctx);
+    check_launch("k_lazy_mulmod_multi");
+  });
+}
+
 int chx_mulmod_bcast_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* x, const uint32_t* y_host,
                          uint32_t* out, size_t count, size_t stride) {
   return with_ctx(ctx, [&] {

@@ -155,6 +155,16 @@ struct Lazy {
     wave_lds_sync();
   }
 
+  // bit 32*L32 of a normalized value below 2^(32 L32 + 1), the same in every
+  // lane of the group (the carry word for cond_sub)
+  __device__ __forceinline__ u32 top32(const u32 (&x)[K]) {
+    constexpr int q = (32 * L32) / RB, off = (32 * L32) % RB;
+    static_assert(q < LP, "bit 32*L32 lies inside the positions");
+    const bool mine = grp::rank() == q / K;
+    const u32 b = mine ? (x[q % K] >> off) : 0u;
+    return grp::group_bits(__ballot(b != 0)) != 0ull ? 1u : 0u;
+  }
+
   // positions (normalized, value < 2^(32 L32)) -> 32-bit limbs (lane slice)
   __device__ __forceinline__ void to32(u32 (&w)[K32], const u32 (&x)[K]) {
     const int g = grp::gid(), r = grp::rank();
@@ -489,6 +499,64 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_modexp_each(const LazyDe
   const int it = item < count ? item : count - 1;
   modexp_each_item<RB, LP, TPI, L32>(M, md, in, exps, e_limbs, nwin, w, out, count, stride, table, tstride, item,
                                      it, item);
+}
+
+// ---------------------------------------------------------------------------
+// Served Hamming evaluation: many queries against one resident index
+// ---------------------------------------------------------------------------
+// encode_hamming_server (hamming/paillier.py:214-230; paillier_GPU_client.cu:986-991
+// over gpu_pointwise_mulmod_hex, cu:672-732; paillier_GPU_lookup_client.cu:2312-2329
+// over gpu_pointwise_mulmod_bigint, lcu:1506) multiplies a query ciphertext into every index
+// ciphertext mod n^2. One launch serves up to MQ_MAX queries over one range of
+// the index: each group loads its index entry once and multiplies it by every
+// query operand. Operands are y_q * R' mod N in positions (k_lazy_table), so one
+// almost-Montgomery product gives x * y_q mod N below 2N, made canonical by
+// one conditional subtraction that includes bit 32*L32.
+constexpr int MQ_MAX = 32;
+struct MultiQuery {
+  int nq;
+  const u32* op[MQ_MAX];  // LP positions each (device)
+  u32* out[MQ_MAX];       // interleaved [L32][out_stride], indexed by the global entry
+};
+
+template <int RB, int LP, int TPI, int L32>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_mulmod_multi(const LazyDev* __restrict__ md,
+                                                                     const u32* __restrict__ x, int begin,
+                                                                     int count, int stride, MultiQuery mq,
+                                                                     int out_stride) {
+  extern __shared__ uint4 smem4[];
+  typedef Lazy<RB, LP, TPI, L32> E;
+  constexpr int K = E::K, K32 = E::K32;
+  u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * (E::SLOTS * E::G);
+  E M;
+  M.init(area, md->N, md->n0);
+  const int local = group_item<TPI>();
+  const int item = begin + (local < count ? local : count - 1);
+  const int r = Grp<TPI>::rank();
+  u32 w[K32], xp[K], n32[K32];
+#pragma unroll
+  for (int k = 0; k < K32; ++k) w[k] = x[(size_t)(r * K32 + k) * stride + item];
+  M.from32(xp, w);
+#pragma unroll
+  for (int k = 0; k < K32; ++k) n32[k] = md->N32[r * K32 + k];
+  for (int q = 0; q < mq.nq; ++q) {
+    u32 y[K];
+    const u32* op = mq.op[q];
+#pragma unroll
+    for (int p = 0; p < K; ++p) y[p] = op[r * K + p];
+    M.stage(xp);
+    wave_lds_sync();
+    M.mul_staged(y);
+    wave_lds_sync();
+    const u32 hi = M.top32(y);
+    M.to32(w, y);
+    cond_sub<K32, TPI>(w, hi, n32);
+    if (local < count) {
+      u32* out = mq.out[q];
+#pragma unroll
+      for (int k = 0; k < K32; ++k) out[(size_t)(r * K32 + k) * out_stride + item] = w[k];
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------

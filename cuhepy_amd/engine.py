@@ -83,6 +83,11 @@ class Context:
     def set_stream(self, stream_ptr: Optional[int]) -> None:
         N.check(N.lib.chx_ctx_set_stream(self.handle, ctypes.c_void_p(stream_ptr or 0)))
 
+    @property
+    def stream_ptr(self) -> int:
+        """The hipStream_t the library enqueues on (0 = the null stream)."""
+        return int(N.lib.chx_ctx_get_stream(self.handle) or 0)
+
     def use_torch_stream(self) -> None:
         import torch
         self.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
@@ -201,6 +206,27 @@ class Modulus:
         Y = ints_to_limbs([y], self.limbs)
         N.check(N.lib.chx_mulmod_bcast_dev(self.ctx.handle, self.handle, _dptr(x), _p32(Y), _dptr(out), count,
                                            stride))
+
+    @property
+    def operand_words(self) -> int:
+        """Words of one multiplier operand for mulmod_multi_dev."""
+        return int(N.lib.chx_mulmod_operand_words(self.handle))
+
+    def operands_dev(self, y, ops, count: int) -> None:
+        """ops[j] = operand form of y[j] (device; y: count x limbs, ops: count x operand_words)."""
+        N.check(N.lib.chx_mulmod_operands_dev(self.ctx.handle, self.handle, _dptr(y), _dptr(ops), count))
+
+    def mulmod_multi_dev(self, x, stride: int, begin: int, count: int, ops: Sequence, outs: Sequence,
+                         out_stride: int) -> None:
+        """outs[q][:, i] = x[:, i] * y_q mod N for i in [begin, begin + count), one launch
+        (x and every outs[q] interleaved; ops[q] from operands_dev)."""
+        nq = len(ops)
+        if nq != len(outs):
+            raise ValueError("ops and outs differ in length")
+        P = (ctypes.c_void_p * max(nq, 1))(*[int(o.data_ptr()) for o in ops])
+        O = (ctypes.c_void_p * max(nq, 1))(*[int(o.data_ptr()) for o in outs])
+        N.check(N.lib.chx_mulmod_multi_dev(self.ctx.handle, self.handle, _dptr(x), stride, begin, count, nq, P, O,
+                                           out_stride))
 
     def modexp_dev(self, base, out, count: int, stride: int, e: int) -> None:
         E = ints_to_limbs([e], limb_count(e))

@@ -1,0 +1,313 @@
+"""Served encrypted kNN: a resident Hamming index, continuous batching, memory admission.
+
+The evaluator is encode_hamming_server (hamming/paillier.py:214-230, over the
+extensions' pointwise mulmod: paillier_GPU_client.cu:986-991,
+paillier_GPU_lookup_client.cu:2312-2329): a query ciphertext y is multiplied
+into every index ciphertext x_i mod n^2, giving the encrypted Hamming
+distances that the client decrypts. The reference runs one call per query
+from Python; here a request loop keeps the index resident in HBM and serves
+many queries at once:
+
+* cyclic scan: the index is cut into ``nchunks`` ranges; every scheduler
+  iteration launches ONE kernel that multiplies the next range by the
+  operands of all active queries (``Modulus.mulmod_multi_dev``), so the index
+  range is read once for all of them;
+* continuous batching: a query that arrives while others are on the GPU joins
+  at the next iteration, at whatever range the scan is on, and is complete
+  after ``nchunks`` iterations (it has then seen every range once); finished
+  queries leave at iteration boundaries;
+* memory admission: each active query holds a result buffer (one ciphertext
+  per index entry); a waiting query is admitted only when a released buffer
+  can be reused or free device memory (minus a reserve) covers a new one,
+  and at most ``max_active`` (<= 32 operands per launch) are active;
+* latency is measured to device completion (the event recorded after the
+  query's last iteration), from the request's arrival time.
+
+The scheduler and the admission policy are plain Python, independent of the
+device, so their invariants are tested on the CPU.
+"""
+
+from __future__ import annotations
+
+import collections
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Callable, Optional
+
+MQ_MAX = 32  # operands per launch (MultiQuery in the kernel)
+
+
+@dataclass
+class Request:
+    rid: int
+    y: int                      # query ciphertext (below n^2)
+    arrival: float              # host clock, seconds
+    keep: bool = False          # keep the result buffer for the caller
+    start_chunk: int = -1
+    left: int = 0
+    admitted: Optional[float] = None
+    done: Optional[float] = None  # device completion, host clock
+    result: object = None         # (limbs, stride) device tensor when kept
+    chunks_seen: list = field(default_factory=list)
+
+    @property
+    def latency(self) -> Optional[float]:
+        return None if self.done is None else self.done - self.arrival
+
+
+class MemoryAdmission:
+    """Admit a request when a pooled result buffer is free or free device
+    memory minus `reserve` covers a new one. `free_bytes` is injectable."""
+
+    def __init__(self, free_bytes: Callable[[], int], request_bytes: int, reserve: int = 0):
+        self.free_bytes = free_bytes
+        self.request_bytes = int(request_bytes)
+        self.reserve = int(reserve)
+        self.pooled = 0     # released buffers ready for reuse
+        self.held = 0       # buffers owned by active requests
+        self.allocated = 0  # buffers ever allocated and not trimmed
+
+    def try_reserve(self) -> Optional[str]:
+        """'pool' (reuse), 'new' (allocate one) or None (wait)."""
+        if self.pooled > 0:
+            self.pooled -= 1
+            self.held += 1
+            return "pool"
+        if self.free_bytes() - self.reserve >= self.request_bytes:
+            self.held += 1
+            self.allocated += 1
+            return "new"
+        return None
+
+    def release(self) -> None:
+        assert self.held > 0
+        self.held -= 1
+        self.pooled += 1
+
+
+class ScanScheduler:
+    """Continuous batching over a cyclic scan of `nchunks` index ranges."""
+
+    def __init__(self, nchunks: int, max_active: int = MQ_MAX):
+        if nchunks < 1:
+            raise ValueError("nchunks must be >= 1")
+        if not 1 <= max_active <= MQ_MAX:
+            raise ValueError(f"max_active must be in [1, {MQ_MAX}]")
+        self.nchunks = nchunks
+        self.max_active = max_active
+        self.waiting: collections.deque[Request] = collections.deque()
+        self.active: list[Request] = []
+        self.cursor = 0
+
+    def submit(self, req: Request) -> None:
+        self.waiting.append(req)
+
+    def admit(self, admission: MemoryAdmission, now: float, on_admit: Callable[[Request, str], None]) -> int:
+        """FIFO admission while slots and memory allow; returns how many joined."""
+        n = 0
+        while self.waiting and len(self.active) < self.max_active:
+            how = admission.try_reserve()
+            if how is None:
+                break
+            req = self.waiting.popleft()
+            req.start_chunk, req.left, req.admitted = self.cursor, self.nchunks, now
+            on_admit(req, how)
+            self.active.append(req)
+            n += 1
+        return n
+
+    def next_iteration(self) -> tuple[int, list[Request], list[Request]]:
+        """(range, queries in this launch, queries finishing with it)."""
+        if not self.active:
+            raise RuntimeError("no active request")
+        chunk = self.cursor
+        batch = list(self.active)
+        finished = []
+        for r in batch:
+            r.chunks_seen.append(chunk)
+            r.left -= 1
+            if r.left == 0:
+                finished.append(r)
+        self.active = [r for r in self.active if r.left > 0]
+        self.cursor = (self.cursor + 1) % self.nchunks
+        return chunk, batch, finished
+
+    def idle(self) -> bool:
+        return not self.waiting and not self.active
+
+
+class KnnServer:
+    """Resident encrypted index + the scheduler above driving the multi-query kernel.
+
+    index: device tensor (limbs, stride) int32, interleaved ciphertexts under
+    modulus n^2 (a ``Modulus``); `count` entries. One launch covers one range
+    of `chunk` entries for all active queries.
+    """
+
+    def __init__(self, ctx, modulus, index, count: int, chunk: Optional[int] = None, max_active: int = MQ_MAX,
+                 reserve_bytes: int = 4 << 30, free_bytes: Optional[Callable[[], int]] = None, depth: int = 2,
+                 clock: Callable[[], float] = time.perf_counter):
+        import torch
+
+        self.torch = torch
+        self.ctx, self.mod, self.index = ctx, modulus, index
+        self.limbs = modulus.limbs
+        self.count = int(count)
+        self.stride = int(index.shape[1])
+        if index.shape[0] != self.limbs or self.stride < self.count:
+            raise ValueError("index must be (limbs, stride >= count)")
+        self.chunk = int(chunk or self.count)
+        self.nchunks = math.ceil(self.count / self.chunk)
+        self.sched = ScanScheduler(self.nchunks, max_active)
+        self.device = torch.device(f"cuda:{ctx.device}")
+        self.result_bytes = self.limbs * self.stride * 4
+        if free_bytes is None:
+            free_bytes = lambda: torch.cuda.mem_get_info(self.device)[0]  # noqa: E731
+        self.admission = MemoryAdmission(free_bytes, self.result_bytes, reserve_bytes)
+        self.pool: list = []
+        self.depth = depth
+        self.clock = clock
+        self.stream = torch.cuda.ExternalStream(ctx.stream_ptr, device=self.device) if ctx.stream_ptr else \
+            torch.cuda.default_stream(self.device)
+        self.inflight: collections.deque = collections.deque()  # (event, finished requests)
+        self.completed: list[Request] = []
+        self.iterations = 0
+        self.launched_pairs = 0
+        self._next_id = 0
+        ctx.sync()
+        torch.cuda.synchronize(self.device)
+        self.base_event = torch.cuda.Event(enable_timing=True)
+        self.base_event.record(self.stream)
+        self.base_event.synchronize()
+        self.t_base = clock()
+
+    # ---- requests ----
+    def submit(self, y: int, arrival: Optional[float] = None, keep: bool = False) -> Request:
+        if not 0 <= y < (1 << (32 * self.limbs)):
+            raise ValueError("query ciphertext does not fit the modulus width")
+        req = Request(self._next_id, int(y), self.clock() if arrival is None else arrival, keep)
+        self._next_id += 1
+        self.sched.submit(req)
+        return req
+
+    def _on_admit(self, req: Request, how: str) -> None:
+        import numpy as np
+
+        from .limbs import ints_to_limbs
+        torch = self.torch
+        buf = self.pool.pop() if how == "pool" else torch.empty((self.limbs, self.stride), dtype=torch.int32,
+                                                                 device=self.device)
+        y = torch.from_numpy(ints_to_limbs([req.y], self.limbs).view(np.int32).copy()).to(self.device,
+                                                                                          non_blocking=False)
+        op = torch.empty((self.mod.operand_words,), dtype=torch.int32, device=self.device)
+        torch.cuda.current_stream(self.device).synchronize()
+        self.mod.operands_dev(y, op, 1)
+        req._buf, req._op, req._y = buf, op, y
+
+    # ---- loop ----
+    def _retire(self, block: bool) -> int:
+        n = 0
+        while self.inflight:
+            ev, finished = self.inflight[0]
+            if not block and not ev.query():
+                break
+            ev.synchronize()
+            self.inflight.popleft()
+            t = self.t_base + self.base_event.elapsed_time(ev) / 1e3
+            for r in finished:
+                r.done = t
+                if r.keep:
+                    r.result = r._buf
+                else:
+                    self.pool.append(r._buf)
+                    self.admission.release()
+                r._buf = r._op = r._y = None
+                self.completed.append(r)
+            n += 1
+            block = False
+        return n
+
+    def release(self, req: Request) -> None:
+        """Return a kept result buffer to the pool."""
+        if req.result is not None:
+            self.pool.append(req.result)
+            req.result = None
+            self.admission.release()
+
+    def pump(self) -> bool:
+        """Retire finished iterations, admit, and launch one iteration if the
+        queue depth allows; returns whether anything was launched."""
+        self._retire(block=False)
+        if len(self.inflight) >= self.depth:
+            return False
+        self.sched.admit(self.admission, self.clock(), self._on_admit)
+        if not self.sched.active:
+            return False
+        chunk, batch, finished = self.sched.next_iteration()
+        begin = chunk * self.chunk
+        cnt = min(self.chunk, self.count - begin)
+        self.mod.mulmod_multi_dev(self.index, self.stride, begin, cnt, [r._op for r in batch],
+                                  [r._buf for r in batch], self.stride)
+        ev = self.torch.cuda.Event(enable_timing=True)
+        ev.record(self.stream)
+        self.inflight.append((ev, finished))
+        self.iterations += 1
+        self.launched_pairs += len(batch) * cnt
+        return True
+
+    def drain(self) -> None:
+        while not self.sched.idle() or self.inflight:
+            if not self.pump():
+                if self.inflight:
+                    self._retire(block=True)
+                elif self.sched.waiting and not self.sched.active:
+                    raise RuntimeError("admission cannot make progress: not enough device memory for one query")
+
+    def run_trace(self, arrivals: list[tuple[float, int]], keep: Callable[[int], bool] = lambda i: False) -> list:
+        """Replay (offset seconds, query ciphertext) arrivals against the host
+        clock; returns the completed requests in arrival order."""
+        t0 = self.clock()
+        reqs = []
+        i = 0
+        while i < len(arrivals) or not self.sched.idle() or self.inflight:
+            now = self.clock()
+            while i < len(arrivals) and t0 + arrivals[i][0] <= now:
+                reqs.append(self.submit(arrivals[i][1], t0 + arrivals[i][0], keep(i)))
+                i += 1
+            launched = self.pump()
+            if not launched:
+                if self.inflight:
+                    # the device is busy: wait for the oldest iteration unless an arrival is due first
+                    nxt = t0 + arrivals[i][0] if i < len(arrivals) else math.inf
+                    if nxt > self.clock():
+                        ev = self.inflight[0][0]
+                        while not ev.query() and self.clock() < nxt:
+                            time.sleep(20e-6)
+                elif i < len(arrivals):
+                    wait = t0 + arrivals[i][0] - self.clock()
+                    if wait > 0:
+                        time.sleep(min(wait, 1e-3))
+        return reqs
+
+
+def poisson_arrivals(rate: float, n: int, rng) -> list[float]:
+    """n arrival offsets (seconds) of a Poisson process of `rate` per second."""
+    t, out = 0.0, []
+    for _ in range(n):
+        t += rng.expovariate(rate)
+        out.append(t)
+    return out
+
+
+def latency_stats(reqs: list[Request]) -> dict:
+    lat = sorted(r.latency for r in reqs if r.latency is not None)
+    if not lat:
+        return {"n": 0}
+
+    def pct(p):
+        k = min(len(lat) - 1, max(0, math.ceil(p / 100 * len(lat)) - 1))
+        return lat[k]
+
+    return {"n": len(lat), "p50_ms": pct(50) * 1e3, "p99_ms": pct(99) * 1e3, "mean_ms": sum(lat) / len(lat) * 1e3,
+            "max_ms": lat[-1] * 1e3}

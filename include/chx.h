@@ -90,7 +90,23 @@ int chx_mulmod(chx_ctx* ctx, const chx_mod* m, const uint32_t* a, const uint32_t
                size_t count);
 int chx_mulmod_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* a, const uint32_t* b, uint32_t* out,
                    size_t count, size_t stride);
-/* out[i] = x[i]*y mod n with y shared (the served-kNN evaluator form) */
+/* ---- served Hamming evaluation (many queries against one resident index) ----
+ * encode_hamming_server multiplies a query ciphertext into each index ciphertext
+ * mod n^2 (paillier_GPU_client.cu:986-991 -> gpu_pointwise_mulmod_hex cu:672-732;
+ * paillier_GPU_lookup_client.cu:2312-2329 -> gpu_pointwise_mulmod_bigint lcu:1506).
+ * A query is first turned into an operand of chx_mulmod_operand_words(m) words
+ * (y: device, count x limbs, any value below 2^(32 limbs); ops: device, count x
+ * words). chx_mulmod_multi_dev then computes, in one launch,
+ *   outs[q][i] = x[i] * y_q mod n   for i in [begin, begin + count), q < nq <= 32
+ * with x interleaved [limbs][stride] and every outs[q] interleaved
+ * [limbs][out_stride], indexed by the global entry i. ops and outs are host
+ * arrays of nq device pointers. Results are canonical (below n). */
+int chx_mulmod_operand_words(const chx_mod* m);
+int chx_mulmod_operands_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* y, uint32_t* ops, size_t count);
+int chx_mulmod_multi_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* x, size_t stride, size_t begin,
+                         size_t count, int nq, const uint32_t* const* ops, uint32_t* const* outs,
+                         size_t out_stride);
+/* out[i] = x[i]*y mod n with y shared (one query, canonical engine) */
 int chx_mulmod_bcast_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* x, const uint32_t* y_host,
                          uint32_t* out, size_t count, size_t stride);
 
