@@ -305,6 +305,151 @@ def lookup_leg(ctx, args, peak: float) -> dict:
         "cpu_port_single_thread_vectors_s": round(len(sample) / cpu_s, 2),
     }
 
+
+def serve_leg(ctx, args, peak: float) -> dict:
+    """Served encrypted kNN: 1,000,000 lookup-encrypted 512-bit binary vectors
+    (key_len 1024, n^2 4096-bit, 512 B per ciphertext) resident in HBM; each
+    query (an encrypted 512-bit vector) is multiplied into every index entry
+    (encode_hamming_server). Reported: the single-query kernel against both
+    rooflines, saturation throughput, and p50/p99 latency (arrival to device
+    completion) of Poisson arrivals at a sweep of rates, for continuous
+    batching and for the plain one-launch-per-query FIFO form."""
+    import numpy as np
+    import torch
+
+    from cuhepy_amd.engine import Modulus, stride_for
+    from cuhepy_amd.limbs import limbs_to_ints, width_for_bits
+    from cuhepy_amd.paillier._lookup_gpu_ext import PaillierGPULookupClient
+    from cuhepy_amd.paillier.packing import embeddings_to_plaintexts, plaintexts_to_digits
+    from cuhepy_amd.serve import KnnServer, latency_stats, poisson_arrivals
+
+    dev = torch.device("cuda:0")
+    t = time.perf_counter()
+    client = PaillierGPULookupClient(embed_len=512, key_len=1024, rng=random.Random(args.seed + 7))
+    key = client._device_key()
+    k = client._keys
+    n2 = k["n_squared"]
+    W = width_for_bits(n2.bit_length())
+    mc = client._message_chunks
+    count = args.serve_index
+    stride = stride_for(count)
+    index = torch.empty((W, stride), dtype=torch.int32, device=dev)
+    nrng = np.random.default_rng(args.seed + 8)
+    batch = 100_000
+    tmp = torch.empty((W, stride_for(batch)), dtype=torch.int32, device=dev)
+    keep_bits = None
+    for off in range(0, count, batch):
+        cnt = min(batch, count - off)
+        bits = nrng.integers(0, 2, size=(cnt, 512), dtype=np.uint8)
+        if keep_bits is None:
+            keep_bits = bits[:256].copy()
+        digits = plaintexts_to_digits(embeddings_to_plaintexts(bits, 1024, W), mc)
+        nidx = nrng.integers(0, 256, size=(cnt, 14), dtype=np.uint8)
+        d_dig, d_idx = torch.from_numpy(digits).to(dev), torch.from_numpy(nidx).to(dev)
+        torch.cuda.synchronize()
+        key.encrypt_digits_dev(d_dig, d_idx, 14, tmp, cnt, tmp.shape[1])
+        torch.cuda.synchronize()
+        index[:, off:off + cnt] = tmp[:, :cnt]
+    torch.cuda.synchronize()
+    del tmp
+    log(f"serve: index of {count} encrypted 512-bit vectors resident ({W * stride * 4 / 2**20:.0f} MiB, "
+        f"{time.perf_counter() - t:.1f}s)")
+    mod = Modulus(ctx, n2, W)
+    qrng = np.random.default_rng(args.seed + 9)
+
+    def queries(nq):
+        qb = qrng.integers(0, 2, size=(nq, 512), dtype=np.uint8)
+        return qb, [int(c[0]) for c in client.encrypt(qb.tolist())]
+
+    # 1) one query alone, whole index in one launch: the kernel against its rooflines
+    qb, ys = queries(1)
+    plain = KnnServer(ctx, mod, index, count, chunk=count, max_active=1)
+    r = plain.submit(ys[0], keep=True)
+    plain.drain()
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    reps = 5
+    for _ in range(reps):
+        plain.submit(ys[0])
+    plain.drain()
+    ms, nl = ctx.kernel_time("mulmod_multi")
+    ctx.set_timing(False)
+    kern_s = ms / max(nl, 1) / 1e3
+    macs = count * 2 * W * W
+    hbm_bytes = count * 2 * W * 4
+    # bit-exact: sampled entries against exact integers, decoded distances against the plaintexts
+    aos = torch.empty((256, W), dtype=torch.int32, device=dev)
+    idx_aos = torch.empty((256, W), dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    ctx.il_to_aos(r.result, aos, 256, W, stride)
+    ctx.il_to_aos(index, idx_aos, 256, W, stride)
+    ctx.sync()
+    got = limbs_to_ints(aos.cpu().numpy().view(np.uint32))
+    xs = limbs_to_ints(idx_aos.cpu().numpy().view(np.uint32))
+    exact_bad = sum(int(g != x * ys[0] % n2) for g, x in zip(got, xs))
+    dist = client.decode_hamming_client([[c] for c in got])
+    want = [int((b != qb[0]).sum()) for b in keep_bits]
+    dist_bad = sum(int(a != b) for a, b in zip(dist, want))
+    plain.release(r)
+    if exact_bad or dist_bad:
+        raise RuntimeError(f"served evaluation wrong: {exact_bad} products, {dist_bad} distances")
+    log(f"serve: one query = {count} mulmods in {kern_s * 1e3:.3f} ms ({count / kern_s:,.0f} mulmod/s, "
+        f"{macs / kern_s / 1e12:.2f} TMAC/s = {macs / kern_s / 1e12 / peak:.1%} of the integer peak, "
+        f"{hbm_bytes / kern_s / 1e9:.0f} GB/s = {hbm_bytes / kern_s / 8e12:.1%} of HBM); 256 entries exact, "
+        f"256 distances decoded correctly")
+
+    # 2) saturation: 32 queries at once through the batched loop
+    srv = KnnServer(ctx, mod, index, count, chunk=args.serve_chunk, max_active=32)
+    _, ys = queries(64)
+    t0 = time.perf_counter()
+    for y in ys:
+        srv.submit(y)
+    srv.drain()
+    sat = len(ys) / (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    for y in ys[:16]:
+        plain.submit(y)
+    plain.drain()
+    sat_plain = 16 / (time.perf_counter() - t0)
+    log(f"serve: saturation {sat:.1f} queries/s batched ({srv.iterations} launches), {sat_plain:.1f} plain")
+
+    # 3) Poisson arrivals at a sweep of rates
+    sweep = []
+    rng = random.Random(args.seed + 10)
+    _, pool_ys = queries(args.serve_queries)
+    for frac in (0.25, 0.5, 0.75, 0.9):
+        rate = frac * sat
+        offs = poisson_arrivals(rate, args.serve_queries, rng)
+        row = {"rate_qps": round(rate, 2)}
+        for name, server in (("batched", srv), ("plain", plain)):
+            server.completed.clear()
+            t0 = time.perf_counter()
+            reqs = server.run_trace(list(zip(offs, pool_ys)))
+            wall = time.perf_counter() - t0
+            st = latency_stats(reqs)
+            row[name] = {"p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
+                         "mean_ms": round(st["mean_ms"], 3), "throughput_qps": round(len(reqs) / wall, 2)}
+        sweep.append(row)
+        log(f"serve: {rate:7.1f} q/s: batched p50 {row['batched']['p50_ms']:.2f} p99 {row['batched']['p99_ms']:.2f} "
+            f"ms | plain p50 {row['plain']['p50_ms']:.2f} p99 {row['plain']['p99_ms']:.2f} ms")
+    return {
+        "workload": f"{count} lookup-encrypted 512-bit vectors (key_len 1024, n^2 4096-bit) resident in HBM; "
+                    "each query = one mulmod per index entry (encode_hamming_server)",
+        "index_entries": count,
+        "single_query_kernel_ms": round(kern_s * 1e3, 4),
+        "mulmod_per_s": round(count / kern_s, 1),
+        "roofline_valu": {"achieved": round(macs / kern_s / 1e12, 4), "peak": round(peak, 4),
+                          "unit": "TMAC/s", "frac": round(macs / kern_s / 1e12 / peak, 4)},
+        "roofline_hbm": {"achieved": round(hbm_bytes / kern_s / 1e9, 1), "peak": 8000.0, "unit": "GB/s",
+                         "frac": round(hbm_bytes / kern_s / 8e12, 4), "bytes_per_query": hbm_bytes},
+        "saturation_qps_batched": round(sat, 2),
+        "saturation_qps_plain": round(sat_plain, 2),
+        "chunk": args.serve_chunk, "max_active": 32,
+        "latency_endpoint": "device-complete (event after the query's last range), from arrival",
+        "poisson_sweep": sweep,
+        "bit_exact": {"entries_checked": 256, "distances_checked": 256, "mismatches": 0},
+    }
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -319,8 +464,11 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--tpi", type=int, default=0, help="force lanes per bignum (0 = automatic)")
     ap.add_argument("--window", type=int, default=0, help="force the window width (0 = automatic)")
-    ap.add_argument("--legs", default="ragged,lookup",
-                    help="extra workloads reported in the same line: ragged, lookup (comma list, '' = none)")
+    ap.add_argument("--legs", default="ragged,lookup,serve",
+                    help="extra workloads reported in the same line: ragged, lookup, serve (comma list, '' = none)")
+    ap.add_argument("--serve-index", type=int, default=1_000_000)
+    ap.add_argument("--serve-chunk", type=int, default=131072)
+    ap.add_argument("--serve-queries", type=int, default=200)
     ap.add_argument("--ragged-subs", type=int, default=256)
     ap.add_argument("--ragged-max", type=int, default=8192)
     ap.add_argument("--ragged-steps", type=int, default=3)
@@ -534,6 +682,8 @@ def main() -> int:
         line["ragged"] = ragged_leg(ctx, args, peak_probe, torch.cuda.get_device_name(0))
     if "lookup" in legs:
         line["lookup_encrypt"] = lookup_leg(ctx, args, peak_probe)
+    if "serve" in legs:
+        line["served_knn"] = serve_leg(ctx, args, peak_probe)
     if args.copy_calibration:
         src = torch.ones(args.copy_calibration * (1 << 20) // 4, dtype=torch.int32, device=dev)
         dst = torch.empty_like(src)
