@@ -135,7 +135,11 @@ def ragged_leg(ctx, args, peak: float, device_name: str) -> dict:
     log(f"ragged: {len(subs)} sub-batches, {items} modexps ({n64} at 2048-bit, {items - n64} at 4096-bit moduli), "
         f"{macs:.3e} MACs, uploaded ({time.perf_counter() - t:.1f}s)")
     t = time.perf_counter()
-    best, tlog = tune_ragged(ctx, wl, device_name, reps=1, force=True)
+    if args.ragged_fixed:
+        v, w, wpc = (int(x) for x in args.ragged_fixed.split(","))
+        best, tlog = {"variant": v, "window": w, "waves": wpc}, []
+    else:
+        best, tlog = tune_ragged(ctx, wl, device_name, reps=1, force=True)
     log(f"ragged autotune ({time.perf_counter() - t:.1f}s): best {best}; " +
         ", ".join(f"v{c['variant']}/w{c['window']}/{c['waves']}wpc {s * 1e3:.0f}ms" for c, s in tlog))
     plan = RaggedPlan(ctx, wl.desc, best["variant"], best["window"], best["waves"])
@@ -472,6 +476,8 @@ def main() -> int:
     ap.add_argument("--ragged-subs", type=int, default=256)
     ap.add_argument("--ragged-max", type=int, default=8192)
     ap.add_argument("--ragged-steps", type=int, default=3)
+    ap.add_argument("--ragged-fixed", default="", metavar="VARIANT,WINDOW,WAVES_PER_CU",
+                    help="skip the autotuner and use this launch shape")
     ap.add_argument("--lookup-vectors", type=int, default=100000)
     ap.add_argument("--lookup-steps", type=int, default=5)
     ap.add_argument("--lookup-cpu-sample", type=int, default=24)

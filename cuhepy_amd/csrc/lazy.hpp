@@ -26,6 +26,16 @@ namespace chx {
 
 __device__ __forceinline__ u64 make64(u32 lo, u32 hi) { return ((u64)hi << 32) | lo; }
 
+// An opaque copy of v: the compiler cannot hoist what is computed from it out
+// of an enclosing loop. The lane-dependent bit offsets and addresses of
+// from32/to32/stage are derived from it, so a persistent loop around a whole
+// exponentiation does not keep dozens of them live in VGPRs across every tile
+// (with TPI = 1 the rank is the constant 0 and stays visible to the compiler).
+__device__ __forceinline__ int opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 template <int RB, int LP, int TPI, int L32>
 struct Lazy {
   static constexpr int K = LP / TPI;
@@ -61,7 +71,7 @@ struct Lazy {
 
   // stage this lane's positions as the streamed operand: row i = position i
   __device__ __forceinline__ void stage(const u32 (&x)[K]) {
-    const int g = grp::gid(), r = grp::rank();
+    const int g = TPI == 1 ? grp::gid() : opaque(grp::gid()), r = TPI == 1 ? 0 : opaque(grp::rank());
 #pragma unroll
     for (int p = 0; p < K; ++p) lds[(r * K + p) * G + g] = x[p];
   }
@@ -140,7 +150,7 @@ struct Lazy {
 
   // 32-bit limbs (lane slice of K32, value < 2^(32 L32)) -> positions
   __device__ __forceinline__ void from32(u32 (&x)[K], const u32 (&w)[K32]) {
-    const int g = grp::gid(), r = grp::rank();
+    const int g = TPI == 1 ? grp::gid() : opaque(grp::gid()), r = TPI == 1 ? 0 : opaque(grp::rank());
 #pragma unroll
     for (int k = 0; k < K32; ++k) lds[(r * K32 + k) * G + g] = w[k];
     wave_lds_sync();
@@ -167,7 +177,7 @@ struct Lazy {
 
   // positions (normalized, value < 2^(32 L32)) -> 32-bit limbs (lane slice)
   __device__ __forceinline__ void to32(u32 (&w)[K32], const u32 (&x)[K]) {
-    const int g = grp::gid(), r = grp::rank();
+    const int g = TPI == 1 ? grp::gid() : opaque(grp::gid()), r = TPI == 1 ? 0 : opaque(grp::rank());
     stage(x);
     wave_lds_sync();
 #pragma unroll
@@ -406,7 +416,7 @@ __device__ __forceinline__ void modexp_each_item(Lazy<RB, LP, TPI, L32>& M, cons
                                                  int slot) {
   typedef Lazy<RB, LP, TPI, L32> E;
   constexpr int K = E::K, K32 = E::K32;
-  const int r = Grp<TPI>::rank();
+  const int r = TPI == 1 ? 0 : opaque(Grp<TPI>::rank());
   auto tstore = [&](int e, const u32 (&v)[K]) {
 #pragma unroll
     for (int p = 0; p < K; ++p) table[((size_t)e * LP + r * K + p) * tstride + slot] = v[p];
