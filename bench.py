@@ -469,7 +469,7 @@ def main() -> int:
     ap.add_argument("--tpi", type=int, default=0, help="force lanes per bignum (0 = automatic)")
     ap.add_argument("--window", type=int, default=0, help="force the window width (0 = automatic)")
     ap.add_argument("--legs", default="ragged,lookup,serve",
-                    help="extra workloads reported in the same line: ragged, lookup, serve (comma list, '' = none)")
+                    help="extra workloads reported in the same line: ragged, lookup, serve (comma list, or none)")
     ap.add_argument("--serve-index", type=int, default=1_000_000)
     ap.add_argument("--serve-chunk", type=int, default=131072)
     ap.add_argument("--serve-queries", type=int, default=200)
@@ -683,7 +683,7 @@ def main() -> int:
                       f"{cpu['single_n']} of them",
             "single_thread": round(cpu["single"], 1),
         }
-    legs = [x for x in args.legs.split(",") if x]
+    legs = [x for x in args.legs.split(",") if x and x != "none"]
     if "ragged" in legs:
         line["ragged"] = ragged_leg(ctx, args, peak_probe, torch.cuda.get_device_name(0))
     if "lookup" in legs:
@@ -696,7 +696,18 @@ def main() -> int:
         for _ in range(3):
             dst.copy_(src)
         torch.cuda.synchronize()
-        log(f"copy calibration: 3 device copies of {args.copy_calibration} MiB")
+        # the same bytes with the lane-group pattern of the exponentiation and
+        # mulmod kernels (k_copy_groups), one launch per (limbs, lanes) shape
+        for limbs, tpi in ((64, 4), (128, 8)):
+            cnt = args.copy_calibration * (1 << 20) // (4 * limbs)
+            st = stride_for(cnt)
+            a = torch.ones((limbs, st), dtype=torch.int32, device=dev)
+            b = torch.empty_like(a)
+            torch.cuda.synchronize()
+            ctx.probe_copy(a, b, cnt, st, limbs, tpi)
+            torch.cuda.synchronize()
+            del a, b
+        log(f"copy calibration: 3 device copies of {args.copy_calibration} MiB, lane-group copies (64, 4), (128, 8)")
     print(json.dumps(line), flush=True)
     return 0
 

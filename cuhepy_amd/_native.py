@@ -107,6 +107,7 @@ SIGNATURES = {
     "chx_host_mont_consts": (c_int, [u32p, c_int, u32p, u32p]),
     "chx_probe_mad": (c_int, [vp, c_int, c_int, POINTER(c_double), POINTER(c_double)]),
     "chx_probe_rate": (c_int, [vp, c_int, c_int, c_int, POINTER(c_double), POINTER(c_double)]),
+    "chx_probe_copy": (c_int, [vp, vp, vp, c_size_t, c_size_t, c_int, c_int]),
     "chx_ctx_set_timing": (c_int, [vp, c_int]),
     "chx_ctx_kernel_time": (c_int, [vp, c_char_p, POINTER(c_double), POINTER(c_int)]),
     "chx_ctx_reset_timing": (c_int, [vp]),

@@ -1624,6 +1624,29 @@ int chx_probe_rate(chx_ctx* ctx, int kind, int blocks, int iters, double* second
   });
 }
 
+int chx_probe_copy(chx_ctx* ctx, const uint32_t* src, uint32_t* dst, size_t count, size_t stride, int limbs,
+                   int tpi) {
+  return with_ctx(ctx, [&] {
+    if (!src || !dst) fail(CHX_EINVAL, "null argument");
+    if (count == 0) return;
+    if (stride < round_up(count, 64)) fail(CHX_EINVAL, "stride %zu below count rounded to 64", stride);
+    bool done = false;
+#define CHX_COPY_CASE(L_, T_)                                                                                  \
+    if (limbs == L_ && tpi == T_) {                                                                            \
+      hipLaunchKernelGGL((k_copy_groups<T_, L_>), grid_for(count, T_), dim3(BLOCK_THREADS), 0, ctx->stream, src, \
+                         dst, (int)count, (int)stride);                                                        \
+      done = true;                                                                                             \
+    }
+    CHX_COPY_CASE(64, 4)
+    CHX_COPY_CASE(64, 8)
+    CHX_COPY_CASE(128, 8)
+    CHX_COPY_CASE(128, 16)
+#undef CHX_COPY_CASE
+    if (!done) fail(CHX_EUNSUPPORTED, "copy probe built for (64, 4|8) and (128, 8|16) only");
+    check_launch("k_copy_groups");
+  });
+}
+
 int chx_probe_mad(chx_ctx* ctx, int blocks, int iters, double* seconds, double* macs) {
   return chx_probe_rate(ctx, 0, blocks, iters, seconds, macs);
 }

@@ -406,6 +406,25 @@ __global__ void k_il_to_aos(const u32* __restrict__ il, u32* __restrict__ aos, i
 }
 
 // ---------------------------------------------------------------------------
+// Calibration copy for the HBM byte counters: copies an interleaved batch
+// with the access pattern of the exponentiation and mulmod kernels (TPI lanes
+// per item, K32 consecutive limb rows per lane, so one load instruction touches
+// 64/TPI consecutive items of TPI rows). Its byte count is known exactly, so
+// FETCH_SIZE / WRITE_SIZE of this kernel give the counters' scale for requests
+// of that shape.
+// ---------------------------------------------------------------------------
+template <int TPI, int L32>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_copy_groups(const u32* __restrict__ src, u32* __restrict__ dst,
+                                                               int count, int stride) {
+  constexpr int K32 = L32 / TPI;
+  const int item = group_item<TPI>();
+  if (item >= count) return;
+  const int r = Grp<TPI>::rank();
+#pragma unroll
+  for (int k = 0; k < K32; ++k) dst[(size_t)(r * K32 + k) * stride + item] = src[(size_t)(r * K32 + k) * stride + item];
+}
+
+// ---------------------------------------------------------------------------
 // Instruction-rate probes: `iters` rounds of 16 independent operations per
 // lane (8 chains x 2); the host times them to pin issue rates on the device.
 //   kind 0: v_mad_u64_u32   (u64)a*b + acc

@@ -130,6 +130,10 @@ class Context:
         return s.value, m.value
 
     # layout conversion on device
+    def probe_copy(self, src, dst, count: int, stride: int, limbs: int, tpi: int) -> None:
+        """Calibration copy with the lane-group access pattern (byte counters)."""
+        N.check(N.lib.chx_probe_copy(self.handle, _dptr(src), _dptr(dst), count, stride, limbs, tpi))
+
     def aos_to_il(self, aos, il, count: int, limbs: int, stride: int) -> None:
         N.check(N.lib.chx_aos_to_il_dev(self.handle, _dptr(aos), _dptr(il), count, limbs, stride))
 

@@ -233,6 +233,11 @@ int chx_probe_mad(chx_ctx* ctx, int blocks, int iters, double* seconds, double* 
 /* same for other instruction kinds: 0 v_mad_u64_u32, 1 v_mul_lo_u32, 2 v_mul_hi_u32,
  * 3 64-bit add, 4 v_fma_f64, 5 v_mad_u32_u24; *ops = operations executed */
 int chx_probe_rate(chx_ctx* ctx, int kind, int blocks, int iters, double* seconds, double* ops);
+/* calibration copy of an interleaved batch (device, [limbs][stride]) with the
+ * lane-group access pattern of the exponentiation kernels: limbs 64 with tpi 4|8,
+ * limbs 128 with tpi 8|16; moves exactly count*limbs*4 bytes each way */
+int chx_probe_copy(chx_ctx* ctx, const uint32_t* src, uint32_t* dst, size_t count, size_t stride, int limbs,
+                   int tpi);
 /* HIP-event timing of the compute kernels enqueued by this context (events
  * recorded on the context's stream around every launch while enabled) */
 int chx_ctx_set_timing(chx_ctx* ctx, int enable);
