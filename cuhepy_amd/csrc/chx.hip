@@ -862,10 +862,14 @@ int chx_mulmod_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* a, const uint
   });
 }
 
+// Lazy configuration of the served multiply (operands are prepared in its
+// positions): configuration 0, measured faster than the 16-lane shape at 4096 bits.
+static constexpr int kServeCfg = 0;
+
 int chx_mulmod_operand_words(const chx_mod* m) {
   if (!m) return 0;
   const auto& cfgs = lazy_cfgs(m->hc.L);
-  return cfgs.empty() ? 0 : cfgs[0].lp;
+  return cfgs.empty() ? 0 : cfgs[kServeCfg].lp;
 }
 
 int chx_mulmod_operands_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* y, uint32_t* ops, size_t count) {
@@ -873,9 +877,9 @@ int chx_mulmod_operands_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* y, u
     if (!m || !y || !ops) fail(CHX_EINVAL, "null argument");
     if (count == 0) return;
     const int L = m->hc.L;
-    CHX_DISPATCH_LAZY(L, 0, {
+    CHX_DISPATCH_LAZY(L, kServeCfg, {
       hipLaunchKernelGGL((k_lazy_table<RB, LP, TPI, L32>), grid_for(count, TPI), dim3(BLOCK_THREADS),
-                         lazy_lds_bytes(LP, L32, TPI), ctx->stream, m->lz.d[0], y, ops, (int)count);
+                         lazy_lds_bytes(LP, L32, TPI), ctx->stream, m->lz.d[kServeCfg], y, ops, (int)count);
     });
     check_launch("k_lazy_table");
   });
@@ -902,9 +906,9 @@ int chx_mulmod_multi_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* x, size
     }
     const int L = m->hc.L;
     record_start(ctx, "mulmod_multi");
-    CHX_DISPATCH_LAZY(L, 0, {
+    CHX_DISPATCH_LAZY(L, kServeCfg, {
       hipLaunchKernelGGL((k_lazy_mulmod_multi<RB, LP, TPI, L32>), grid_for(count, TPI), dim3(BLOCK_THREADS),
-                         lazy_lds_bytes(LP, L32, TPI), ctx->stream, m->lz.d[0], x, (int)begin, (int)count,
+                         lazy_lds_bytes(LP, L32, TPI), ctx->stream, m->lz.d[kServeCfg], x, (int)begin, (int)count,
                          (int)stride, mq, (int)out_stride);
     });
     record_stop(ctx);
