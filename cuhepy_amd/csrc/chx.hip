@@ -177,7 +177,9 @@ const std::vector<int>& built_tpi(int limbs) {
 
 // Number of lanes that keeps every SIMD of the chip busy with 2 waves.
 size_t target_lanes(int device) {
+  static std::mutex mu;
   static std::map<int, size_t> cache;
+  std::lock_guard<std::mutex> lock(mu);  // contexts of different threads share the cache
   auto it = cache.find(device);
   if (it != cache.end()) return it->second;
   hipDeviceProp_t prop;
@@ -1597,7 +1599,7 @@ int chx_host_mont_consts(const uint32_t* n, int limbs, uint32_t* n0inv, uint32_t
 
 int chx_probe_rate(chx_ctx* ctx, int kind, int blocks, int iters, double* seconds, double* ops) {
   return with_ctx(ctx, [&] {
-    if (blocks <= 0 || iters <= 0 || !seconds || !ops || kind < 0 || kind > 5) fail(CHX_EINVAL, "bad argument");
+    if (blocks <= 0 || iters <= 0 || !seconds || !ops || kind < 0 || kind > 7) fail(CHX_EINVAL, "bad argument");
     ctx->slot[S_AUX].ensure((size_t)blocks * 256 * 8);
     u64* sink = ctx->slot[S_AUX].as<u64>();
     auto launch = [&](int it) {
@@ -1607,7 +1609,9 @@ int chx_probe_rate(chx_ctx* ctx, int kind, int blocks, int iters, double* second
         case 2: hipLaunchKernelGGL(k_probe<2>, dim3(blocks), dim3(256), 0, ctx->stream, sink, it, 7u); break;
         case 3: hipLaunchKernelGGL(k_probe<3>, dim3(blocks), dim3(256), 0, ctx->stream, sink, it, 7u); break;
         case 4: hipLaunchKernelGGL(k_probe<4>, dim3(blocks), dim3(256), 0, ctx->stream, sink, it, 7u); break;
-        default: hipLaunchKernelGGL(k_probe<5>, dim3(blocks), dim3(256), 0, ctx->stream, sink, it, 7u); break;
+        case 5: hipLaunchKernelGGL(k_probe<5>, dim3(blocks), dim3(256), 0, ctx->stream, sink, it, 7u); break;
+        case 6: hipLaunchKernelGGL(k_probe<6>, dim3(blocks), dim3(256), 0, ctx->stream, sink, it, 7u); break;
+        default: hipLaunchKernelGGL(k_probe<7>, dim3(blocks), dim3(256), 0, ctx->stream, sink, it, 7u); break;
       }
     };
     hipEvent_t a, b;

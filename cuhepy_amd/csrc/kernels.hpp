@@ -433,6 +433,8 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_copy_groups(const u32* __rest
 //   kind 3: 64-bit add      acc + v
 //   kind 4: v_fma_f64
 //   kind 5: v_mad_u32_u24   24x24 + 32
+//   kind 6: v_mad_u64_u32 with an SGPR-pair carry-out (the compiler's form; kind 0 writes VCC)
+//   kind 7: kind 6 plus one v_add_u32 per two MADs (ops counted = MADs)
 // ---------------------------------------------------------------------------
 // The operations are written as inline asm so the compiler can neither hoist
 // nor fold them; each asm statement is one ALU instruction on registers (no
@@ -463,6 +465,15 @@ __global__ __launch_bounds__(256) void k_probe(u64* __restrict__ sink, int iters
         if constexpr (KIND == 3) asm volatile("v_lshl_add_u64 %0, %0, 0, %1" : "+v"(acc[c]) : "v"(b64));
         if constexpr (KIND == 4) asm volatile("v_fma_f64 %0, %0, %1, %2" : "+v"(d[c]) : "v"(x), "v"(y));
         if constexpr (KIND == 5) asm volatile("v_mad_u32_u24 %0, %0, %1, %0" : "+v"(w[c]) : "v"(b));
+        if constexpr (KIND == 6) {  // v_mad_u64_u32 with the carry-out in an SGPR pair (as compiled code does)
+          u64 cy;
+          asm volatile("v_mad_u64_u32 %0, %1, %2, %3, %0" : "+v"(acc[c]), "=s"(cy) : "v"(w[c]), "v"(b));
+        }
+        if constexpr (KIND == 7) {  // 2 such MADs + 1 independent 32-bit add: issue-rate mix
+          u64 cy;
+          asm volatile("v_mad_u64_u32 %0, %1, %2, %3, %0" : "+v"(acc[c]), "=s"(cy) : "v"(w[c]), "v"(b));
+          if (c & 1) asm volatile("v_add_u32 %0, %0, %1" : "+v"(w[c]) : "v"(b));
+        }
       }
     }
   }

@@ -4,8 +4,11 @@
 // accumulators. A limb product is < 2^(2 RB), and a position receives at most
 // 2 products per row for LP rows, so 2 * LP * 2^(2 RB) < 2^64 and no carry has
 // to be propagated inside the product: every limb product is exactly one
-// in-place v_mad_u64_u32 (64-bit accumulator as the addend), which issues at
-// the full VALU rate on CDNA4. Carries are resolved once per product.
+// in-place v_mad_u64_u32 (64-bit accumulator as the addend). Carries are
+// resolved once per product. A wider radix (RB = 29: 10% fewer positions at
+// 2048 bits) overflows after 31 rows and needs carry folds inside the product
+// (`fold`, supported below); measured on the MI355X it was 14-20% slower than
+// RB = 28 (the folds and the lost occupancy cost more than the saved MACs).
 //
 // Montgomery product with R' = 2^(RB*LP) > 4N ("almost Montgomery", Walter
 // 2002): inputs and outputs stay below 2N, so no conditional subtraction is
@@ -45,14 +48,24 @@ struct Lazy {
   static_assert(LP % TPI == 0, "LP must be a multiple of TPI");
   static_assert(L32 % TPI == 0 && K32 % 4 == 0, "32-bit limbs per lane must be a multiple of 4");
   static_assert(RB * LP >= 32 * L32 + 2, "R' must exceed 4N");
-  static_assert((double)2 * LP * (double)(1ull << (2 * RB)) < 18446744073709551616.0, "accumulator bound");
+  // Carry folding. Between two folds a position receives at most 2 products
+  // (< 2^(2 RB) each) per row, on top of what a fold leaves (< 2^RB + 2^(64-RB))
+  // and one row carry (< 2^(64-RB)). FMAX rows keep that below 2^64. When all
+  // LP rows fit, no fold is needed (e.g. RB = 28 at 2048-bit moduli); otherwise the
+  // accumulators are folded every FR rows (a block of K rows, or half of one).
+  static constexpr double TWO64 = 18446744073709551616.0;
+  static constexpr int FMAX =
+      (int)((TWO64 - 4.0 * (double)(1ull << (64 - RB))) / (2.0 * (double)(1ull << (2 * RB))));
+  static constexpr bool FOLD = LP > FMAX;
+  static constexpr int FR = !FOLD ? LP : (K <= FMAX ? K * (FMAX / K) : K / 2);
+  static_assert(FR <= FMAX || !FOLD, "fold interval exceeds the accumulator bound");
+  static_assert(!FOLD || K <= FMAX || (K % 2 == 0 && K / 2 <= FMAX), "block too long to fold at half");
   static constexpr int SLOTS = (LP > L32 ? LP : L32);  // u32 rows of the staging area
   typedef Grp<TPI> grp;
 
   u32* lds;     // this wave's staging area: SLOTS rows of G u32
   u32 n0;       // -N^-1 mod 2^RB
   u32 N[K];     // modulus positions (this lane's slice)
-  u64 r0mask;   // all ones in rank 0, zero elsewhere
 
   __device__ __forceinline__ void init(u32* wave_area, const u32* __restrict__ npos, u32 n0inv) {
     lds = wave_area;
@@ -60,13 +73,25 @@ struct Lazy {
     const int r = grp::rank();
 #pragma unroll
     for (int p = 0; p < K; ++p) N[p] = npos[r * K + p];
-    r0mask = grp::rank() == 0 ? ~0ull : 0ull;
   }
 
-  __device__ __forceinline__ u64 shift_down64(u64 v) {
-    const u32 lo = grp::shift_down((u32)v);
-    const u32 hi = grp::shift_down((u32)(v >> 32));
-    return make64(lo, hi);
+  // One carry step on every position (not rippled): position p keeps its low
+  // RB bits plus the bits of position p-1 above RB; the top position of the
+  // top lane keeps all its bits. The value sum V[p] 2^(RB p) is unchanged.
+  // O = register of relative position 0 (the rotation of the row loop).
+  // In place, top position first, so each step reads the old value below it.
+  template <int O>
+  __device__ __forceinline__ void fold(u64 (&V)[K]) {
+    static_assert(K >= 2, "fold needs two positions per lane");
+    const u64 ctop = V[(K - 1 + O) % K] >> RB;
+    const u64 cin = make64(grp::shift_up((u32)ctop), grp::shift_up((u32)(ctop >> 32)));
+    const u64 topmask = grp::rank() == TPI - 1 ? ~0ull : (u64)MASK;
+#pragma unroll
+    for (int p = K - 1; p >= 1; --p) {
+      const u64 m = p == K - 1 ? topmask : (u64)MASK;
+      V[(p + O) % K] = (V[(p + O) % K] & m) + (V[(p - 1 + O) % K] >> RB);
+    }
+    V[O % K] = (V[O % K] & (u64)MASK) + cin;
   }
 
   // stage this lane's positions as the streamed operand: row i = position i
@@ -111,26 +136,40 @@ struct Lazy {
 #pragma unroll 1
     for (int blk = 0; blk < TPI; ++blk) {
       const u32* arow = lds + (blk * K) * G + g;
-#pragma unroll
-      for (int s = 0; s < K; ++s) {
-        const u32 a = arow[s * G];
-#pragma unroll
-        for (int p = 0; p < K; ++p) {
-          const int q = (p + s) % K;
-          V[q] = (u64)a * b[p] + V[q];
-        }
-        const u32 m = grp::bcast0(((u32)V[s] * n0) & MASK);
-#pragma unroll
-        for (int p = 0; p < K; ++p) {
-          const int q = (p + s) % K;
-          V[q] = (u64)m * N[p] + V[q];
-        }
-        const u64 v0 = V[s];
-        V[(s + 1) % K] += (v0 >> RB) & r0mask;
-        V[s] = shift_down64(v0);
+      row<0>(V, b, arow);
+      if constexpr (FOLD && FR >= K) {
+        if ((blk + 1) % (FR / K) == 0) fold<0>(V);  // wave-uniform
       }
     }
     normalize(b, V);
+  }
+
+  // rows S..K-1 of one block (unrolled by recursion over S)
+  template <int S>
+  __device__ __forceinline__ void row(u64 (&V)[K], const u32 (&b)[K], const u32* arow) {
+    if constexpr (S < K) {
+      const u32 a = arow[S * G];
+#pragma unroll
+      for (int p = 0; p < K; ++p) {
+        const int q = (p + S) % K;
+        V[q] = (u64)a * b[p] + V[q];
+      }
+      const u32 m = grp::bcast0(((u32)V[S] * n0) & MASK);
+#pragma unroll
+      for (int p = 0; p < K; ++p) {
+        const int q = (p + S) % K;
+        V[q] = (u64)m * N[p] + V[q];
+      }
+      // Every lane carries the bits of its bottom position above RB into its
+      // next position and passes only the low RB bits down to the lane below
+      // (value unchanged). Rank 0's low bits are 0 after m*N, so the group's
+      // top rank receives 0 from the next group without a select.
+      const u64 v0 = V[S];
+      V[(S + 1) % K] += v0 >> RB;
+      V[S] = (u64)grp::shift_down_z0((u32)v0 & MASK);
+      if constexpr (FOLD && FR < K && (S + 1) % FR == 0) fold<(S + 1) % K>(V);
+      row<S + 1>(V, b, arow);
+    }
   }
 
   // x = x * y (both below 2N)

@@ -46,9 +46,18 @@ template <int TPI> struct Grp {
     if constexpr (TPI == 1) {
       return v;
     } else if constexpr (TPI == 2) {
-      return (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0xA0, 0xF, 0xF, false);  // quad_perm [0,0,2,2]
+      // every lane has a source in its quad: no "old" value to initialise
+      return (u32)__builtin_amdgcn_mov_dpp((int)v, 0xA0, 0xF, 0xF, false);  // quad_perm [0,0,2,2]
     } else if constexpr (TPI == 4) {
-      return (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x00, 0xF, 0xF, false);  // quad_perm [0,0,0,0]
+      return (u32)__builtin_amdgcn_mov_dpp((int)v, 0x00, 0xF, 0xF, false);  // quad_perm [0,0,0,0]
+    } else if constexpr (TPI == 8 || TPI == 16) {
+      // quad broadcast, then copy quads up the row: row_shr:4 into banks 1 and 3
+      // (lanes 4-7, 12-15 of each row of 16); for 16 lanes also row_shr:8 into
+      // banks 2 and 3. DPP only, no LDS round trip.
+      int x = __builtin_amdgcn_mov_dpp((int)v, 0x00, 0xF, 0xF, false);  // quad_perm [0,0,0,0]
+      x = __builtin_amdgcn_update_dpp(x, x, 0x114, 0xF, 0xA, false);     // row_shr:4, banks 1, 3
+      if constexpr (TPI == 16) x = __builtin_amdgcn_update_dpp(x, x, 0x118, 0xF, 0xC, false);  // row_shr:8
+      return (u32)x;
     } else if constexpr (TPI == 64) {
       return (u32)__builtin_amdgcn_readfirstlane((int)v);
     } else {
@@ -67,6 +76,21 @@ template <int TPI> struct Grp {
     } else {
       u32 x = (u32)__builtin_amdgcn_ds_bpermute((lane_id() + 1) << 2, (int)v);
       return (rank() == TPI - 1) ? 0u : x;
+    }
+  }
+
+  // lane r receives v from lane r+1 of the wave (the top rank from the next
+  // group's rank 0, the wave's last lane 0). Equals shift_down when every
+  // group's rank 0 passes 0, without the select that zeroes the top rank.
+  __device__ __forceinline__ static u32 shift_down_z0(u32 v) {
+    if constexpr (TPI == 1) {
+      return 0u;
+    } else if constexpr (TPI <= 16) {
+      // row_shl:1 within rows of 16 lanes; bound_ctrl gives lane 15 of a row 0
+      // (a group never spans rows, so that lane is a top rank)
+      return (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x101, 0xF, 0xF, true);
+    } else {
+      return (u32)__builtin_amdgcn_ds_bpermute(((lane_id() + 1) & 63) << 2, (int)v);
     }
   }
 

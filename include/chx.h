@@ -230,8 +230,10 @@ int chx_host_mont_consts(const uint32_t* n, int limbs, uint32_t* n0inv, uint32_t
 /* time `iters` rounds of 16 dependent-free v_mad_u64_u32 per lane on `blocks`
  * blocks of 256 lanes; returns seconds and the number of 32x32->64 MACs */
 int chx_probe_mad(chx_ctx* ctx, int blocks, int iters, double* seconds, double* macs);
-/* same for other instruction kinds: 0 v_mad_u64_u32, 1 v_mul_lo_u32, 2 v_mul_hi_u32,
- * 3 64-bit add, 4 v_fma_f64, 5 v_mad_u32_u24; *ops = operations executed */
+/* same for other instruction kinds: 0 v_mad_u64_u32 (carry-out to VCC), 1 v_mul_lo_u32,
+ * 2 v_mul_hi_u32, 3 64-bit add, 4 v_fma_f64, 5 v_mad_u32_u24, 6 v_mad_u64_u32 with the
+ * carry-out in an SGPR pair, 7 kind 6 plus one v_add_u32 per two MADs; *ops = the
+ * operations of the kind (MADs for 6 and 7) */
 int chx_probe_rate(chx_ctx* ctx, int kind, int blocks, int iters, double* seconds, double* ops);
 /* calibration copy of an interleaved batch (device, [limbs][stride]) with the
  * lane-group access pattern of the exponentiation kernels: limbs 64 with tpi 4|8,
