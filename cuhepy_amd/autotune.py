@@ -83,10 +83,11 @@ def search(candidates: Iterable[dict], run: Callable[[dict], Callable[[], None]]
     return best, results
 
 
-def tune_ragged(ctx, workload, device_name: str, variants=(0, 1, 2), windows=(4, 5, 6), waves=(4, 8),
+def tune_ragged(ctx, workload, device_name: str, variants=(0, 1, 2, 3), windows=(4, 5, 6), waves=(4, 8),
                 reps: int = 2, force: bool = False) -> tuple[dict, list]:
     """Coordinate search for the ragged persistent launch: variant (lanes per
-    bignum per width), then window width, then persistent waves per CU."""
+    bignum per width, one kernel per width or one mixed kernel), then window
+    width, then persistent waves per CU."""
     from .engine import RaggedPlan
 
     widths = sorted({(s.limbs, s.count // 1024) for s in workload.subs})

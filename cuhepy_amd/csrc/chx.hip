@@ -1050,20 +1050,29 @@ struct chx_ragged {
 namespace {
 
 // variant 0: per-width persistent kernels (4 / 8 lanes), variant 1: per-width
-// (8 / 16 lanes), variant 2: one mixed persistent kernel (8 / 16 lanes)
-const int kRaggedCfg[3] = {0, 1, 1};  // lazy configuration index used by each variant
+// (8 / 16 lanes), variant 2: one mixed persistent kernel (8 / 16 lanes),
+// variant 3: one mixed persistent kernel (4 / 8 lanes). The mixed queues hold
+// the costliest tiles first (longest-processing-time order), so the cheap ones
+// fill the tail.
+const int kRaggedCfg[4] = {0, 1, 1, 0};  // lazy configuration index used by each variant
 
 void ragged_enqueue(chx_ragged* rg) {
   chx_ctx* ctx = rg->ctx;
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, rg->d_counter, 0u, 2);
-  if (rg->variant == 2) {
-    constexpr int RB1 = 28, LP1 = 80, T1 = 8, RB2 = 27, LP2 = 160, T2 = 16;
-    typedef Lazy<RB1, LP1, T1, 64> E1;
-    typedef Lazy<RB2, LP2, T2, 128> E2;
+  auto mixed = [&](auto e1, auto e2, auto kernel) {
+    typedef decltype(e1) E1;
+    typedef decltype(e2) E2;
     constexpr int AREA = (E1::SLOTS * E1::G > E2::SLOTS * E2::G) ? E1::SLOTS * E1::G : E2::SLOTS * E2::G;
-    hipLaunchKernelGGL((k_ragged2<RB1, LP1, T1, RB2, LP2, T2>), dim3(rg->blocks), dim3(BLOCK_THREADS),
-                       (size_t)WAVES_PER_BLOCK * AREA * 4, ctx->stream, rg->d_q[0], rg->q_nsubs[0], rg->q_tiles[0],
-                       rg->d_counter, rg->window, rg->d_scratch, rg->wave_words);
+    hipLaunchKernelGGL(kernel, dim3(rg->blocks), dim3(BLOCK_THREADS), (size_t)WAVES_PER_BLOCK * AREA * 4,
+                       ctx->stream, rg->d_q[0], rg->q_nsubs[0], rg->q_tiles[0], rg->d_counter, rg->window,
+                       rg->d_scratch, rg->wave_words);
+  };
+  if (rg->variant == 2) {
+    mixed(Lazy<28, 80, 8, 64>(), Lazy<27, 160, 16, 128>(), k_ragged2<28, 80, 8, 27, 160, 16>);
+    return;
+  }
+  if (rg->variant == 3) {
+    mixed(Lazy<28, 76, 4, 64>(), Lazy<27, 152, 8, 128>(), k_ragged2<28, 76, 4, 27, 152, 8>);
     return;
   }
   for (int q = 0; q < 2; ++q) {
@@ -1087,7 +1096,7 @@ int chx_ragged_create(chx_ctx* ctx, const chx_ragged_sub* subs, int nsubs, int v
                       int waves_per_cu, chx_ragged** out) {
   return with_ctx(ctx, [&] {
     if (!subs || nsubs <= 0 || !out) fail(CHX_EINVAL, "bad argument");
-    if (variant < 0 || variant > 2) fail(CHX_EUNSUPPORTED, "ragged variant %d not built", variant);
+    if (variant < 0 || variant > 3) fail(CHX_EUNSUPPORTED, "ragged variant %d not built", variant);
     if (window == 0) window = 5;
     if (window < 2 || window > 6) fail(CHX_EINVAL, "window %d out of range [2, 6]", window);
     if (waves_per_cu == 0) waves_per_cu = 8;
@@ -1099,10 +1108,23 @@ int chx_ragged_create(chx_ctx* ctx, const chx_ragged_sub* subs, int nsubs, int v
     rg->nsubs = nsubs;
     rg->subs.assign(subs, subs + nsubs);
     const int ci = kRaggedCfg[variant];
+    const bool is_mixed = variant >= 2;
     std::vector<RaggedSub> q[2];
     int tiles = 0;
     int lp_max_g = 0;
-    for (int s = 0; s < nsubs; ++s) {
+    // queue order: input order per width; costliest tiles first when mixed
+    std::vector<int> order(nsubs);
+    for (int s = 0; s < nsubs; ++s) order[s] = s;
+    if (is_mixed) {
+      auto tile_cost = [&](int s) {
+        const int L = subs[s].mod ? subs[s].mod->hc.L : 0;
+        const auto& cf = lazy_cfgs(L == 128 ? 128 : 64)[ci];
+        return (double)subs[s].e_bits * cf.lp * cf.lp * (64 / cf.tpi);
+      };
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return tile_cost(a) > tile_cost(b); });
+    }
+    std::vector<RaggedSub> by_input(nsubs);
+    for (int s : order) {
       const chx_ragged_sub& u = subs[s];
       if (!u.mod || !u.base || !u.exps || !u.out) fail(CHX_EINVAL, "sub-batch %d: null pointer", s);
       const int L = u.mod->hc.L;
@@ -1122,14 +1144,15 @@ int chx_ragged_create(chx_ctx* ctx, const chx_ragged_sub* subs, int nsubs, int v
       r.stride = (int)u.stride;
       r.e_limbs = (int)u.e_limbs;
       r.nwin = ((int)u.e_bits + window - 1) / window;
-      const int qi = variant == 2 ? 0 : (L == 64 ? 0 : 1);
+      const int qi = is_mixed ? 0 : (L == 64 ? 0 : 1);
       r.tile0 = rg->q_tiles[qi];
       rg->q_tiles[qi] += ((int)u.count + G - 1) / G;
       tiles += ((int)u.count + G - 1) / G;
       lp_max_g = std::max(lp_max_g, cfg.lp * G);
-      rg->hsubs.push_back(r);
+      by_input[s] = r;
       q[qi].push_back(r);
     }
+    rg->hsubs = by_input;
     for (int i = 0; i < 2; ++i) {
       rg->q_nsubs[i] = (int)q[i].size();
       if (!q[i].empty()) rg->d_q[i] = dev_upload(q[i].data(), q[i].size(), ctx->stream);

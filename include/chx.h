@@ -143,7 +143,9 @@ typedef struct {
 /* variant 0: one persistent kernel per width (4 lanes per 2048-bit, 8 per
  * 4096-bit modulus), both in the one graph; variant 1: the same with 8 and 16
  * lanes; variant 2: a single persistent kernel over both widths (8 and 16
- * lanes). window 2..6 (0 = 5); waves_per_cu: persistent waves per CU (0 = 8). */
+ * lanes); variant 3: a single persistent kernel over both widths (4 and 8
+ * lanes). The single-kernel variants queue the costliest tiles first.
+ * window 2..6 (0 = 5); waves_per_cu: persistent waves per CU (0 = 8). */
 int chx_ragged_create(chx_ctx* ctx, const chx_ragged_sub* subs, int nsubs, int variant, int window,
                       int waves_per_cu, chx_ragged** out);
 /* persistent launch; use_graph != 0 replays the captured hipGraph */

@@ -21,7 +21,7 @@ def workload(ctx):
     return DeviceWorkload(ctx, subs)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 def test_ragged_persistent_bit_exact(ctx, workload, variant):
     from cuhepy_amd.engine import RaggedPlan
     plan = RaggedPlan(ctx, workload.desc, variant=variant, window=0, waves_per_cu=2)
