@@ -70,7 +70,7 @@ def cpu_baseline(cp: dict, cs: list[int], want: list[int], cores: int) -> dict:
         assert got == want
         rate1 = len(cs) / (t1 - t0)
         return {"engine": engine, "single": 2 * rate1, "multi": 2 * rate1, "cores": 1, "n": len(cs)}
-    single_n = max(64, len(cs) // 8)
+    single_n = min(len(cs), 2048)
     dec = G.CrtDecryptor(**cp)
     t0 = time.perf_counter()
     got1 = [dec(c) for c in cs[:single_n]]
@@ -464,7 +464,8 @@ def main() -> int:
     ap.add_argument("--batch", type=int, default=65536, help="ciphertexts per step")
     ap.add_argument("--key-len", type=int, default=1024, help="prime bits (n has 2*key_len bits)")
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--cpu-sample", type=int, default=2048, help="ciphertexts in the CPU-baseline sample")
+    ap.add_argument("--cpu-sample", type=int, default=65536,
+                    help="ciphertexts in the CPU-baseline sample (all processes; one thread times 2,048 of them)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-chunk", type=int, default=1024)
     ap.add_argument("--no-plain", action="store_true")

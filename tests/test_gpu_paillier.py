@@ -159,3 +159,27 @@ def test_lookup_seeded_every_variant(ctx):
         got = limbs_to_ints(key.encrypt_digits(digits, idx))
         assert got == want, f"tpi={tpi}"
         assert limbs_to_ints(key.decrypt(got)) == ms, f"decrypt tpi={tpi}"
+
+
+def test_plain_encrypt_decrypt_1024_messages():
+    """The reference's CPU-runnable case: key_len 1024-bit n (512-bit primes,
+    n^2 2048-bit), keypair from seeded primes, 1024 messages m uniform in [0, n)
+    and r uniform in Z_n*, one batch: every ciphertext equals (1+n)^m r^n mod
+    n^2 computed with GMP (the engine gmpy2 wraps) and every decryption, plain
+    and CRT, returns m (scheme.py:63-104)."""
+    from cuhepy_amd.engine import Context, PaillierKey
+    from cuhepy_amd.limbs import limbs_to_ints
+    from oracle import gmp_engine as G
+    powm = G.powm if G.available() else pow
+    keys = O.key_gen(512, 20241)
+    n, n2 = keys["pk"]["n"], keys["pk"]["n_squared"]
+    assert n.bit_length() == 1024
+    key = PaillierKey(Context.default(0), n, keys["sk"]["phi"], keys["sk"]["inv"], keys["p"], keys["q"])
+    rng = random.Random(1024)
+    ms = [rng.randrange(n) for _ in range(1024)]
+    rs = [O.random_r(n, rng) for _ in range(1024)]
+    cs = limbs_to_ints(key.encrypt(ms, rs))
+    assert cs == [(1 + m * n) * powm(r, n, n2) % n2 for m, r in zip(ms, rs)]
+    assert cs[:8] == [O.encrypt(m, keys["pk"], r) for m, r in zip(ms[:8], rs[:8])]
+    assert limbs_to_ints(key.decrypt(cs)) == ms
+    assert limbs_to_ints(key.decrypt_crt(cs)) == ms
