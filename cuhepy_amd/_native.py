@@ -72,6 +72,7 @@ SIGNATURES = {
     "chx_mulmod_bcast_dev": (c_int, [vp, vp, vp, u32p, vp, c_size_t, c_size_t]),
     "chx_mulmod_operand_words": (c_int, [vp]),
     "chx_mulmod_operands_dev": (c_int, [vp, vp, vp, vp, c_size_t]),
+    "chx_mulmod_operands": (c_int, [vp, vp, u32p, vp, c_size_t]),
     "chx_mulmod_multi_dev": (c_int, [vp, vp, vp, c_size_t, c_size_t, c_size_t, c_int, POINTER(vp), POINTER(vp),
                                      c_size_t]),
     "chx_modexp": (c_int, [vp, vp, u32p, c_size_t, u32p, c_int, u32p]),

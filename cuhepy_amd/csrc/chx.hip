@@ -874,16 +874,33 @@ int chx_mulmod_operand_words(const chx_mod* m) {
   return cfgs.empty() ? 0 : cfgs[kServeCfg].lp;
 }
 
+static void launch_operands(chx_ctx* ctx, const chx_mod* m, const uint32_t* y, uint32_t* ops, size_t count) {
+  const int L = m->hc.L;
+  CHX_DISPATCH_LAZY(L, kServeCfg, {
+    hipLaunchKernelGGL((k_lazy_table<RB, LP, TPI, L32>), grid_for(count, TPI), dim3(BLOCK_THREADS),
+                       lazy_lds_bytes(LP, L32, TPI), ctx->stream, m->lz.d[kServeCfg], y, ops, (int)count);
+  });
+  check_launch("k_lazy_table");
+}
+
 int chx_mulmod_operands_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* y, uint32_t* ops, size_t count) {
   return with_ctx(ctx, [&] {
     if (!m || !y || !ops) fail(CHX_EINVAL, "null argument");
     if (count == 0) return;
-    const int L = m->hc.L;
-    CHX_DISPATCH_LAZY(L, kServeCfg, {
-      hipLaunchKernelGGL((k_lazy_table<RB, LP, TPI, L32>), grid_for(count, TPI), dim3(BLOCK_THREADS),
-                         lazy_lds_bytes(LP, L32, TPI), ctx->stream, m->lz.d[kServeCfg], y, ops, (int)count);
-    });
-    check_launch("k_lazy_table");
+    launch_operands(ctx, m, y, ops, count);
+  });
+}
+
+int chx_mulmod_operands(chx_ctx* ctx, const chx_mod* m, const uint32_t* y, uint32_t* ops, size_t count) {
+  return with_ctx(ctx, [&] {
+    if (!m || !y || !ops) fail(CHX_EINVAL, "null argument");
+    if (count == 0) return;
+    const size_t bytes = count * (size_t)m->hc.L * sizeof(uint32_t);
+    ctx->slot[S_AUX].ensure(bytes);
+    // pageable source: the copy has read it when the call returns; the kernel
+    // follows it on the context's stream (no host-device synchronisation)
+    hip_check(hipMemcpyAsync(ctx->slot[S_AUX].p, y, bytes, hipMemcpyHostToDevice, ctx->stream), "H2D");
+    launch_operands(ctx, m, ctx->slot[S_AUX].as<uint32_t>(), ops, count);
   });
 }
 

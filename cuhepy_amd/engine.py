@@ -4,7 +4,9 @@ Everything here calls ``libchx.so``; there is no CPU path. Host-buffer calls
 take and return numpy ``uint32`` limb batches (see ``limbs.py``); the
 ``*_dev`` methods take torch tensors that already live on the GPU in the
 limb-interleaved layout (limb j of item i at ``[j, i]`` of a
-``(limbs, stride)`` tensor) and enqueue on torch's current stream.
+``(limbs, stride)`` tensor) and enqueue on the context's stream (its own
+non-blocking stream unless ``set_stream``/``use_torch_stream`` chose another):
+work PyTorch does on those tensors on another stream must be synchronised.
 """
 
 from __future__ import annotations
@@ -89,8 +91,19 @@ class Context:
         return int(N.lib.chx_ctx_get_stream(self.handle) or 0)
 
     def use_torch_stream(self) -> None:
+        """Put this context and PyTorch on one stream, so their work is ordered.
+
+        A non-default PyTorch stream is handed to the library. PyTorch's default
+        stream cannot be (its handle is NULL, which selects the context's own
+        non-blocking stream), so in that case PyTorch is switched onto the
+        context's stream instead (``torch.cuda.set_stream``)."""
         import torch
-        self.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
+        cur = torch.cuda.current_stream(self.device)
+        if cur.cuda_stream:
+            self.set_stream(cur.cuda_stream)
+        else:
+            self.set_stream(None)
+            torch.cuda.set_stream(torch.cuda.ExternalStream(self.stream_ptr, device=torch.device("cuda", self.device)))
 
     def sync(self) -> None:
         N.check(N.lib.chx_ctx_sync(self.handle))
@@ -219,6 +232,14 @@ class Modulus:
     def operands_dev(self, y, ops, count: int) -> None:
         """ops[j] = operand form of y[j] (device; y: count x limbs, ops: count x operand_words)."""
         N.check(N.lib.chx_mulmod_operands_dev(self.ctx.handle, self.handle, _dptr(y), _dptr(ops), count))
+
+    def operands(self, y: np.ndarray, ops) -> None:
+        """ops[j] = operand form of host limbs y[j] (count x limbs), enqueued on the
+        context's stream without synchronising."""
+        Y = np.ascontiguousarray(y, dtype=np.uint32)
+        if Y.ndim != 2 or Y.shape[1] != self.limbs:
+            raise ValueError(f"y must be (count, {self.limbs}) limbs")
+        N.check(N.lib.chx_mulmod_operands(self.ctx.handle, self.handle, _p32(Y), _dptr(ops), Y.shape[0]))
 
     def mulmod_multi_dev(self, x, stride: int, begin: int, count: int, ops: Sequence, outs: Sequence,
                          out_stride: int) -> None:

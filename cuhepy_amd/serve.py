@@ -192,17 +192,16 @@ class KnnServer:
         return req
 
     def _on_admit(self, req: Request, how: str) -> None:
-        import numpy as np
-
         from .limbs import ints_to_limbs
         torch = self.torch
-        buf = self.pool.pop() if how == "pool" else torch.empty((self.limbs, self.stride), dtype=torch.int32,
-                                                                 device=self.device)
-        y = torch.from_numpy(ints_to_limbs([req.y], self.limbs).view(np.int32).copy()).to(self.device,
-                                                                                          non_blocking=False)
-        op = torch.empty((self.mod.operand_words,), dtype=torch.int32, device=self.device)
-        torch.cuda.current_stream(self.device).synchronize()
-        self.mod.operands_dev(y, op, 1)
+        with torch.cuda.stream(self.stream):  # allocations belong to the stream the kernels use
+            buf = self.pool.pop() if how == "pool" else torch.empty((self.limbs, self.stride), dtype=torch.int32,
+                                                                     device=self.device)
+            op = torch.empty((self.mod.operand_words,), dtype=torch.int32, device=self.device)
+        # the query's operand is prepared on the library's stream, behind the
+        # iterations in flight and ahead of the next launch: no synchronisation
+        y = ints_to_limbs([req.y], self.limbs)
+        self.mod.operands(y, op)
         req._buf, req._op, req._y = buf, op, y
 
     # ---- loop ----

@@ -103,6 +103,8 @@ int chx_mulmod_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* a, const uint
  * arrays of nq device pointers. Results are canonical (below n). */
 int chx_mulmod_operand_words(const chx_mod* m);
 int chx_mulmod_operands_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* y, uint32_t* ops, size_t count);
+/* the same from host limbs (count x limbs): stream-ordered, no synchronisation */
+int chx_mulmod_operands(chx_ctx* ctx, const chx_mod* m, const uint32_t* y, uint32_t* ops, size_t count);
 int chx_mulmod_multi_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* x, size_t stride, size_t begin,
                          size_t count, int nq, const uint32_t* const* ops, uint32_t* const* outs,
                          size_t out_stride);
