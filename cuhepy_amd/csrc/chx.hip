@@ -67,6 +67,23 @@ int guarded(F&& f) {
 
 size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
 
+// Kernels index items with 32-bit ints and limb rows as row * stride + item:
+// batches are bounded so every such index fits.
+constexpr size_t kMaxStride = (size_t)1 << 28;
+
+// stride of a host-buffer call's device copy (count rounded up to 64)
+size_t batch_stride(size_t count) {
+  if (count > kMaxStride) fail(CHX_EINVAL, "batch of %zu items exceeds the maximum of %zu", count, kMaxStride);
+  return round_up(count, 64);
+}
+
+// a caller's interleaved device batch: stride >= count rounded up to 64, both bounded
+void check_stride(size_t count, size_t stride) {
+  if (count > kMaxStride || stride > kMaxStride)
+    fail(CHX_EINVAL, "batch of %zu items / stride %zu exceeds the maximum of %zu", count, stride, kMaxStride);
+  if (stride < round_up(count, 64)) fail(CHX_EINVAL, "stride %zu below count rounded to 64", stride);
+}
+
 // ---------------------------------------------------------------------------
 // device buffers
 // ---------------------------------------------------------------------------
@@ -845,7 +862,7 @@ int chx_mulmod(chx_ctx* ctx, const chx_mod* m, const uint32_t* a, const uint32_t
     if (!m || !a || !b || !out) fail(CHX_EINVAL, "null argument");
     if (count == 0) return;
     const int L = m->hc.L;
-    const size_t stride = round_up(count, 64);
+    const size_t stride = batch_stride(count);
     uint32_t* da = upload_il(ctx, S_IN0, a, count, L, stride);
     uint32_t* db = upload_il(ctx, S_IN1, b, count, L, stride);
     ctx->slot[S_OUT].ensure(stride * L * 4);
@@ -859,7 +876,7 @@ int chx_mulmod_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* a, const uint
   return with_ctx(ctx, [&] {
     if (!m || !a || !b || !out) fail(CHX_EINVAL, "null argument");
     if (count == 0) return;
-    if (stride < round_up(count, 64)) fail(CHX_EINVAL, "stride %zu below count rounded to 64", stride);
+    check_stride(count, stride);
     launch_mulmod(ctx, m->d, m->hc.L, a, b, out, count, stride);
   });
 }
@@ -914,7 +931,7 @@ int chx_mulmod_multi_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* x, size
     if (begin + count > stride) fail(CHX_EINVAL, "range [%zu, %zu) exceeds the index stride %zu", begin,
                                      begin + count, stride);
     if (begin + count > out_stride) fail(CHX_EINVAL, "range exceeds the output stride %zu", out_stride);
-    if (stride > (size_t)INT32_MAX / 2 || out_stride > (size_t)INT32_MAX / 2) fail(CHX_EINVAL, "stride too large");
+    if (stride > kMaxStride || out_stride > kMaxStride) fail(CHX_EINVAL, "stride exceeds the maximum of %zu", kMaxStride);
     MultiQuery mq;
     memset(&mq, 0, sizeof(mq));
     mq.nq = nq;
@@ -941,7 +958,7 @@ int chx_mulmod_bcast_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* x, cons
     if (!m || !x || !y_host || !out) fail(CHX_EINVAL, "null argument");
     if (count == 0) return;
     const int L = m->hc.L;
-    if (stride < round_up(count, 64)) fail(CHX_EINVAL, "stride %zu below count rounded to 64", stride);
+    check_stride(count, stride);
     BN yR = host::to_mont_host(host::bn_mod(host::bn_from(y_host, L, L), m->hc.N), m->hc);
     ctx->slot[S_AUX].ensure(L * 4);
     hip_check(hipMemcpyAsync(ctx->slot[S_AUX].p, yR.data(), L * 4, hipMemcpyHostToDevice, ctx->stream), "H2D");
@@ -979,7 +996,7 @@ int chx_modexp(chx_ctx* ctx, const chx_mod* m, const uint32_t* base, size_t coun
       memcpy(out, one.data(), one.size() * 4);
       return;
     }
-    const size_t stride = round_up(count, 64);
+    const size_t stride = batch_stride(count);
     uint32_t* db = upload_il(ctx, S_IN0, base, count, L, stride);
     ctx->slot[S_AUX].ensure(s.h.size() * 4);
     hip_check(hipMemcpyAsync(ctx->slot[S_AUX].p, s.h.data(), s.h.size() * 4, hipMemcpyHostToDevice, ctx->stream),
@@ -997,7 +1014,7 @@ int chx_modexp_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* base, uint32_
   return with_ctx(ctx, [&] {
     if (!m || !base || !out || !e) fail(CHX_EINVAL, "null argument");
     if (count == 0) return;
-    if (stride < round_up(count, 64)) fail(CHX_EINVAL, "stride %zu below count rounded to 64", stride);
+    check_stride(count, stride);
     const int L = m->hc.L;
     ExpSched s = make_sched(ctx, e, e_limbs, L, CHX_OP_MODEXP);
     if (s.h.empty()) fail(CHX_EINVAL, "zero exponent is handled by the host-buffer entry point only");
@@ -1020,7 +1037,7 @@ int chx_modexp_each(chx_ctx* ctx, const chx_mod* m, const uint32_t* base, size_t
     int ebits = 0;
     for (size_t i = 0; i < count * (size_t)e_limbs; ++i)
       if (exps[i]) ebits = std::max(ebits, 32 * (int)(i % e_limbs) + 32 - __builtin_clz(exps[i]));
-    const size_t stride = round_up(count, 64);
+    const size_t stride = batch_stride(count);
     uint32_t* db = upload_il(ctx, S_IN0, base, count, L, stride);
     uint32_t* de = upload_il(ctx, S_IN1, exps, count, e_limbs, stride);
     ctx->slot[S_OUT].ensure(stride * L * 4);
@@ -1036,7 +1053,7 @@ int chx_modexp_each_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* base, co
     if (!m || !base || !out || !exps || e_limbs <= 0 || e_bits <= 0 || e_bits > 32 * e_limbs)
       fail(CHX_EINVAL, "bad argument");
     if (count == 0) return;
-    if (stride < round_up(count, 64)) fail(CHX_EINVAL, "stride %zu below count rounded to 64", stride);
+    check_stride(count, stride);
     launch_modexp_each(ctx, m->lz, base, exps, e_limbs, e_bits, out, count, stride);
   });
 }
@@ -1146,7 +1163,8 @@ int chx_ragged_create(chx_ctx* ctx, const chx_ragged_sub* subs, int nsubs, int v
       if (!u.mod || !u.base || !u.exps || !u.out) fail(CHX_EINVAL, "sub-batch %d: null pointer", s);
       const int L = u.mod->hc.L;
       if (L != 64 && L != 128) fail(CHX_EUNSUPPORTED, "sub-batch %d: width %d limbs (64 or 128 supported)", s, L);
-      if (u.count == 0 || u.stride < round_up(u.count, 64)) fail(CHX_EINVAL, "sub-batch %d: bad count/stride", s);
+      if (u.count == 0 || u.stride < round_up(u.count, 64) || u.stride > kMaxStride)
+        fail(CHX_EINVAL, "sub-batch %d: bad count/stride", s);
       if (u.e_limbs == 0 || u.e_bits == 0 || u.e_bits > 32 * u.e_limbs)
         fail(CHX_EINVAL, "sub-batch %d: bad exponent size", s);
       const LazyCfg cfg = lazy_cfgs(L)[ci];
@@ -1362,7 +1380,7 @@ int chx_paillier_encrypt(chx_ctx* ctx, const chx_pkey* kc, const uint32_t* m, co
     const int L = k->W;
     check_below(m, count, L, k->n.N, "m");
     check_below(r, count, L, k->n.N, "r");
-    const size_t stride = round_up(count, 64);
+    const size_t stride = batch_stride(count);
     uint32_t* dm = upload_il(ctx, S_IN0, m, count, L, stride);
     uint32_t* dr = upload_il(ctx, S_IN1, r, count, L, stride);
     ctx->slot[S_OUT].ensure(stride * L * 4);
@@ -1377,7 +1395,7 @@ int chx_paillier_encrypt_dev(chx_ctx* ctx, const chx_pkey* kc, const uint32_t* m
   return with_ctx(ctx, [&] {
     if (!k || !m || !r || !c) fail(CHX_EINVAL, "null argument");
     if (count == 0) return;
-    if (stride < round_up(count, 64)) fail(CHX_EINVAL, "stride %zu below count rounded to 64", stride);
+    check_stride(count, stride);
     launch_encrypt(k, m, r, c, count, stride);
   });
 }
@@ -1398,7 +1416,7 @@ int chx_paillier_decrypt(chx_ctx* ctx, const chx_pkey* kc, const uint32_t* c, ui
     if (count == 0) return;
     const int L = k->W;
     check_below(c, count, L, k->n2.N, "c");
-    const size_t stride = round_up(count, 64);
+    const size_t stride = batch_stride(count);
     uint32_t* dc = upload_il(ctx, S_IN0, c, count, L, stride);
     ctx->slot[S_OUT].ensure(stride * L * 4);
     paillier_decrypt_impl(k, dc, ctx->slot[S_OUT].as<uint32_t>(), count, stride, nullptr, nullptr);
@@ -1412,7 +1430,7 @@ int chx_paillier_decrypt_dev(chx_ctx* ctx, const chx_pkey* kc, const uint32_t* c
   return with_ctx(ctx, [&] {
     if (!k || !c || !m) fail(CHX_EINVAL, "null argument");
     if (count == 0) return;
-    if (stride < round_up(count, 64)) fail(CHX_EINVAL, "stride %zu below count rounded to 64", stride);
+    check_stride(count, stride);
     paillier_decrypt_impl(k, c, m, count, stride, nullptr, nullptr);
   });
 }
@@ -1424,7 +1442,7 @@ int chx_paillier_decrypt_crt(chx_ctx* ctx, const chx_pkey* kc, const uint32_t* c
     if (count == 0) return;
     const int L = k->W;
     check_below(c, count, L, k->n2.N, "c");
-    const size_t stride = round_up(count, 64);
+    const size_t stride = batch_stride(count);
     uint32_t* dc = upload_il(ctx, S_IN0, c, count, L, stride);
     ctx->slot[S_OUT].ensure(stride * k->H * 4);
     launch_crt(k, dc, ctx->slot[S_OUT].as<uint32_t>(), count, stride);
@@ -1438,7 +1456,7 @@ int chx_paillier_decrypt_crt_dev(chx_ctx* ctx, const chx_pkey* kc, const uint32_
   return with_ctx(ctx, [&] {
     if (!k || !c || !m) fail(CHX_EINVAL, "null argument");
     if (count == 0) return;
-    if (stride < round_up(count, 64)) fail(CHX_EINVAL, "stride %zu below count rounded to 64", stride);
+    check_stride(count, stride);
     launch_crt(k, c, m, count, stride);
   });
 }
@@ -1451,7 +1469,7 @@ int chx_paillier_decode_hamming(chx_ctx* ctx, const chx_pkey* kc, const uint32_t
     if (count == 0) return;
     const int L = k->W;
     check_below(c, count, L, k->n2.N, "c");
-    const size_t stride = round_up(count, 64);
+    const size_t stride = batch_stride(count);
     uint32_t* dc = upload_il(ctx, S_IN0, c, count, L, stride);
     ctx->slot[S_IN1].ensure(L * 4);
     hip_check(hipMemcpyAsync(ctx->slot[S_IN1].p, mask, L * 4, hipMemcpyHostToDevice, ctx->stream), "H2D");
@@ -1530,7 +1548,7 @@ int chx_lookup_encrypt(chx_ctx* ctx, const chx_lkey* kc, const uint8_t* digits, 
     for (size_t i = 0; i < count * (size_t)nmult; ++i)
       if (noise_idx[i] >= k->noise) fail(CHX_EINVAL, "noise index %u out of range", noise_idx[i]);
     const int L = k->W;
-    const size_t stride = round_up(count, 64);
+    const size_t stride = batch_stride(count);
     const size_t dbytes = count * (size_t)k->chunks, nbytes = count * (size_t)nmult;
     ctx->slot[S_IN0].ensure(dbytes + 16);
     ctx->slot[S_IN1].ensure(nbytes + 16);
@@ -1550,7 +1568,7 @@ int chx_lookup_encrypt_dev(chx_ctx* ctx, const chx_lkey* kc, const uint8_t* digi
   return with_ctx(ctx, [&] {
     if (!k || !digits || !c || (nmult > 0 && !noise_idx)) fail(CHX_EINVAL, "null argument");
     if (count == 0) return;
-    if (stride < round_up(count, 64)) fail(CHX_EINVAL, "stride %zu below count rounded to 64", stride);
+    check_stride(count, stride);
     launch_lookup_encrypt(k, digits, noise_idx, nmult, c, count, stride);
   });
 }
@@ -1570,7 +1588,7 @@ int chx_lookup_decrypt(chx_ctx* ctx, const chx_lkey* kc, const uint32_t* c, uint
     if (count == 0) return;
     const int L = k->W;
     check_below(c, count, L, k->n2.N, "c");
-    const size_t stride = round_up(count, 64);
+    const size_t stride = batch_stride(count);
     uint32_t* dc = upload_il(ctx, S_IN0, c, count, L, stride);
     ctx->slot[S_OUT].ensure(stride * L * 4);
     lookup_decrypt_impl(k, dc, ctx->slot[S_OUT].as<uint32_t>(), count, stride, nullptr, nullptr);
@@ -1586,7 +1604,7 @@ int chx_lookup_decode_hamming(chx_ctx* ctx, const chx_lkey* kc, const uint32_t* 
     if (count == 0) return;
     const int L = k->W;
     check_below(c, count, L, k->n2.N, "c");
-    const size_t stride = round_up(count, 64);
+    const size_t stride = batch_stride(count);
     uint32_t* dc = upload_il(ctx, S_IN0, c, count, L, stride);
     ctx->slot[S_IN1].ensure(L * 4);
     hip_check(hipMemcpyAsync(ctx->slot[S_IN1].p, mask, L * 4, hipMemcpyHostToDevice, ctx->stream), "H2D");
@@ -1603,16 +1621,16 @@ int chx_lookup_decode_hamming(chx_ctx* ctx, const chx_lkey* kc, const uint32_t* 
 // ---------------------------------------------------------------------------
 int chx_aos_to_il_dev(chx_ctx* ctx, const uint32_t* aos, uint32_t* il, size_t count, int limbs, size_t stride) {
   return with_ctx(ctx, [&] {
-    if (!aos || !il || limbs <= 0) fail(CHX_EINVAL, "bad argument");
-    if (stride < count) fail(CHX_EINVAL, "stride below count");
+    if (!aos || !il || limbs <= 0 || limbs > 4096) fail(CHX_EINVAL, "bad argument");
+    if (stride < count || stride > kMaxStride) fail(CHX_EINVAL, "stride %zu below count or above the maximum", stride);
     if (count) aos_to_il(ctx, aos, il, count, limbs, stride);
   });
 }
 
 int chx_il_to_aos_dev(chx_ctx* ctx, const uint32_t* il, uint32_t* aos, size_t count, int limbs, size_t stride) {
   return with_ctx(ctx, [&] {
-    if (!aos || !il || limbs <= 0) fail(CHX_EINVAL, "bad argument");
-    if (stride < count) fail(CHX_EINVAL, "stride below count");
+    if (!aos || !il || limbs <= 0 || limbs > 4096) fail(CHX_EINVAL, "bad argument");
+    if (stride < count || stride > kMaxStride) fail(CHX_EINVAL, "stride %zu below count or above the maximum", stride);
     if (count) il_to_aos(ctx, il, aos, count, limbs, stride);
   });
 }
@@ -1677,7 +1695,7 @@ int chx_probe_copy(chx_ctx* ctx, const uint32_t* src, uint32_t* dst, size_t coun
   return with_ctx(ctx, [&] {
     if (!src || !dst) fail(CHX_EINVAL, "null argument");
     if (count == 0) return;
-    if (stride < round_up(count, 64)) fail(CHX_EINVAL, "stride %zu below count rounded to 64", stride);
+    check_stride(count, stride);
     bool done = false;
 #define CHX_COPY_CASE(L_, T_)                                                                                  \
     if (limbs == L_ && tpi == T_) {                                                                            \

@@ -143,3 +143,19 @@ def test_modexp_each_every_variant(ctx, limbs):
             want = [pow(b, e, n) for b, e in zip(bases, exps)]
             bad = [i for i, (g, w) in enumerate(zip(got, want)) if g != w]
             assert not bad, f"tpi={tpi} window={window} first bad {bad[:5]}"
+
+
+def test_oversize_batches_rejected(ctx):
+    """Counts or strides beyond the 2^28 bound are refused before any launch
+    (kernels index items with 32-bit integers)."""
+    import torch
+    from cuhepy_amd.engine import Modulus
+    mod = Modulus(ctx, (1 << 2047) + 12345, 64)
+    t = torch.zeros((64, 64), dtype=torch.int32, device="cuda:0")
+    big = 1 << 29
+    with pytest.raises(ValueError):
+        mod.mulmod_dev(t, t, t, big, big)
+    with pytest.raises(ValueError):
+        mod.modexp_dev(t, t, big, big, 65537)
+    with pytest.raises(ValueError):
+        ctx.aos_to_il(t, t, 64, 64, big)
