@@ -60,6 +60,11 @@ struct Lazy {
   static constexpr int FR = !FOLD ? LP : (K <= FMAX ? K * (FMAX / K) : K / 2);
   static_assert(FR <= FMAX || !FOLD, "fold interval exceeds the accumulator bound");
   static_assert(!FOLD || K <= FMAX || (K % 2 == 0 && K / 2 <= FMAX), "block too long to fold at half");
+  // Symmetric squaring (row_sq) puts up to 3 products' worth into a position
+  // per row; it is used when LP such rows fit without folds.
+  static constexpr int FMAX_SQ =
+      (int)((TWO64 - 4.0 * (double)(1ull << (64 - RB))) / (3.0 * (double)(1ull << (2 * RB))));
+  static constexpr bool SYM = !FOLD && LP <= FMAX_SQ;
   static constexpr int SLOTS = (LP > L32 ? LP : L32);  // u32 rows of the staging area
   typedef Grp<TPI> grp;
 
@@ -154,22 +159,71 @@ struct Lazy {
         const int q = (p + S) % K;
         V[q] = (u64)a * b[p] + V[q];
       }
-      const u32 m = grp::bcast0(((u32)V[S] * n0) & MASK);
-#pragma unroll
-      for (int p = 0; p < K; ++p) {
-        const int q = (p + S) % K;
-        V[q] = (u64)m * N[p] + V[q];
-      }
-      // Every lane carries the bits of its bottom position above RB into its
-      // next position and passes only the low RB bits down to the lane below
-      // (value unchanged). Rank 0's low bits are 0 after m*N, so the group's
-      // top rank receives 0 from the next group without a select.
-      const u64 v0 = V[S];
-      V[(S + 1) % K] += v0 >> RB;
-      V[S] = (u64)grp::shift_down_z0((u32)v0 & MASK);
+      reduce<S>(V);
       if constexpr (FOLD && FR < K && (S + 1) % FR == 0) fold<(S + 1) % K>(V);
       row<S + 1>(V, b, arow);
     }
+  }
+
+  // Second half of row S: add m*N for the digit m that clears relative
+  // position 0, then shift down one position.
+  template <int S>
+  __device__ __forceinline__ void reduce(u64 (&V)[K]) {
+    const u32 m = grp::bcast0(((u32)V[S] * n0) & MASK);
+#pragma unroll
+    for (int p = 0; p < K; ++p) {
+      const int q = (p + S) % K;
+      V[q] = (u64)m * N[p] + V[q];
+    }
+    // Every lane carries the bits of its bottom position above RB into its
+    // next position and passes only the low RB bits down to the lane below
+    // (value unchanged). Rank 0's low bits are 0 after m*N, so the group's
+    // top rank receives 0 from the next group without a select.
+    const u64 v0 = V[S];
+    V[(S + 1) % K] += v0 >> RB;
+    V[S] = (u64)grp::shift_down_z0((u32)v0 & MASK);
+  }
+
+  // Squaring rows (operand x staged in LDS and held in b). Term x_i x_j of the
+  // square belongs at absolute position i + j. The row loop meets it twice: in
+  // row i at relative position j, and in row j at relative position i. In row
+  // S of a block, lane position p meets the terms with i mod K = S and
+  // j mod K = p, and its mirror has the two residues swapped. So each row adds
+  // 2 x_i x_j where p > S, nothing where p < S (the mirror, which has p > S,
+  // adds the doubled term), and the single term x_i x_j where p = S (the
+  // mirror also has p = S and adds the other half; for i = j it is the square
+  // term). The choice depends only on (S, p), so every lane of the wave runs
+  // the same code: K(K+1)/2 product MADs per block instead of K^2. Each term
+  // is still added in a row <= i + j, before the reduction of its position.
+  // A position receives at most 2 x_i x_j + m N_p < 3 * 2^(2 RB) per row.
+  template <int S>
+  __device__ __forceinline__ void row_sq(u64 (&V)[K], const u32 (&b)[K], const u32* arow) {
+    if constexpr (S < K) {
+      const u32 a = arow[S * G];
+      const u32 a2 = a << 1;
+      V[(2 * S) % K] = (u64)a * b[S] + V[(2 * S) % K];
+#pragma unroll
+      for (int p = S + 1; p < K; ++p) {
+        const int q = (p + S) % K;
+        V[q] = (u64)a2 * b[p] + V[q];
+      }
+      reduce<S>(V);
+      row_sq<S + 1>(V, b, arow);
+    }
+  }
+
+  // b = b^2 R'^-1 mod N (below 2N), in place; b also staged in LDS.
+  __device__ __forceinline__ void sqr_staged(u32 (&b)[K]) {
+    const int g = grp::gid();
+    u64 V[K];
+#pragma unroll
+    for (int p = 0; p < K; ++p) V[p] = 0;
+#pragma unroll 1
+    for (int blk = 0; blk < TPI; ++blk) {
+      const u32* arow = lds + (blk * K) * G + g;
+      row_sq<0>(V, b, arow);
+    }
+    normalize(b, V);
   }
 
   // x = x * y (both below 2N)
@@ -183,7 +237,10 @@ struct Lazy {
   __device__ __forceinline__ void sqr(u32 (&x)[K]) {
     stage(x);
     wave_lds_sync();
-    mul_staged(x);
+    if constexpr (SYM)
+      sqr_staged(x);
+    else
+      mul_staged(x);
     wave_lds_sync();
   }
 
