@@ -486,8 +486,9 @@ int pick_lazy(chx_ctx* ctx, int L32, size_t count, int op = CHX_OP_MODEXP) {
   return last;
 }
 
-size_t lazy_lds_bytes(int lp, int l32, int tpi) {
-  return (size_t)WAVES_PER_BLOCK * (size_t)std::max(lp, l32) * (64 / tpi) * 4;
+template <int RB, int LP, int TPI, int L32>
+size_t lazy_lds_bytes() {
+  return (size_t)WAVES_PER_BLOCK * Lazy<RB, LP, TPI, L32>::AREA * 4;
 }
 
 #define CHX_DISPATCH_LAZY(L32_, IDX_, ...)                                                        \
@@ -529,7 +530,7 @@ void run_lazy(chx_ctx* ctx, int L32, const LazySet* sets, const ExpSched* const*
   record_start(ctx, "lazy_modexp");
   CHX_DISPATCH_LAZY(L32, ci, {
     hipLaunchKernelGGL((k_lazy_modexp<RB, LP, TPI, L32>), grid_for(count, TPI, njobs), dim3(BLOCK_THREADS),
-                       lazy_lds_bytes(LP, L32, TPI), ctx->stream, jb[0], jb[1], (int)count, (int)stride, tentries,
+                       (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, jb[0], jb[1], (int)count, (int)stride, tentries,
                        (int)tstride);
   });
   record_stop(ctx);
@@ -572,7 +573,7 @@ void launch_modexp_each(chx_ctx* ctx, const LazySet& ls, const uint32_t* base, c
   record_start(ctx, "lazy_modexp_each");
   CHX_DISPATCH_LAZY(L, ci, {
     hipLaunchKernelGGL((k_lazy_modexp_each<RB, LP, TPI, L32>), grid_for(count, TPI), dim3(BLOCK_THREADS),
-                       lazy_lds_bytes(LP, L32, TPI), ctx->stream, ls.d[ci], base, exps, e_limbs, nwin, w, out,
+                       (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, ls.d[ci], base, exps, e_limbs, nwin, w, out,
                        (int)count, (int)stride, ctx->slot[S_TAB].as<uint32_t>(), (int)tstride);
   });
   record_stop(ctx);
@@ -672,7 +673,7 @@ void launch_lookup_encrypt(chx_lkey* k, const uint8_t* digits, const uint8_t* ni
   record_start(ctx, "lookup_encrypt");
   CHX_DISPATCH_LAZY(L, ci, {
     hipLaunchKernelGGL((k_lazy_lookup_encrypt<RB, LP, TPI, L32>), grid_for(count, TPI), dim3(BLOCK_THREADS),
-                       lazy_lds_bytes(LP, L32, TPI), ctx->stream, k->lz_n2.d[ci], k->d_gtab, k->d_ntab, digits,
+                       (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, k->lz_n2.d[ci], k->d_gtab, k->d_ntab, digits,
                        k->chunks, nidx, nmult, k->noise, c, (int)count, (int)stride);
   });
   record_stop(ctx);
@@ -696,7 +697,7 @@ uint32_t* lazy_table_upload(chx_ctx* ctx, const LazySet& ls, const uint32_t* hos
   hip_check(hipMemcpyAsync(src, host, entries * L * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream), "H2D");
   CHX_DISPATCH_LAZY(L, ci, {
     hipLaunchKernelGGL((k_lazy_table<RB, LP, TPI, L32>), grid_for(entries, TPI), dim3(BLOCK_THREADS),
-                       lazy_lds_bytes(LP, L32, TPI), ctx->stream, ls.d[ci], src, dst, (int)entries);
+                       (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, ls.d[ci], src, dst, (int)entries);
   });
   check_launch("k_lazy_table");
   hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
@@ -895,7 +896,7 @@ static void launch_operands(chx_ctx* ctx, const chx_mod* m, const uint32_t* y, u
   const int L = m->hc.L;
   CHX_DISPATCH_LAZY(L, kServeCfg, {
     hipLaunchKernelGGL((k_lazy_table<RB, LP, TPI, L32>), grid_for(count, TPI), dim3(BLOCK_THREADS),
-                       lazy_lds_bytes(LP, L32, TPI), ctx->stream, m->lz.d[kServeCfg], y, ops, (int)count);
+                       (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, m->lz.d[kServeCfg], y, ops, (int)count);
   });
   check_launch("k_lazy_table");
 }
@@ -944,7 +945,7 @@ int chx_mulmod_multi_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* x, size
     record_start(ctx, "mulmod_multi");
     CHX_DISPATCH_LAZY(L, kServeCfg, {
       hipLaunchKernelGGL((k_lazy_mulmod_multi<RB, LP, TPI, L32>), grid_for(count, TPI), dim3(BLOCK_THREADS),
-                         lazy_lds_bytes(LP, L32, TPI), ctx->stream, m->lz.d[kServeCfg], x, (int)begin, (int)count,
+                         (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, m->lz.d[kServeCfg], x, (int)begin, (int)count,
                          (int)stride, mq, (int)out_stride);
     });
     record_stop(ctx);
@@ -1096,7 +1097,7 @@ void ragged_enqueue(chx_ragged* rg) {
   auto mixed = [&](auto e1, auto e2, auto kernel) {
     typedef decltype(e1) E1;
     typedef decltype(e2) E2;
-    constexpr int AREA = (E1::SLOTS * E1::G > E2::SLOTS * E2::G) ? E1::SLOTS * E1::G : E2::SLOTS * E2::G;
+    constexpr int AREA = E1::AREA > E2::AREA ? E1::AREA : E2::AREA;
     hipLaunchKernelGGL(kernel, dim3(rg->blocks), dim3(BLOCK_THREADS), (size_t)WAVES_PER_BLOCK * AREA * 4,
                        ctx->stream, rg->d_q[0], rg->q_nsubs[0], rg->q_tiles[0], rg->d_counter, rg->window,
                        rg->d_scratch, rg->wave_words);
@@ -1116,7 +1117,7 @@ void ragged_enqueue(chx_ragged* rg) {
     CHX_DISPATCH_LAZY(L, ci, {
       typedef Lazy<RB, LP, TPI, L32> E;
       hipLaunchKernelGGL((k_ragged1<RB, LP, TPI, L32>), dim3(rg->blocks), dim3(BLOCK_THREADS),
-                         (size_t)WAVES_PER_BLOCK * E::SLOTS * E::G * 4, ctx->stream, rg->d_q[q], rg->q_nsubs[q],
+                         (size_t)WAVES_PER_BLOCK * E::AREA * 4, ctx->stream, rg->d_q[q], rg->q_nsubs[q],
                          rg->q_tiles[q], rg->d_counter + q, rg->window, rg->d_scratch, rg->wave_words);
     });
   }
@@ -1259,7 +1260,7 @@ int chx_ragged_run_plain(chx_ragged* rg) {
       const size_t tstride = round_up(u.count, BLOCK_THREADS);
       CHX_DISPATCH_LAZY(L, ci, {
         hipLaunchKernelGGL((k_lazy_modexp_each<RB, LP, TPI, L32>), grid_for(u.count, TPI), dim3(BLOCK_THREADS),
-                           lazy_lds_bytes(LP, L32, TPI), ctx->stream, rg->hsubs[s].md, u.base, u.exps,
+                           (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, rg->hsubs[s].md, u.base, u.exps,
                            (int)u.e_limbs, rg->hsubs[s].nwin, rg->window, u.out, (int)u.count, (int)u.stride,
                            rg->d_plain_tab, (int)tstride);
       });

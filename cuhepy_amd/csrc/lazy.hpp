@@ -66,15 +66,20 @@ struct Lazy {
       (int)((TWO64 - 4.0 * (double)(1ull << (64 - RB))) / (3.0 * (double)(1ull << (2 * RB))));
   static constexpr bool SYM = !FOLD && LP <= FMAX_SQ;
   static constexpr int SLOTS = (LP > L32 ? LP : L32);  // u32 rows of the staging area
+  // u32 words of one wave's staging area: squarings stage (x_i, 2 x_i) pairs
+  static constexpr int AREA = (SYM && 2 * LP > SLOTS ? 2 * LP : SLOTS) * G;
   typedef Grp<TPI> grp;
 
-  u32* lds;     // this wave's staging area: SLOTS rows of G u32
+  u32* lds;     // this wave's staging area: AREA u32 words
   u32 n0;       // -N^-1 mod 2^RB
+  u32 maskv;    // MASK held in a VGPR: a DPP move folds into an AND with it
   u32 N[K];     // modulus positions (this lane's slice)
 
   __device__ __forceinline__ void init(u32* wave_area, const u32* __restrict__ npos, u32 n0inv) {
     lds = wave_area;
     n0 = n0inv;
+    maskv = MASK;
+    asm volatile("" : "+v"(maskv));
     const int r = grp::rank();
 #pragma unroll
     for (int p = 0; p < K; ++p) N[p] = npos[r * K + p];
@@ -106,8 +111,24 @@ struct Lazy {
     for (int p = 0; p < K; ++p) lds[(r * K + p) * G + g] = x[p];
   }
 
-  // carry-resolve accumulators into RB-bit positions (value < R' guaranteed)
-  __device__ __forceinline__ void normalize(u32 (&X)[K], const u64 (&V)[K]) {
+  // stage the operand of a squaring: row i = the pair (x_i, 2 x_i) (u64)
+  __device__ __forceinline__ void stage_sq(const u32 (&x)[K]) {
+    const int g = TPI == 1 ? grp::gid() : opaque(grp::gid()), r = TPI == 1 ? 0 : opaque(grp::rank());
+    u64* l64 = reinterpret_cast<u64*>(lds);
+#pragma unroll
+    for (int p = 0; p < K; ++p) l64[(r * K + p) * G + g] = make64(x[p], x[p] << 1);
+  }
+
+  // Settle accumulators into positions for the next product (value < R'
+  // guaranteed, so nothing leaves the group's top position). Each lane ripples
+  // its own positions, then passes its carry (< 2^(64-RB)) one lane up, where
+  // it is absorbed by positions 0 and 1 only: every position ends below
+  // D = 2^RB + 2^(64-2RB) + 1 instead of 2^RB. Products of such positions stay
+  // below 2^(2RB) (1 + 2^(66-3RB)), which the row bounds above absorb (they
+  // leave more than 10% of 2^64 unused). renorm() finishes the job where exact
+  // RB-bit positions are needed (to32, top32).
+  __device__ __forceinline__ void settle(u32 (&X)[K], const u64 (&V)[K]) {
+    static_assert(TPI == 1 || K >= 2, "settle needs two positions per lane");
     u64 c = 0;
 #pragma unroll
     for (int p = 0; p < K; ++p) {
@@ -116,14 +137,30 @@ struct Lazy {
       c = t >> RB;
     }
     if constexpr (TPI > 1) {
+      const u64 cin = make64(grp::shift_up((u32)c), grp::shift_up((u32)(c >> 32)));
+      const u64 t = (u64)X[0] + cin;
+      X[0] = (u32)t & MASK;
+      X[1] += (u32)(t >> RB);
+    }
+  }
+
+  // Exact RB-bit positions from settled ones (value < R'), in place.
+  __device__ __forceinline__ void renorm(u32 (&X)[K]) {
+    u32 c = 0;
+#pragma unroll
+    for (int p = 0; p < K; ++p) {
+      const u32 t = X[p] + c;
+      X[p] = t & MASK;
+      c = t >> RB;
+    }
+    if constexpr (TPI > 1) {
 #pragma unroll 1
       for (int round = 1; round < TPI; ++round) {
-        u64 cin = make64(grp::shift_up((u32)c), grp::shift_up((u32)(c >> 32)));
-        c = 0;
+        u32 cin = grp::shift_up(c);
 #pragma unroll
         for (int p = 0; p < K; ++p) {
-          const u64 t = (u64)X[p] + cin;
-          X[p] = (u32)t & MASK;
+          const u32 t = X[p] + cin;
+          X[p] = t & MASK;
           cin = t >> RB;
         }
         c = cin;
@@ -146,7 +183,7 @@ struct Lazy {
         if ((blk + 1) % (FR / K) == 0) fold<0>(V);  // wave-uniform
       }
     }
-    normalize(b, V);
+    settle(b, V);
   }
 
   // rows S..K-1 of one block (unrolled by recursion over S)
@@ -169,7 +206,7 @@ struct Lazy {
   // position 0, then shift down one position.
   template <int S>
   __device__ __forceinline__ void reduce(u64 (&V)[K]) {
-    const u32 m = grp::bcast0(((u32)V[S] * n0) & MASK);
+    const u32 m = grp::bcast0((u32)V[S] * n0) & maskv;
 #pragma unroll
     for (int p = 0; p < K; ++p) {
       const int q = (p + S) % K;
@@ -181,7 +218,7 @@ struct Lazy {
     // top rank receives 0 from the next group without a select.
     const u64 v0 = V[S];
     V[(S + 1) % K] += v0 >> RB;
-    V[S] = (u64)grp::shift_down_z0((u32)v0 & MASK);
+    V[S] = (u64)(grp::shift_down_z0((u32)v0) & maskv);
   }
 
   // Squaring rows (operand x staged in LDS and held in b). Term x_i x_j of the
@@ -197,10 +234,10 @@ struct Lazy {
   // is still added in a row <= i + j, before the reduction of its position.
   // A position receives at most 2 x_i x_j + m N_p < 3 * 2^(2 RB) per row.
   template <int S>
-  __device__ __forceinline__ void row_sq(u64 (&V)[K], const u32 (&b)[K], const u32* arow) {
+  __device__ __forceinline__ void row_sq(u64 (&V)[K], const u32 (&b)[K], const u64* arow) {
     if constexpr (S < K) {
-      const u32 a = arow[S * G];
-      const u32 a2 = a << 1;
+      const u64 pr = arow[S * G];
+      const u32 a = (u32)pr, a2 = (u32)(pr >> 32);
       V[(2 * S) % K] = (u64)a * b[S] + V[(2 * S) % K];
 #pragma unroll
       for (int p = S + 1; p < K; ++p) {
@@ -212,7 +249,7 @@ struct Lazy {
     }
   }
 
-  // b = b^2 R'^-1 mod N (below 2N), in place; b also staged in LDS.
+  // b = b^2 R'^-1 mod N (below 2N), in place; b also staged in LDS (stage_sq).
   __device__ __forceinline__ void sqr_staged(u32 (&b)[K]) {
     const int g = grp::gid();
     u64 V[K];
@@ -220,10 +257,10 @@ struct Lazy {
     for (int p = 0; p < K; ++p) V[p] = 0;
 #pragma unroll 1
     for (int blk = 0; blk < TPI; ++blk) {
-      const u32* arow = lds + (blk * K) * G + g;
+      const u64* arow = reinterpret_cast<const u64*>(lds) + (blk * K) * G + g;
       row_sq<0>(V, b, arow);
     }
-    normalize(b, V);
+    settle(b, V);
   }
 
   // x = x * y (both below 2N)
@@ -235,12 +272,15 @@ struct Lazy {
   }
 
   __device__ __forceinline__ void sqr(u32 (&x)[K]) {
-    stage(x);
-    wave_lds_sync();
-    if constexpr (SYM)
+    if constexpr (SYM) {
+      stage_sq(x);
+      wave_lds_sync();
       sqr_staged(x);
-    else
+    } else {
+      stage(x);
+      wave_lds_sync();
       mul_staged(x);
+    }
     wave_lds_sync();
   }
 
@@ -261,8 +301,8 @@ struct Lazy {
     wave_lds_sync();
   }
 
-  // bit 32*L32 of a normalized value below 2^(32 L32 + 1), the same in every
-  // lane of the group (the carry word for cond_sub)
+  // bit 32*L32 of a value below 2^(32 L32 + 1) in exact positions (renorm),
+  // the same in every lane of the group (the carry word for cond_sub)
   __device__ __forceinline__ u32 top32(const u32 (&x)[K]) {
     constexpr int q = (32 * L32) / RB, off = (32 * L32) % RB;
     static_assert(q < LP, "bit 32*L32 lies inside the positions");
@@ -271,7 +311,7 @@ struct Lazy {
     return grp::group_bits(__ballot(b != 0)) != 0ull ? 1u : 0u;
   }
 
-  // positions (normalized, value < 2^(32 L32)) -> 32-bit limbs (lane slice)
+  // exact positions (renorm; value < 2^(32 L32)) -> 32-bit limbs (lane slice)
   __device__ __forceinline__ void to32(u32 (&w)[K32], const u32 (&x)[K]) {
     const int g = TPI == 1 ? grp::gid() : opaque(grp::gid()), r = TPI == 1 ? 0 : opaque(grp::rank());
     stage(x);
@@ -325,7 +365,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_modexp(LazyJob j0, LazyJ
   typedef Lazy<RB, LP, TPI, L32> E;
   constexpr int K = E::K, K32 = E::K32;
   const LazyJob jb = blockIdx.y == 0 ? j0 : j1;
-  u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * (E::SLOTS * E::G);
+  u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * E::AREA;
   E M;
   M.init(area, jb.md->N, jb.md->n0);
   const int item = group_item<TPI>();
@@ -390,6 +430,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_modexp(LazyJob j0, LazyJ
     one[0] = r == 0 ? 1u : 0u;
     M.mul_into(x, one);
   }
+  M.renorm(x);
   M.to32(w, x);
   u32 n32[K32];
 #pragma unroll
@@ -411,7 +452,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_table(const LazyDev* __r
   extern __shared__ uint4 smem4[];
   typedef Lazy<RB, LP, TPI, L32> E;
   constexpr int K = E::K, K32 = E::K32;
-  u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * (E::SLOTS * E::G);
+  u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * E::AREA;
   E M;
   M.init(area, md->N, md->n0);
   const int item = group_item<TPI>();
@@ -448,7 +489,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_lookup_encrypt(const Laz
   extern __shared__ uint4 smem4[];
   typedef Lazy<RB, LP, TPI, L32> E;
   constexpr int K = E::K, K32 = E::K32;
-  u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * (E::SLOTS * E::G);
+  u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * E::AREA;
   E M;
   M.init(area, md->N, md->n0);
   const int item = group_item<TPI>();
@@ -483,6 +524,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_lookup_encrypt(const Laz
     M.mul_into(acc, one);
   }
   u32 w[K32];
+  M.renorm(acc);
   M.to32(w, acc);
   u32 n32[K32];
 #pragma unroll
@@ -572,6 +614,7 @@ __device__ __forceinline__ void modexp_each_item(Lazy<RB, LP, TPI, L32>& M, cons
     M.mul_into(x, one);
   }
   u32 wv[K32];
+  M.renorm(x);
   M.to32(wv, x);
   u32 n32[K32];
 #pragma unroll
@@ -598,7 +641,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_modexp_each(const LazyDe
                                                                     int tstride) {
   extern __shared__ uint4 smem4[];
   typedef Lazy<RB, LP, TPI, L32> E;
-  u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * (E::SLOTS * E::G);
+  u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * E::AREA;
   E M;
   M.init(area, md->N, md->n0);
   const int item = group_item<TPI>();
@@ -633,7 +676,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_mulmod_multi(const LazyD
   extern __shared__ uint4 smem4[];
   typedef Lazy<RB, LP, TPI, L32> E;
   constexpr int K = E::K, K32 = E::K32;
-  u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * (E::SLOTS * E::G);
+  u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * E::AREA;
   E M;
   M.init(area, md->N, md->n0);
   const int local = group_item<TPI>();
@@ -654,6 +697,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_mulmod_multi(const LazyD
     wave_lds_sync();
     M.mul_staged(y);
     wave_lds_sync();
+    M.renorm(y);
     const u32 hi = M.top32(y);
     M.to32(w, y);
     cond_sub<K32, TPI>(w, hi, n32);
@@ -723,7 +767,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_ragged1(const RaggedSub* __re
                                                            u32* __restrict__ scratch, size_t wave_words) {
   extern __shared__ uint4 smem4[];
   typedef Lazy<RB, LP, TPI, L32> E;
-  u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * (E::SLOTS * E::G);
+  u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * E::AREA;
   u32* table = scratch + ((size_t)blockIdx.x * (BLOCK_THREADS / 64) + threadIdx.x / 64) * wave_words;
   int local = 0;
   for (int s; (s = ragged_next(counter, total_tiles, subs, nsubs, &local)) >= 0;)
@@ -739,7 +783,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_ragged2(const RaggedSub* __re
   extern __shared__ uint4 smem4[];
   typedef Lazy<RB1, LP1, TPI1, 64> E1;
   typedef Lazy<RB2, LP2, TPI2, 128> E2;
-  constexpr int AREA = (E1::SLOTS * E1::G > E2::SLOTS * E2::G) ? E1::SLOTS * E1::G : E2::SLOTS * E2::G;
+  constexpr int AREA = E1::AREA > E2::AREA ? E1::AREA : E2::AREA;
   u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * AREA;
   u32* table = scratch + ((size_t)blockIdx.x * (BLOCK_THREADS / 64) + threadIdx.x / 64) * wave_words;
   int local = 0;

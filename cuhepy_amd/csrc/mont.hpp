@@ -49,7 +49,9 @@ template <int TPI> struct Grp {
       // every lane has a source in its quad: no "old" value to initialise
       return (u32)__builtin_amdgcn_mov_dpp((int)v, 0xA0, 0xF, 0xF, false);  // quad_perm [0,0,2,2]
     } else if constexpr (TPI == 4) {
-      return (u32)__builtin_amdgcn_mov_dpp((int)v, 0x00, 0xF, 0xF, false);  // quad_perm [0,0,0,0]
+      // quad_perm [0,0,0,0]; every source lane exists, so bound_ctrl and the
+      // old value 0 never take effect (they let the move fold into a VOP2 user)
+      return (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x00, 0xF, 0xF, true);
     } else if constexpr (TPI == 8 || TPI == 16) {
       // quad broadcast, then copy quads up the row: row_shr:4 into banks 1 and 3
       // (lanes 4-7, 12-15 of each row of 16); for 16 lanes also row_shr:8 into
