@@ -348,7 +348,7 @@ struct LazyDev {
 // blockIdx.y selects one of two (modulus, schedule, input, output) sets so the
 // CRT halves run in one launch. Input: 32-bit limbs (any value < 2^(32 L32));
 // output: canonical 32-bit limbs. Window table: positions, per item, in global
-// scratch ([entry][position][item]).
+// scratch ([entry][item][position]: a lane's positions are contiguous).
 struct LazyJob {
   const LazyDev* md;
   const u32* sched;
@@ -387,11 +387,11 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_modexp(LazyJob j0, LazyJ
   u32* tab = jb.table;
   auto tstore = [&](int e, const u32 (&v)[K]) {
 #pragma unroll
-    for (int p = 0; p < K; ++p) tab[((size_t)e * LP + r * K + p) * tstride + item] = v[p];
+    for (int p = 0; p < K; ++p) tab[((size_t)e * tstride + item) * LP + r * K + p] = v[p];
   };
   auto tload = [&](int e, u32 (&v)[K]) {
 #pragma unroll
-    for (int p = 0; p < K; ++p) v[p] = tab[((size_t)e * LP + r * K + p) * tstride + item];
+    for (int p = 0; p < K; ++p) v[p] = tab[((size_t)e * tstride + item) * LP + r * K + p];
   };
   tstore(0, x);
   if (tentries > 1) {
@@ -544,7 +544,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_lookup_encrypt(const Laz
 // covers the widest exponent of the batch.
 // One item per group: out[item] = in[item]^exps[item] mod N, fixed window
 // `w`, `nwin` windows. Table: 2^w entries of LP positions at
-// table[(e * LP + pos) * tstride + slot]. `it` is the clamped item used for
+// table[(e * tstride + slot) * LP + pos]. `it` is the clamped item used for
 // loads (groups past `count` compute a duplicate and skip the store).
 template <int RB, int LP, int TPI, int L32>
 __device__ __forceinline__ void modexp_each_item(Lazy<RB, LP, TPI, L32>& M, const LazyDev* __restrict__ md,
@@ -557,11 +557,11 @@ __device__ __forceinline__ void modexp_each_item(Lazy<RB, LP, TPI, L32>& M, cons
   const int r = TPI == 1 ? 0 : opaque(Grp<TPI>::rank());
   auto tstore = [&](int e, const u32 (&v)[K]) {
 #pragma unroll
-    for (int p = 0; p < K; ++p) table[((size_t)e * LP + r * K + p) * tstride + slot] = v[p];
+    for (int p = 0; p < K; ++p) table[((size_t)e * tstride + slot) * LP + r * K + p] = v[p];
   };
   auto tload = [&](int e, u32 (&v)[K]) {
 #pragma unroll
-    for (int p = 0; p < K; ++p) v[p] = table[((size_t)e * LP + r * K + p) * tstride + slot];
+    for (int p = 0; p < K; ++p) v[p] = table[((size_t)e * tstride + slot) * LP + r * K + p];
   };
   auto digit = [&](int win) -> int {
     const int bit = win * w;
