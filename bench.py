@@ -217,6 +217,33 @@ class _TableRows:
         return _Row()
 
 
+def _lookup_products(digits, items_per_wave: int, nmult: int) -> int:
+    """Table products the lookup kernel executes for this digit matrix: per
+    wave, one per row pair with a nonzero digit when every digit of the pair is
+    in the packing alphabet, else one per row with a nonzero digit; plus the
+    noise factors and the final conversion of every item (all items of a wave
+    run every product the wave runs)."""
+    import numpy as np
+    count, chunks = digits.shape
+    waves = -(-count // items_per_wave)
+    pad = np.zeros((waves * items_per_wave, chunks), dtype=digits.dtype)
+    pad[:count] = digits
+    d = pad.reshape(waves, items_per_wave, chunks)
+    per_wave = np.zeros(waves, dtype=np.int64)
+    for j in range(0, chunks - 1, 2):
+        pair = d[:, :, j:j + 2]
+        pairable = ~((pair & 0xAA) != 0).any(axis=(1, 2))
+        nz_pair = (pair != 0).any(axis=(1, 2))
+        nz_rows = (pair != 0).any(axis=1).sum(axis=1)
+        per_wave += np.where(pairable, nz_pair.astype(np.int64), nz_rows)
+    if chunks % 2:
+        per_wave += (d[:, :, -1] != 0).any(axis=1)
+    # every item of a wave takes part in every product the wave runs
+    real = np.full(waves, items_per_wave, dtype=np.int64)
+    real[-1] = count - (waves - 1) * items_per_wave
+    return int((per_wave * real).sum()) + (nmult + 1) * count
+
+
 def lookup_leg(ctx, args, peak: float) -> dict:
     """Paillier-lookup table encryption: 100,000 binary vectors of 512 bits
     (Bernoulli(0.5)), key_len 1024 (n 2048-bit, n^2 4096-bit), alpha 280,
@@ -225,7 +252,8 @@ def lookup_leg(ctx, args, peak: float) -> dict:
     import numpy as np
     import torch
 
-    from cuhepy_amd.engine import stride_for
+    from cuhepy_amd._native import OP_LOOKUP
+    from cuhepy_amd.engine import Context, stride_for
     from cuhepy_amd.limbs import limbs_to_ints, width_for_bits
     from cuhepy_amd.paillier._lookup_gpu_ext import PaillierGPULookupClient
     from cuhepy_amd.paillier.packing import embeddings_to_plaintexts, plaintexts_to_digits
@@ -251,22 +279,33 @@ def lookup_leg(ctx, args, peak: float) -> dict:
     d_idx = torch.from_numpy(nidx).to(dev)
     out = torch.zeros((W, stride), dtype=torch.int32, device=dev)
     torch.cuda.synchronize()
-    key.encrypt_digits_dev(d_dig, d_idx, 14, out, count, stride)  # warm
-    torch.cuda.synchronize()
-    ctx.reset_timing()
-    ctx.set_timing(True)
-    t0 = time.perf_counter()
-    for _ in range(args.lookup_steps):
-        key.encrypt_digits_dev(d_dig, d_idx, 14, out, count, stride)
-    torch.cuda.synchronize()
-    wall = (time.perf_counter() - t0) / args.lookup_steps
-    ms, nl = ctx.kernel_time("lookup_encrypt")
-    ctx.set_timing(False)
-    launch_s = ms / max(nl, 1) / 1e3
+
+    def timed(rows: int) -> tuple[float, float]:
+        """(kernel seconds per launch, wall seconds per batch) with `rows`
+        8-bit table rows per product (1 = single rows, 0 = row pairs)."""
+        ctx.set_tuning(OP_LOOKUP, W, 0, rows)
+        key.encrypt_digits_dev(d_dig, d_idx, 14, out, count, stride)  # warm
+        torch.cuda.synchronize()
+        ctx.reset_timing()
+        ctx.set_timing(True)
+        t0 = time.perf_counter()
+        for _ in range(args.lookup_steps):
+            key.encrypt_digits_dev(d_dig, d_idx, 14, out, count, stride)
+        torch.cuda.synchronize()
+        w = (time.perf_counter() - t0) / args.lookup_steps
+        ms, nl = ctx.kernel_time("lookup_encrypt")
+        ctx.set_timing(False)
+        return ms / max(nl, 1) / 1e3, w
+
+    single_s, _ = timed(1)
+    launch_s, wall = timed(0)  # row pairs (the default); its output is checked below
+    ctx.set_tuning(OP_LOOKUP, W, 0, 0)
     nnz = int((digits != 0).sum())
     mulmods = nnz + 13 * count
     macs = mulmods * 2 * W * W
     achieved = macs / launch_s / 1e12
+    executed = _lookup_products(digits, 64 // Context.supported_tpi(W, op=OP_LOOKUP)[0], 14)
+    executed_tmac = executed * 2 * W * W / launch_s / 1e12
     # round trip: every ciphertext decrypts to its plaintext (GPU decrypt)
     aos = torch.empty((count, W), dtype=torch.int32, device=dev)
     ctx.il_to_aos(out, aos, count, W, stride)
@@ -287,7 +326,9 @@ def lookup_leg(ctx, args, peak: float) -> dict:
     got = limbs_to_ints(c_aos[sample])
     f_bad = sum(int(a != b) for a, b in zip(got, want))
     log(f"lookup: {count} vectors in {launch_s * 1e3:.2f} ms ({count / launch_s:,.0f} vectors/s, "
-        f"{mulmods / launch_s:,.0f} mulmod/s, {achieved:.2f} TMAC/s, {achieved / peak:.1%}); round-trip "
+        f"{mulmods / launch_s:,.0f} mulmod/s, {achieved:.2f} TMAC/s, {achieved / peak:.1%}); row pairs execute "
+        f"{executed / count:.1f} products per vector ({executed_tmac:.2f} TMAC/s executed, "
+        f"{executed_tmac / peak:.1%}); single rows {single_s * 1e3:.2f} ms; round-trip "
         f"mismatches {rt_bad}; oracle mismatches {f_bad}/{len(sample)}; CPU {len(sample) / cpu_s:.1f} vectors/s")
     if rt_bad or f_bad:
         raise RuntimeError(f"lookup encryption wrong: round trip {rt_bad}, oracle {f_bad}")
@@ -305,6 +346,10 @@ def lookup_leg(ctx, args, peak: float) -> dict:
         "mulmod_per_s": round(mulmods / launch_s, 1),
         "achieved_tmac_s": round(achieved, 4),
         "frac_of_peak": round(achieved / peak, 4),
+        "row_pairs": {"executed_products": executed, "products_per_vector": round(executed / count, 2),
+                      "executed_tmac_s": round(executed_tmac, 4), "executed_frac_of_peak": round(executed_tmac / peak, 4),
+                      "single_rows_kernel_ms": round(single_s * 1e3, 3),
+                      "speedup_over_single_rows": round(single_s / launch_s, 3)},
         "wall_ms_per_batch": round(wall * 1e3, 3),
         "client_path_vectors_per_s": round(sl / client_s, 1),
         "bit_exact": {"round_trip_vectors": count, "oracle_vectors": len(sample), "mismatches": 0},

@@ -174,6 +174,7 @@ struct chx_lkey {
   uint32_t* d_gar = nullptr;   // g_a_inv * R mod n
   uint32_t* d_gtab = nullptr;  // Montgomery-form g table, AoS
   uint32_t* d_ntab = nullptr;  // Montgomery-form noise table, AoS
+  uint32_t* d_ptab = nullptr;  // row-pair table (k_lazy_pair_table), AoS; null if chunks < 2
   LazySet lz_n2;
   BN e_a;
   std::map<int, ExpSched> scheds;
@@ -670,11 +671,14 @@ void launch_lookup_encrypt(chx_lkey* k, const uint8_t* digits, const uint8_t* ni
   if (!k->d_gtab || !k->d_ntab) fail(CHX_EINVAL, "lookup key was created without tables (decrypt-only)");
   const int L = k->W;
   const int ci = pick_lazy(ctx, L, count, CHX_OP_LOOKUP);
+  // rows per table product: the context's lookup window (0 = pairs when built)
+  const int rows = ctx->tune_window[CHX_OP_LOOKUP][lidx(L)];
+  const uint32_t* ptab = rows == 1 ? nullptr : k->d_ptab;
   record_start(ctx, "lookup_encrypt");
   CHX_DISPATCH_LAZY(L, ci, {
     hipLaunchKernelGGL((k_lazy_lookup_encrypt<RB, LP, TPI, L32>), grid_for(count, TPI), dim3(BLOCK_THREADS),
-                       (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, k->lz_n2.d[ci], k->d_gtab, k->d_ntab, digits,
-                       k->chunks, nidx, nmult, k->noise, c, (int)count, (int)stride);
+                       (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, k->lz_n2.d[ci], k->d_gtab, ptab, k->d_ntab,
+                       digits, k->chunks, nidx, nmult, k->noise, c, (int)count, (int)stride);
   });
   record_stop(ctx);
   check_launch("k_lazy_lookup_encrypt");
@@ -702,6 +706,30 @@ uint32_t* lazy_table_upload(chx_ctx* ctx, const LazySet& ls, const uint32_t* hos
   check_launch("k_lazy_table");
   hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
   (void)hipFree(src);
+  return dst;
+}
+
+// row-pair table of `pairs` row pairs from the device g table (both in the
+// positions of configuration 0); returns the buffer (pairs x 256 entries)
+uint32_t* lazy_pair_table(chx_ctx* ctx, const LazySet& ls, const uint32_t* gtab, size_t pairs) {
+  const int L = ls.L32;
+  const LazyCfg cfg = lazy_cfgs(L)[0];
+  const int ci = 0;
+  const size_t entries = pairs * 256;
+  if (entries > (size_t)INT32_MAX / 2) fail(CHX_EINVAL, "too many table rows");
+  uint32_t* dst = nullptr;
+  hipError_t e = hipMalloc(&dst, entries * cfg.lp * sizeof(uint32_t));
+  if (e != hipSuccess) fail(CHX_ENOMEM, "hipMalloc: %s", hipGetErrorString(e));
+  CHX_DISPATCH_LAZY(L, ci, {
+    hipLaunchKernelGGL((k_lazy_pair_table<RB, LP, TPI, L32>), grid_for(entries, TPI), dim3(BLOCK_THREADS),
+                       (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, ls.d[ci], gtab, dst, (int)entries);
+  });
+  hipError_t le = hipGetLastError();
+  if (le != hipSuccess) {
+    (void)hipFree(dst);
+    fail(CHX_EDEVICE, "k_lazy_pair_table: %s", hipGetErrorString(le));
+  }
+  hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
   return dst;
 }
 
@@ -794,6 +822,7 @@ int chx_ctx_set_tuning(chx_ctx* ctx, int op, int limbs, int tpi, int window) {
       if (!ok) fail(CHX_EUNSUPPORTED, "tpi=%d is not built for op %d at %d limbs", tpi, op, limbs);
     }
     if (window < 0 || window > 7) fail(CHX_EINVAL, "window %d out of range [1, 7]", window);
+    if (op == CHX_OP_LOOKUP && window > 2) fail(CHX_EINVAL, "lookup rows per product %d out of range [1, 2]", window);
     ctx->tune_tpi[op][li] = tpi;
     ctx->tune_window[op][li] = window;
   });
@@ -1518,6 +1547,7 @@ int chx_lookup_key_create(chx_ctx* ctx, const uint32_t* n, int n_limbs, const ui
       // tables to the lazy Montgomery domain on the device (once per key)
       k->d_gtab = lazy_table_upload(ctx, k->lz_n2, g_table, (size_t)chunks * 256);
       k->d_ntab = lazy_table_upload(ctx, k->lz_n2, noise_table, (size_t)noise_entries);
+      if (chunks >= 2) k->d_ptab = lazy_pair_table(ctx, k->lz_n2, k->d_gtab, chunks / 2);
     }
     *out = k.release();
   });
@@ -1527,7 +1557,8 @@ int chx_lookup_key_destroy(chx_lkey* k) {
   if (!k) return CHX_OK;
   int rc = guarded([&] {
     hip_check(hipSetDevice(k->ctx->device), "hipSetDevice");
-    for (void* p : {(void*)k->d_n2, (void*)k->d_n, (void*)k->d_gar, (void*)k->d_gtab, (void*)k->d_ntab})
+    for (void* p : {(void*)k->d_n2, (void*)k->d_n, (void*)k->d_gar, (void*)k->d_gtab, (void*)k->d_ntab,
+                    (void*)k->d_ptab})
       if (p) (void)hipFree(p);
     free_lazy_set(k->lz_n2);
     for (auto& s : k->scheds)

@@ -192,10 +192,15 @@ inline std::vector<uint32_t> sliding_schedule(const BN& e, int w) {
   return s;
 }
 
+// Sliding-window width for an e-bit exponent: the w that minimises the
+// multiplies beyond the e squarings, 2^(w-1) - 1 to build the odd powers plus
+// about e / (w + 1) window multiplies (HAC 14.85). Capped at 6, so a per-item
+// table holds at most 32 entries.
 inline int default_window(int ebits) {
-  if (ebits >= 768) return 5;
-  if (ebits >= 192) return 4;
-  if (ebits >= 48) return 3;
+  if (ebits >= 672) return 6;
+  if (ebits >= 240) return 5;
+  if (ebits >= 80) return 4;
+  if (ebits >= 24) return 3;
   return 2;
 }
 

@@ -144,7 +144,11 @@ struct Lazy {
     }
   }
 
-  // Exact RB-bit positions from settled ones (value < R'), in place.
+  // Exact RB-bit positions from settled ones (value < R'), in place. Settled
+  // positions are below 2^RB + 2^(64-2RB) + 1, so every carry of a lane's own
+  // ripple is 0 or 1; the carries between lanes are then resolved at once with
+  // a generate/propagate lookahead over the group (a lane propagates when all
+  // its positions are MASK), and one more ripple adds each lane's carry-in.
   __device__ __forceinline__ void renorm(u32 (&X)[K]) {
     u32 c = 0;
 #pragma unroll
@@ -154,16 +158,16 @@ struct Lazy {
       c = t >> RB;
     }
     if constexpr (TPI > 1) {
-#pragma unroll 1
-      for (int round = 1; round < TPI; ++round) {
-        u32 cin = grp::shift_up(c);
+      u32 all = X[0];
 #pragma unroll
-        for (int p = 0; p < K; ++p) {
-          const u32 t = X[p] + cin;
-          X[p] = t & MASK;
-          cin = t >> RB;
-        }
-        c = cin;
+      for (int p = 1; p < K; ++p) all &= X[p];
+      u32 cout;
+      u32 cin = grp::lookahead(c != 0, all == MASK, &cout);
+#pragma unroll
+      for (int p = 0; p < K; ++p) {
+        const u32 t = X[p] + cin;
+        X[p] = t & MASK;
+        cin = t >> RB;
       }
     }
   }
@@ -471,15 +475,61 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_table(const LazyDev* __r
   }
 }
 
+// Digits of the reference's bit packing ("0 b_0 0 b_1 ...", hamming/paillier.py:81-95):
+// every data bit sits at an even position, so an 8-bit digit is one of the 16
+// values with bits 1, 3, 5, 7 clear. spread4 maps 4 data bits to that digit,
+// squeeze4 back.
+__device__ __forceinline__ int spread4(int v) { return (v & 1) | ((v & 2) << 1) | ((v & 4) << 2) | ((v & 8) << 3); }
+__device__ __forceinline__ int squeeze4(int d) { return (d & 1) | ((d >> 1) & 2) | ((d >> 2) & 4) | ((d >> 3) & 8); }
+constexpr int PACKED_ZERO_BITS = 0xAA;
+
+// Row-pair table for lookup encryption: entry e of pair j is
+//   g_table[2j][spread4(e & 15)] * g_table[2j+1][spread4(e >> 4)] mod n^2
+//   = g^((spread4(e & 15) + 256 spread4(e >> 4)) 2^(16 j)),
+// so one product covers the two 8-bit digits of a packed message (a 16-bit
+// digit, HAC 14.109 with a wider window). In and out: AoS positions in lazy
+// Montgomery form (the product of two such values is the form of the
+// product). One group per entry; entries = pairs * 256.
+template <int RB, int LP, int TPI, int L32>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_pair_table(const LazyDev* __restrict__ md,
+                                                                   const u32* __restrict__ gtab,
+                                                                   u32* __restrict__ out, int entries) {
+  extern __shared__ uint4 smem4[];
+  typedef Lazy<RB, LP, TPI, L32> E;
+  constexpr int K = E::K;
+  u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * E::AREA;
+  E M;
+  M.init(area, md->N, md->n0);
+  const int item = group_item<TPI>();
+  const int it = item < entries ? item : entries - 1;
+  const int r = Grp<TPI>::rank();
+  const int j = it >> 8, e = it & 255;
+  const u32* lo = gtab + ((size_t)(2 * j) * 256 + spread4(e & 15)) * LP + r * K;
+  const u32* hi = gtab + ((size_t)(2 * j + 1) * 256 + spread4(e >> 4)) * LP + r * K;
+  u32 x[K], y[K];
+#pragma unroll
+  for (int p = 0; p < K; ++p) x[p] = lo[p], y[p] = hi[p];
+  M.mul_into(x, y);
+  if (item < entries) {
+#pragma unroll
+    for (int p = 0; p < K; ++p) out[(size_t)item * LP + r * K + p] = x[p];
+  }
+}
+
 // Lookup-table encryption (lookup.py:213-252, lcu:839-948) in the lazy domain:
 //   c = prod_i g_table[i][d_i] * prod_k noise[idx_k] mod n^2
 // Tables are AoS positions in lazy Montgomery form. Rows where every item of
 // the wave has digit 0 are skipped (g_table[i][0] = 1); other items with
 // digit 0 multiply by their row's j = 0 entry (the Montgomery one), so the
-// wave stays convergent and the product is unchanged.
+// wave stays convergent and the product is unchanged. With a row-pair table
+// (ptab, k_lazy_pair_table), rows 2j and 2j+1 take one product from it when
+// every item of the wave has both digits in the packing alphabet; otherwise
+// the wave multiplies the two rows one at a time. Either way the result is
+// the same residue.
 template <int RB, int LP, int TPI, int L32>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_lookup_encrypt(const LazyDev* __restrict__ md,
                                                                        const u32* __restrict__ gtab,
+                                                                       const u32* __restrict__ ptab,
                                                                        const u32* __restrict__ ntab,
                                                                        const unsigned char* __restrict__ digits,
                                                                        int chunks,
@@ -499,11 +549,28 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_lookup_encrypt(const Laz
 #pragma unroll
   for (int p = 0; p < K; ++p) acc[p] = md->R1[r * K + p];
   const unsigned char* dg = digits + (size_t)it * chunks;
-  for (int i = 0; i < chunks; ++i) {
-    const int d = dg[i];
-    if (__ballot(d != 0) == 0ull) continue;  // uniform skip
+  for (int i = 0; i < chunks;) {
+    // one step: a row pair (even i, wave-uniform test) or a single row
+    const int d0 = dg[i];
+    const u32* e = gtab + ((size_t)i * 256 + d0) * LP;
+    bool act;
+    int step = 1;
+    if (ptab != nullptr && (i & 1) == 0 && i + 1 < chunks) {
+      const int d1 = dg[i + 1];
+      if (__ballot(((d0 | d1) & PACKED_ZERO_BITS) != 0) == 0ull) {
+        step = 2;
+        e = ptab + ((size_t)(i >> 1) * 256 + (squeeze4(d0) | (squeeze4(d1) << 4))) * LP;
+        act = __ballot((d0 | d1) != 0) != 0ull;
+      } else {
+        act = __ballot(d0 != 0) != 0ull;
+      }
+    } else {
+      act = __ballot(d0 != 0) != 0ull;
+    }
+    i += step;
+    if (!act) continue;  // every item multiplies by one: uniform skip
     u32 y[K];
-    const u32* e = gtab + ((size_t)i * 256 + d) * LP + r * K;
+    e += r * K;
 #pragma unroll
     for (int p = 0; p < K; ++p) y[p] = e[p];
     M.mul_into(acc, y);

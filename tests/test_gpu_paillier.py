@@ -161,6 +161,40 @@ def test_lookup_seeded_every_variant(ctx):
         assert limbs_to_ints(key.decrypt(got)) == ms, f"decrypt tpi={tpi}"
 
 
+def test_lookup_row_pairs_packed_and_mixed(ctx):
+    """Packed embeddings take the row-pair table (one product per two 8-bit
+    digits); waves with a message outside the packing alphabet fall back to
+    single rows. Both must equal the oracle, with pairs on (window 0/2) and
+    off (window 1)."""
+    from cuhepy_amd.engine import LookupKey
+    from cuhepy_amd.limbs import ints_to_limbs, limbs_to_ints
+    from cuhepy_amd.paillier.packing import plaintexts_to_digits
+    keys = O.lookup_key_gen(512, 280, 4242, with_tables=True)
+    n = keys["pk"]["n"]
+    table = keys["g_table"]
+    key = LookupKey(ctx, n, keys["sk"]["a"], keys["sk"]["g_a_inv"], [table[i] for i in range(len(table))],
+                    keys["noise_table"])
+    rng = random.Random(12)
+    embed_len = 512
+    embs = [[rng.getrandbits(1) for _ in range(embed_len)] for _ in range(70)]
+    embs += [[0] * embed_len, [1] * embed_len, [1] + [0] * (embed_len - 1)]
+    packed = [O.embed_to_plaintexts(e, embed_len, 512)[0] for e in embs]
+    # every 16th message is an arbitrary residue: its wave uses single rows
+    mixed = [rng.randrange(n) if i % 16 == 5 else m for i, m in enumerate(packed)]
+    for ms in (packed, mixed):
+        idx = np.array([[rng.randrange(256) for _ in range(14)] for _ in ms], dtype=np.uint8)
+        want = [O.lookup_encrypt(m, keys["pk"], table, keys["noise_table"], keys["message_chunks"], ix.tolist())
+                for m, ix in zip(ms, idx)]
+        digits = plaintexts_to_digits(ints_to_limbs(ms, key.width), key.chunks)
+        if ms is packed:
+            assert not (digits & 0xAA).any()  # the packing alphabet: the pair path applies
+        for rows in (0, 2, 1):
+            ctx.set_tuning(OP_LOOKUP, key.width, 0, rows)
+            got = limbs_to_ints(key.encrypt_digits(digits, idx))
+            assert got == want, f"rows per product {rows}"
+        assert limbs_to_ints(key.decrypt(got)) == ms
+
+
 def test_plain_encrypt_decrypt_1024_messages():
     """The reference's CPU-runnable case: key_len 1024-bit n (512-bit primes,
     n^2 2048-bit), keypair from seeded primes, 1024 messages m uniform in [0, n)
