@@ -142,6 +142,7 @@ struct chx_mod {
   host::MontConsts hc;
   ModDev* d = nullptr;
   LazySet lz;
+  uint32_t* d_fixb = nullptr;  // served-multiply constants 2^(RB(i+2)) R' mod N (built on first use)
 };
 
 struct ExpSched {
@@ -194,6 +195,20 @@ const std::vector<int>& built_tpi(int limbs) {
 }
 
 // Number of lanes that keeps every SIMD of the chip busy with 2 waves.
+// compute units of `device` (cached; 256 if the query fails)
+int device_cus(int device) {
+  static std::mutex mu;
+  static std::map<int, int> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find(device);
+  if (it != cache.end()) return it->second;
+  hipDeviceProp_t prop;
+  int v = 256;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0) v = prop.multiProcessorCount;
+  cache[device] = v;
+  return v;
+}
+
 size_t target_lanes(int device) {
   static std::mutex mu;
   static std::map<int, size_t> cache;
@@ -878,6 +893,7 @@ int chx_mod_destroy(chx_mod* m) {
   int rc = guarded([&] {
     hip_check(hipSetDevice(m->ctx->device), "hipSetDevice");
     if (m->d) (void)hipFree(m->d);
+    if (m->d_fixb) (void)hipFree(m->d_fixb);
     free_lazy_set(m->lz);
   });
   delete m;
@@ -912,25 +928,49 @@ int chx_mulmod_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* a, const uint
 }
 
 // Lazy configuration of the served multiply (operands are prepared in its
-// positions): configuration 0, measured faster than the 16-lane shape at 4096 bits.
+// positions): configuration 0.
 static constexpr int kServeCfg = 0;
 
+// Query operand of the served multiply: the table Y_i = y 2^(RB (i + 2)) mod N,
+// LP rows of TPI chunks of CH words (k_fixed_operand, k_fixed_mulmod).
 int chx_mulmod_operand_words(const chx_mod* m) {
   if (!m) return 0;
   const auto& cfgs = lazy_cfgs(m->hc.L);
-  return cfgs.empty() ? 0 : cfgs[kServeCfg].lp;
+  if (cfgs.empty()) return 0;
+  const LazyCfg c = cfgs[kServeCfg];
+  const int k = c.lp / c.tpi;
+  return c.lp * c.tpi * ((k + 3) / 4 * 4);
 }
 
-static void launch_operands(chx_ctx* ctx, const chx_mod* m, const uint32_t* y, uint32_t* ops, size_t count) {
+// B_i = 2^(RB (i + 2)) R' mod N as positions (LP rows), uploaded once per modulus
+static const uint32_t* fixed_constants(chx_ctx* ctx, chx_mod* m) {
+  if (m->d_fixb) return m->d_fixb;
+  const LazyCfg c = lazy_cfgs(m->hc.L)[kServeCfg];
+  std::vector<uint32_t> h((size_t)c.lp * c.lp);
+  BN one(m->hc.L, 0u);
+  one[0] = 1;
+  for (int i = 0; i < c.lp; ++i) {
+    BN b = host::bn_shl_mod(one, c.rb * (i + 2 + c.lp), m->hc.N);
+    to_positions(b, c.rb, c.lp, h.data() + (size_t)i * c.lp);
+  }
+  m->d_fixb = dev_upload(h.data(), h.size(), ctx->stream);
+  return m->d_fixb;
+}
+
+static void launch_operands(chx_ctx* ctx, chx_mod* m, const uint32_t* y, uint32_t* ops, size_t count) {
   const int L = m->hc.L;
+  const uint32_t* B = fixed_constants(ctx, m);
+  const LazyCfg c = lazy_cfgs(L)[kServeCfg];
+  if (count * (size_t)c.lp > (size_t)INT32_MAX / 64) fail(CHX_EINVAL, "too many operands (%zu)", count);
   CHX_DISPATCH_LAZY(L, kServeCfg, {
-    hipLaunchKernelGGL((k_lazy_table<RB, LP, TPI, L32>), grid_for(count, TPI), dim3(BLOCK_THREADS),
-                       (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, m->lz.d[kServeCfg], y, ops, (int)count);
+    hipLaunchKernelGGL((k_fixed_operand<RB, LP, TPI, L32>), grid_for(count * LP, TPI), dim3(BLOCK_THREADS),
+                       (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, m->lz.d[kServeCfg], B, y, ops, (int)count);
   });
-  check_launch("k_lazy_table");
+  check_launch("k_fixed_operand");
 }
 
-int chx_mulmod_operands_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* y, uint32_t* ops, size_t count) {
+int chx_mulmod_operands_dev(chx_ctx* ctx, const chx_mod* mc, const uint32_t* y, uint32_t* ops, size_t count) {
+  chx_mod* m = const_cast<chx_mod*>(mc);
   return with_ctx(ctx, [&] {
     if (!m || !y || !ops) fail(CHX_EINVAL, "null argument");
     if (count == 0) return;
@@ -938,7 +978,8 @@ int chx_mulmod_operands_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* y, u
   });
 }
 
-int chx_mulmod_operands(chx_ctx* ctx, const chx_mod* m, const uint32_t* y, uint32_t* ops, size_t count) {
+int chx_mulmod_operands(chx_ctx* ctx, const chx_mod* mc, const uint32_t* y, uint32_t* ops, size_t count) {
+  chx_mod* m = const_cast<chx_mod*>(mc);
   return with_ctx(ctx, [&] {
     if (!m || !y || !ops) fail(CHX_EINVAL, "null argument");
     if (count == 0) return;
@@ -949,6 +990,19 @@ int chx_mulmod_operands(chx_ctx* ctx, const chx_mod* m, const uint32_t* y, uint3
     hip_check(hipMemcpyAsync(ctx->slot[S_AUX].p, y, bytes, hipMemcpyHostToDevice, ctx->stream), "H2D");
     launch_operands(ctx, m, ctx->slot[S_AUX].as<uint32_t>(), ops, count);
   });
+}
+
+// workgroups of k_fixed_mulmod: enough to fill every CU at the occupancy its
+// LDS and registers allow (the kernel strides over the tiles)
+extern "C++" {
+template <class F>
+static unsigned fixed_blocks(chx_ctx* ctx, F kernel, int threads, size_t lds, size_t tiles) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds) != hipSuccess || per_cu < 1)
+    per_cu = 1;
+  const size_t want = (size_t)device_cus(ctx->device) * per_cu;
+  return (unsigned)std::max<size_t>(1, std::min(tiles, want));
+}
 }
 
 int chx_mulmod_multi_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* x, size_t stride, size_t begin,
@@ -971,14 +1025,22 @@ int chx_mulmod_multi_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* x, size
       mq.out[q] = outs[q];
     }
     const int L = m->hc.L;
+    const size_t tiles = (count + FIX_TILE - 1) / FIX_TILE;
     record_start(ctx, "mulmod_multi");
     CHX_DISPATCH_LAZY(L, kServeCfg, {
-      hipLaunchKernelGGL((k_lazy_mulmod_multi<RB, LP, TPI, L32>), grid_for(count, TPI), dim3(BLOCK_THREADS),
-                         (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, m->lz.d[kServeCfg], x, (int)begin, (int)count,
-                         (int)stride, mq, (int)out_stride);
+      auto kern = k_fixed_mulmod<RB, LP, TPI, L32>;
+      const size_t lds = (size_t)FixedLds<RB, LP, TPI, L32>::WORDS * 4;
+      static std::once_flag lds_attr;
+      std::call_once(lds_attr, [&] {
+        hip_check(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                  "hipFuncSetAttribute");
+      });
+      hipLaunchKernelGGL(kern, dim3(fixed_blocks(ctx, kern, 64 * TPI, lds, tiles)), dim3(64 * TPI), lds,
+                         ctx->stream, m->lz.d[kServeCfg], x, (int)begin, (int)count, (int)stride, mq,
+                         (int)out_stride);
     });
     record_stop(ctx);
-    check_launch("k_lazy_mulmod_multi");
+    check_launch("k_fixed_mulmod");
   });
 }
 

@@ -724,54 +724,208 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_modexp_each(const LazyDe
 // over gpu_pointwise_mulmod_hex, cu:672-732; paillier_GPU_lookup_client.cu:2312-2329
 // over gpu_pointwise_mulmod_bigint, lcu:1506) multiplies a query ciphertext into every index
 // ciphertext mod n^2. One launch serves up to MQ_MAX queries over one range of
-// the index: each group loads its index entry once and multiplies it by every
-// query operand. Operands are y_q * R' mod N in positions (k_lazy_table), so one
-// almost-Montgomery product gives x * y_q mod N below 2N, made canonical by
-// one conditional subtraction that includes bit 32*L32.
+// the index (k_fixed_mulmod below): the entries are converted to positions once
+// and multiplied by every query's operand table.
 constexpr int MQ_MAX = 32;
 struct MultiQuery {
   int nq;
-  const u32* op[MQ_MAX];  // LP positions each (device)
+  const u32* op[MQ_MAX];  // operand tables (k_fixed_operand), device
   u32* out[MQ_MAX];       // interleaved [L32][out_stride], indexed by the global entry
 };
 
+// ---------------------------------------------------------------------------
+// Served Hamming evaluation with a fixed query operand
+// ---------------------------------------------------------------------------
+// For one query y and many index entries x, x * y mod N is computed without
+// Montgomery reduction rows: with the query's table
+//   Y_i = y * 2^(RB (i + 2)) mod N   (i < LP, lazy: below 2N)
+// and x = sum_i x_i 2^(RB i) in RB-bit positions,
+//   V = sum_i x_i Y_i  ==  x y 2^(2 RB)   (mod N),
+// a product of an LP-vector by an LP x LP matrix: LP^2 multiply-adds instead
+// of the 2 LP^2 of a Montgomery product. Two REDC rows (Montgomery 1985) then
+// divide by 2^(2 RB): the result is x y mod N below
+//   N (LP 2^(RB+1) / 2^(2 RB) + 1) < 2N,
+// and one conditional subtraction makes it canonical. A position of V
+// receives LP products below 2^RB * (2^RB + 2^(64-2RB) + 1) (settled table
+// positions): below 2^64 for every built configuration.
+//
+// Work split of one launch (a workgroup of TPI waves, tiles of 64 entries):
+//  A  the tile's entries to RB-bit positions (from32, TPI lanes per entry),
+//     stored position-major in LDS (xs[i][e]);
+//  B  wave w sums the positions [w K, w K + K) of V for all 64 entries, one
+//     entry per lane: row i adds x_i[e] * Y_i[w K + p] for p < K. The table
+//     chunk is the same for every lane of the wave, so it is read with scalar
+//     loads and enters v_mad_u64_u32 as an SGPR operand;
+//  C  each lane ripples its K positions (RB-bit) and passes them and its
+//     carry through LDS;
+//  D  TPI lanes per entry again: add the carry of the chunk below (the top
+//     chunk keeps its own carry above the top position), two REDC rows,
+//     settle, exact positions, 32-bit limbs, conditional subtraction, store.
+// Table layout per query: LP rows of TPI chunks of CH words (K used).
+template <int K>
+struct FixedShape {
+  static constexpr int CH = (K + 3) / 4 * 4;  // chunk words, a multiple of 4 (scalar loads)
+};
+
+// Query tables: out[q][i][r CH + p] = y_q * 2^(RB (i + 2)) mod N (positions),
+// from y_q (AoS 32-bit limbs, L32 per query) and the modulus constants
+// B_i = 2^(RB (i + 2)) R' mod N (AoS positions, LP per row): one lazy product
+// y_q * B_i R'^-1 per group (q, i).
 template <int RB, int LP, int TPI, int L32>
-__global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_mulmod_multi(const LazyDev* __restrict__ md,
-                                                                     const u32* __restrict__ x, int begin,
-                                                                     int count, int stride, MultiQuery mq,
-                                                                     int out_stride) {
+__global__ __launch_bounds__(BLOCK_THREADS) void k_fixed_operand(const LazyDev* __restrict__ md,
+                                                                 const u32* __restrict__ B,
+                                                                 const u32* __restrict__ y, u32* __restrict__ out,
+                                                                 int count) {
   extern __shared__ uint4 smem4[];
   typedef Lazy<RB, LP, TPI, L32> E;
-  constexpr int K = E::K, K32 = E::K32;
+  constexpr int K = E::K, K32 = E::K32, CH = FixedShape<K>::CH;
   u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * E::AREA;
   E M;
   M.init(area, md->N, md->n0);
-  const int local = group_item<TPI>();
-  const int item = begin + (local < count ? local : count - 1);
+  const int item = group_item<TPI>();
+  const int total = count * LP;
+  const int it = item < total ? item : total - 1;
+  const int q = it / LP, i = it % LP;
   const int r = Grp<TPI>::rank();
-  u32 w[K32], xp[K], n32[K32];
+  u32 w[K32], x[K], b[K];
 #pragma unroll
-  for (int k = 0; k < K32; ++k) w[k] = x[(size_t)(r * K32 + k) * stride + item];
-  M.from32(xp, w);
+  for (int k = 0; k < K32; ++k) w[k] = y[(size_t)q * L32 + r * K32 + k];
+  M.from32(x, w);
 #pragma unroll
-  for (int k = 0; k < K32; ++k) n32[k] = md->N32[r * K32 + k];
-  for (int q = 0; q < mq.nq; ++q) {
-    u32 y[K];
-    const u32* op = mq.op[q];
+  for (int p = 0; p < K; ++p) b[p] = B[(size_t)i * LP + r * K + p];
+  M.mul_into(x, b);
+  if (item < total) {
 #pragma unroll
-    for (int p = 0; p < K; ++p) y[p] = op[r * K + p];
-    M.stage(xp);
-    wave_lds_sync();
-    M.mul_staged(y);
-    wave_lds_sync();
-    M.renorm(y);
-    const u32 hi = M.top32(y);
-    M.to32(w, y);
-    cond_sub<K32, TPI>(w, hi, n32);
-    if (local < count) {
-      u32* out = mq.out[q];
+    for (int p = 0; p < K; ++p) out[((size_t)q * LP + i) * (TPI * CH) + r * CH + p] = x[p];
+  }
+}
+
+constexpr int FIX_TILE = 64;  // index entries per workgroup tile
+// table rows per scalar-load batch (FIX_ROWS x K words of SGPRs): 4, or the
+// largest of 2, 1 that divides LP
+template <int LP>
+struct FixedRows {
+  static constexpr int R = LP % 4 == 0 ? 4 : LP % 2 == 0 ? 2 : 1;
+};
+
+// LDS words of one k_fixed_mulmod workgroup: xs, then ex + cw overlaying the
+// per-wave Lazy staging areas
+template <int RB, int LP, int TPI, int L32>
+struct FixedLds {
+  typedef Lazy<RB, LP, TPI, L32> E;
+  static constexpr int XS = LP * FIX_TILE;
+  static constexpr int EX = FIX_TILE * LP + FIX_TILE * TPI * 2;
+  static constexpr int AREAS = TPI * E::SLOTS * E::G;
+  static constexpr int WORDS = XS + (EX > AREAS ? EX : AREAS);
+};
+
+template <int RB, int LP, int TPI, int L32>
+__global__ __launch_bounds__(64 * TPI) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_fixed_mulmod(const LazyDev* __restrict__ md,
+                                                           const u32* __restrict__ x, int begin, int count,
+                                                           int stride, MultiQuery mq, int out_stride) {
+  extern __shared__ uint4 smem4[];
+  typedef Lazy<RB, LP, TPI, L32> E;
+  typedef FixedLds<RB, LP, TPI, L32> S;
+  constexpr int K = E::K, K32 = E::K32, G = E::G, CH = FixedShape<K>::CH;
+  constexpr u32 MASK = E::MASK;
+  u32* xs = reinterpret_cast<u32*>(smem4);  // [LP][FIX_TILE]
+  u32* ex = xs + S::XS;                     // [FIX_TILE][LP]
+  u64* cw = reinterpret_cast<u64*>(ex + FIX_TILE * LP);  // [FIX_TILE][TPI]
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
+  u32* area = ex + wave * (E::SLOTS * G);  // Lazy staging (phases A and D), overlays ex
+  const int lane = lane_id();
+  const int g = Grp<TPI>::gid(), r = Grp<TPI>::rank();
+  const int tiles = (count + FIX_TILE - 1) / FIX_TILE;
+  for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    const int base = tile * FIX_TILE;
+    // A: positions of this wave's G entries (TPI lanes each) into xs
+    {
+      const int e = wave * G + g;
+      const int local = base + e < count ? base + e : count - 1;
+      const size_t item = (size_t)begin + local;
+      u32 w[K32], xp[K];
 #pragma unroll
-      for (int k = 0; k < K32; ++k) out[(size_t)(r * K32 + k) * out_stride + item] = w[k];
+      for (int k = 0; k < K32; ++k) w[k] = x[(size_t)(r * K32 + k) * stride + item];
+      E M;
+      M.lds = area;  // from32 uses only the staging area
+      M.from32(xp, w);
+#pragma unroll
+      for (int p = 0; p < K; ++p) xs[(r * K + p) * FIX_TILE + e] = xp[p];
+    }
+    __syncthreads();
+    for (int q = 0; q < mq.nq; ++q) {
+      // B: positions [wave K, wave K + K) of V for entry `lane`
+      // the table does not change during the launch: read it through the
+      // constant address space, so the wave-uniform loads are scalar
+      typedef __attribute__((address_space(4))) const u32 cu32;
+      const cu32* Y = (const cu32*)(mq.op[q] + wave * CH);
+      u64 acc[K];
+#pragma unroll
+      for (int p = 0; p < K; ++p) acc[p] = 0;
+      // Rows in batches of FIX_ROWS: the batch's scalar loads (and its x
+      // reads) are issued together, so one wait covers them all; a scalar
+      // load can return out of order, so every use of one waits for all.
+      constexpr int FIX_ROWS = FixedRows<LP>::R;
+#pragma unroll 1
+      for (int i0 = 0; i0 < LP; i0 += FIX_ROWS) {
+        u32 yv[FIX_ROWS][K], xv[FIX_ROWS];
+#pragma unroll
+        for (int j = 0; j < FIX_ROWS; ++j) {
+          const cu32* yr = Y + (size_t)(i0 + j) * (TPI * CH);
+#pragma unroll
+          for (int p = 0; p < K; ++p) yv[j][p] = yr[p];
+          xv[j] = xs[(i0 + j) * FIX_TILE + lane];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < FIX_ROWS; ++j) {
+#pragma unroll
+          for (int p = 0; p < K; ++p) acc[p] = (u64)xv[j] * yv[j][p] + acc[p];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      // C: ripple to RB-bit positions; the carry leaves through LDS
+      {
+        u64 c = 0;
+#pragma unroll
+        for (int p = 0; p < K; ++p) {
+          const u64 t = acc[p] + c;
+          ex[lane * LP + wave * K + p] = (u32)t & MASK;
+          c = t >> RB;
+        }
+        cw[lane * TPI + wave] = c;
+      }
+      __syncthreads();
+      // D: TPI lanes per entry; reduction and canonical output
+      const int e = wave * G + g;
+      u64 V[K];
+#pragma unroll
+      for (int p = 0; p < K; ++p) V[p] = ex[e * LP + r * K + p];
+      if (r > 0) V[0] += cw[e * TPI + r - 1];
+      if (r == TPI - 1) V[K - 1] += cw[e * TPI + TPI - 1] << RB;
+      __syncthreads();  // ex is read everywhere before the staging areas (overlaying it) are written
+      E M;  // the modulus slice is loaded here only, so it is not live during B
+      M.init(area, md->N, md->n0);
+      M.template reduce<0>(V);
+      M.template reduce<1>(V);
+      u64 U[K];
+#pragma unroll
+      for (int p = 0; p < K; ++p) U[p] = V[(p + 2) % K];  // undo the two rows' register rotation
+      u32 X[K], w[K32], n32[K32];
+      M.settle(X, U);
+      M.renorm(X);
+      const u32 hi = M.top32(X);
+      M.to32(w, X);
+#pragma unroll
+      for (int k = 0; k < K32; ++k) n32[k] = md->N32[r * K32 + k];
+      cond_sub<K32, TPI>(w, hi, n32);
+      if (base + e < count) {
+        u32* out = mq.out[q];
+        const size_t item = (size_t)begin + base + e;
+#pragma unroll
+        for (int k = 0; k < K32; ++k) out[(size_t)(r * K32 + k) * out_stride + item] = w[k];
+      }
+      __syncthreads();  // staging areas and ex are reused by the next query or tile
     }
   }
 }
