@@ -843,9 +843,13 @@ __global__ __launch_bounds__(64 * TPI) __attribute__((amdgpu_waves_per_eu(4, 8))
       const int e = wave * G + g;
       const int local = base + e < count ? base + e : count - 1;
       const size_t item = (size_t)begin + local;
+      // an opaque stride keeps the per-limb addresses from being hoisted out
+      // of the tile loop (they would be held, or spilled, across it)
+      const size_t so = (size_t)opaque(stride);
+      const u32* xb = x + (size_t)(r * K32) * so + item;
       u32 w[K32], xp[K];
 #pragma unroll
-      for (int k = 0; k < K32; ++k) w[k] = x[(size_t)(r * K32 + k) * stride + item];
+      for (int k = 0; k < K32; ++k) w[k] = xb[(size_t)k * so];
       E M;
       M.lds = area;  // from32 uses only the staging area
       M.from32(xp, w);
@@ -920,10 +924,10 @@ __global__ __launch_bounds__(64 * TPI) __attribute__((amdgpu_waves_per_eu(4, 8))
       for (int k = 0; k < K32; ++k) n32[k] = md->N32[r * K32 + k];
       cond_sub<K32, TPI>(w, hi, n32);
       if (base + e < count) {
-        u32* out = mq.out[q];
-        const size_t item = (size_t)begin + base + e;
+        const size_t so = (size_t)opaque(out_stride);
+        u32* ob = mq.out[q] + (size_t)(r * K32) * so + (size_t)begin + base + e;
 #pragma unroll
-        for (int k = 0; k < K32; ++k) out[(size_t)(r * K32 + k) * out_stride + item] = w[k];
+        for (int k = 0; k < K32; ++k) ob[(size_t)k * so] = w[k];
       }
       __syncthreads();  // staging areas and ex are reused by the next query or tile
     }
