@@ -2,13 +2,15 @@
 //
 // Numbers are held in LP positions of RB bits (RB = 28 or 27) inside 64-bit
 // accumulators. A limb product is < 2^(2 RB), and a position receives at most
-// 2 products per row for LP rows, so 2 * LP * 2^(2 RB) < 2^64 and no carry has
+// 2 products per row of a general product (3 products' worth in a squaring
+// row, see row_sq) for LP rows, so 3 * LP * 2^(2 RB) < 2^64 and no carry has
 // to be propagated inside the product: every limb product is exactly one
 // in-place v_mad_u64_u32 (64-bit accumulator as the addend). Carries are
-// resolved once per product. A wider radix (RB = 29: 10% fewer positions at
-// 2048 bits) overflows after 31 rows and needs carry folds inside the product
-// (`fold`, supported below); measured on the MI355X it was 14-20% slower than
-// RB = 28 (the folds and the lost occupancy cost more than the saved MACs).
+// resolved once per product (settle). A wider radix (RB = 29: 10% fewer
+// positions at 2048 bits) overflows after 31 rows and needs carry folds inside
+// the product (`fold`, supported below); measured on the MI355X it was 14-20%
+// slower than RB = 28 (the folds and the lost occupancy cost more than the
+// saved MACs).
 //
 // Montgomery product with R' = 2^(RB*LP) > 4N ("almost Montgomery", Walter
 // 2002): inputs and outputs stay below 2N, so no conditional subtraction is
@@ -19,9 +21,10 @@
 // a_i * b + m_i * N and shifts down one position. The K rows of a block are
 // fully unrolled and the accumulator registers are renamed instead of moved
 // (position p of row s lives in register (p + s) mod K), so the shift costs no
-// instructions inside a lane; across lanes the bottom position moves one lane
-// down with two DPP row_shl moves (lo/hi words). a_i comes from LDS (one
-// group-uniform read per row), m_i from rank 0 by DPP broadcast.
+// instructions inside a lane; across lanes each lane's bottom position passes
+// its low RB bits one lane down (one masked DPP row_shl) and carries the rest
+// into its own next position. a_i comes from LDS (one group-uniform read per
+// row), m_i from rank 0 by a masked DPP broadcast.
 #pragma once
 #include "mont.hpp"
 
