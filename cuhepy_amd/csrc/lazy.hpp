@@ -753,8 +753,9 @@ struct MultiQuery {
 // positions): below 2^64 for every built configuration.
 //
 // Work split of one launch (a workgroup of TPI waves, tiles of 64 entries):
-//  A  the tile's entries to RB-bit positions (from32, TPI lanes per entry),
-//     stored position-major in LDS (xs[i][e]);
+//  A  wave w converts positions [w K, w K + K) of the tile's 64 entries (one
+//     entry per lane) from the index limbs, stored position-major in LDS
+//     (xs[i][e]);
 //  B  wave w sums the positions [w K, w K + K) of V for all 64 entries, one
 //     entry per lane: row i adds x_i[e] * Y_i[w K + p] for p < K. The table
 //     chunk is the same for every lane of the wave, so it is read with scalar
@@ -822,6 +823,39 @@ struct FixedLds {
   static constexpr int WORDS = XS + (EX > AREAS ? EX : AREAS);
 };
 
+// Positions [W K, W K + K) of one index entry (K = LP / TPI): position q is
+// bits [RB q, RB q + RB) of the entry's 32-bit limbs xb[j * so]. With W a
+// template argument every limb index and bit offset is a constant: the limbs
+// the chunk spans are loaded once, each position is one funnel shift and a
+// mask. Stored to out[q * FIX_TILE].
+template <int RB, int LP, int TPI, int L32, int W>
+__device__ __forceinline__ void fixed_positions_w(const u32* __restrict__ xb, size_t so, u32* out) {
+  constexpr int K = LP / TPI;
+  constexpr u32 MASK = (1u << RB) - 1u;
+  constexpr int J0 = (RB * W * K) >> 5;                  // first limb of the chunk
+  constexpr int J1 = (RB * (W * K + K) - 1) >> 5;        // last limb (may exceed L32 - 1)
+  constexpr int NL = J1 - J0 + 2;                        // limbs read, plus one above
+  u32 lim[NL];
+#pragma unroll
+  for (int t = 0; t < NL; ++t) lim[t] = J0 + t < L32 ? xb[(size_t)(J0 + t) * so] : 0u;
+#pragma unroll
+  for (int p = 0; p < K; ++p) {
+    const int q = W * K + p;
+    const int b = RB * q, t = (b >> 5) - J0, o = b & 31;
+    out[q * FIX_TILE] = (u32)(make64(lim[t], lim[t + 1]) >> o) & MASK;
+  }
+}
+
+template <int RB, int LP, int TPI, int L32, int W = 0>
+__device__ __forceinline__ void fixed_positions(int wave, const u32* __restrict__ xb, size_t so, u32* out) {
+  if constexpr (W < TPI) {
+    if (wave == W)
+      fixed_positions_w<RB, LP, TPI, L32, W>(xb, so, out);
+    else
+      fixed_positions<RB, LP, TPI, L32, W + 1>(wave, xb, so, out);
+  }
+}
+
 template <int RB, int LP, int TPI, int L32>
 __global__ __launch_bounds__(64 * TPI) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_fixed_mulmod(const LazyDev* __restrict__ md,
                                                            const u32* __restrict__ x, int begin, int count,
@@ -841,23 +875,15 @@ __global__ __launch_bounds__(64 * TPI) __attribute__((amdgpu_waves_per_eu(4, 8))
   const int tiles = (count + FIX_TILE - 1) / FIX_TILE;
   for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
     const int base = tile * FIX_TILE;
-    // A: positions of this wave's G entries (TPI lanes each) into xs
+    // A: wave w converts positions [w K, w K + K) of all 64 entries (one
+    // entry per lane) straight from the index limbs into xs
     {
-      const int e = wave * G + g;
-      const int local = base + e < count ? base + e : count - 1;
-      const size_t item = (size_t)begin + local;
+      const int local = base + lane < count ? base + lane : count - 1;
       // an opaque stride keeps the per-limb addresses from being hoisted out
       // of the tile loop (they would be held, or spilled, across it)
       const size_t so = (size_t)opaque(stride);
-      const u32* xb = x + (size_t)(r * K32) * so + item;
-      u32 w[K32], xp[K];
-#pragma unroll
-      for (int k = 0; k < K32; ++k) w[k] = xb[(size_t)k * so];
-      E M;
-      M.lds = area;  // from32 uses only the staging area
-      M.from32(xp, w);
-#pragma unroll
-      for (int p = 0; p < K; ++p) xs[(r * K + p) * FIX_TILE + e] = xp[p];
+      const u32* xb = x + (size_t)begin + local;
+      fixed_positions<RB, LP, TPI, L32>(wave, xb, so, xs + lane);
     }
     __syncthreads();
     for (int q = 0; q < mq.nq; ++q) {
