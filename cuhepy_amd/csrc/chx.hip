@@ -1029,7 +1029,7 @@ int chx_mulmod_multi_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* x, size
     record_start(ctx, "mulmod_multi");
     CHX_DISPATCH_LAZY(L, kServeCfg, {
       auto kern = k_fixed_mulmod<RB, LP, TPI, L32>;
-      const size_t lds = (size_t)FixedLds<RB, LP, TPI, L32>::WORDS * 4;
+      const size_t lds = (size_t)FixedLds<LP, TPI>::WORDS * 4;
       static std::once_flag lds_attr;
       std::call_once(lds_attr, [&] {
         hip_check(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),

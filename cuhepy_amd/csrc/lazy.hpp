@@ -812,15 +812,20 @@ struct FixedRows {
   static constexpr int R = LP % 4 == 0 ? 4 : LP % 2 == 0 ? 2 : 1;
 };
 
-// LDS words of one k_fixed_mulmod workgroup: xs, then ex + cw overlaying the
-// per-wave Lazy staging areas
-template <int RB, int LP, int TPI, int L32>
+// LDS of one k_fixed_mulmod workgroup (u32 words): the tile's positions
+// xs[LP][64], then the per-query exchange areas of the finish: REDC digits
+// md[2][64], chunk carries cw[TPI][64] (u64), carry lookahead flags
+// gp[TPI][64], each chunk's two lowest positions nb[TPI][2][64], borrow
+// lookahead flags bp[TPI][64].
+template <int LP, int TPI>
 struct FixedLds {
-  typedef Lazy<RB, LP, TPI, L32> E;
   static constexpr int XS = LP * FIX_TILE;
-  static constexpr int EX = FIX_TILE * LP + FIX_TILE * TPI * 2;
-  static constexpr int AREAS = TPI * E::SLOTS * E::G;
-  static constexpr int WORDS = XS + (EX > AREAS ? EX : AREAS);
+  static constexpr int MD = XS;
+  static constexpr int CW = MD + 2 * FIX_TILE;  // even: u64-aligned
+  static constexpr int GP = CW + 2 * TPI * FIX_TILE;
+  static constexpr int NB = GP + TPI * FIX_TILE;
+  static constexpr int BP = NB + 2 * TPI * FIX_TILE;
+  static constexpr int WORDS = BP + TPI * FIX_TILE;
 };
 
 // Positions [W K, W K + K) of one index entry (K = LP / TPI): position q is
@@ -856,22 +861,194 @@ __device__ __forceinline__ void fixed_positions(int wave, const u32* __restrict_
   }
 }
 
+typedef __attribute__((address_space(4))) const u32 cu32;  // read-only, wave-uniform: scalar loads
+
+// Finish of one query for chunk W (wave W, one entry per lane; W a template
+// argument, so every position and limb index is a constant). acc holds
+// positions [W K, W K + K) of V = sum_i x_i Y_i. Steps (each exchange through
+// LDS behind a workgroup barrier, five per query):
+//  1  wave 0 computes the two REDC digits m1, m2 from positions 0 and 1;
+//  2  every chunk adds (m1 + m2 2^RB) N: V is now divisible by 2^(2 RB);
+//  3  each chunk ripples to RB-bit positions and hands its carry (< 2^37) to
+//     the chunk above, which adds it at its position 0 and ripples again: the
+//     carry out is then one bit, and a chunk propagates a carry only if all its
+//     positions are MASK; a generate/propagate lookahead over the chunks
+//     gives each its carry-in, and a last ripple makes every position exact.
+//     The top chunk keeps what leaves position LP-1 (positions LP, LP+1);
+//  4  R = V / 2^(2 RB) (below 2N): chunk W forms the 32-bit limbs whose lowest
+//     bit lies in its positions, with the two lowest positions of the chunk
+//     above for limbs that straddle the boundary;
+//  5  R - N on those limbs; a borrow lookahead over the chunks decides R >= N
+//     (or bit 32 L32 of R set), and each chunk stores R or R - N.
+template <int RB, int LP, int TPI, int L32, int W>
+__device__ __forceinline__ void fixed_finish_w(const LazyDev* __restrict__ md, u64 (&acc)[LP / TPI], u32* lds,
+                                               int lane, u32* __restrict__ ob, size_t so, bool store) {
+  constexpr int K = LP / TPI, T = TPI;
+  constexpr u32 MASK = (1u << RB) - 1u;
+  typedef FixedLds<LP, TPI> S;
+  u32* mdg = lds + S::MD;
+  u64* cw = reinterpret_cast<u64*>(lds + S::CW);
+  u32* gp = lds + S::GP;
+  u32* nb = lds + S::NB;
+  u32* bp = lds + S::BP;
+  const cu32* Nc = (const cu32*)md->N;
+  const cu32* N32 = (const cu32*)md->N32;
+  static_assert(K >= 2, "positions 0 and 1 must lie in chunk 0");
+  // 1: REDC digits (positions 0 and 1 are complete sums: nothing lies below)
+  if constexpr (W == 0) {
+    const u32 n0 = *(const cu32*)&md->n0;
+    const u32 m1 = ((u32)acc[0] * n0) & MASK;
+    const u64 t0 = acc[0] + (u64)m1 * Nc[0];
+    const u64 v1 = acc[1] + (u64)m1 * Nc[1] + (t0 >> RB);
+    const u32 m2 = ((u32)v1 * n0) & MASK;
+    mdg[lane] = m1;
+    mdg[FIX_TILE + lane] = m2;
+  }
+  __syncthreads();
+  // 2: add (m1 + m2 2^RB) N on this chunk (the top chunk's m2 N_(LP-1) lies at
+  // position LP, above every chunk: it joins the top carry in step 3)
+  const u32 m2 = mdg[FIX_TILE + lane];
+  {
+    const u32 m1 = mdg[lane];
+#pragma unroll
+    for (int p = 0; p < K; ++p) {
+      const int q = W * K + p;
+      acc[p] = (u64)m1 * Nc[q] + acc[p];
+      if (q >= 1) acc[p] = (u64)m2 * Nc[q - 1] + acc[p];
+    }
+  }
+  // 3: exact positions
+  u32 X[K];
+  u64 ctop = 0;
+  {
+    u64 c = 0;
+#pragma unroll
+    for (int p = 0; p < K; ++p) {
+      const u64 t = acc[p] + c;
+      X[p] = (u32)t & MASK;
+      c = t >> RB;
+    }
+    if constexpr (W < T - 1) cw[W * FIX_TILE + lane] = c;
+    else ctop = c + (u64)m2 * Nc[LP - 1];  // everything at positions >= LP (below 2^47: V < 2N 2^(2RB))
+  }
+  __syncthreads();
+  {
+    u64 cin = 0;
+    if constexpr (W > 0) cin = cw[(W - 1) * FIX_TILE + lane];
+    u32 all = MASK;
+#pragma unroll
+    for (int p = 0; p < K; ++p) {
+      const u64 t = (u64)X[p] + cin;
+      X[p] = (u32)t & MASK;
+      cin = t >> RB;
+      all &= X[p];
+    }
+    if constexpr (W < T - 1) gp[W * FIX_TILE + lane] = (u32)cin | (all == MASK ? 2u : 0u);
+    else ctop += cin;
+  }
+  __syncthreads();
+  {
+    u32 cin = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const u32 f = gp[w * FIX_TILE + lane];
+      cin = (f & 1u) | ((f >> 1) & cin);
+    }
+#pragma unroll
+    for (int p = 0; p < K; ++p) {
+      const u32 t = X[p] + cin;
+      X[p] = t & MASK;
+      cin = t >> RB;
+    }
+    if constexpr (W == T - 1) ctop += cin;
+  }
+  // 4: limbs of R whose lowest bit lies in this chunk
+  nb[(2 * W) * FIX_TILE + lane] = X[0];
+  nb[(2 * W + 1) * FIX_TILE + lane] = X[1];
+  __syncthreads();
+  u32 up0, up1;  // positions W K + K and W K + K + 1
+  if constexpr (W < T - 1) {
+    up0 = nb[(2 * W + 2) * FIX_TILE + lane];
+    up1 = nb[(2 * W + 3) * FIX_TILE + lane];
+  } else {
+    up0 = (u32)ctop & MASK;
+    up1 = (u32)(ctop >> RB);
+  }
+  auto pos = [&](int q) -> u32 {  // q in [W K, W K + K + 2); 0 above
+    const int o = q - W * K;
+    return o < K ? X[o] : o == K ? up0 : o == K + 1 ? up1 : 0u;
+  };
+  constexpr int B0 = RB * W * K - 2 * RB;        // R bits of this chunk: [B0, B1)
+  constexpr int B1 = RB * (W * K + K) - 2 * RB;
+  constexpr int J0 = B0 <= 0 ? 0 : (B0 + 31) / 32;
+  constexpr int J1 = W == T - 1 ? L32 : ((B1 + 31) / 32 < L32 ? (B1 + 31) / 32 : L32);
+  constexpr int NJ = J1 - J0 > 0 ? J1 - J0 : 1;
+  u32 rl[NJ];
+#pragma unroll
+  for (int j = J0; j < J1; ++j) {
+    const int b = 32 * j + 2 * RB, q = b / RB, o = b % RB;
+    u64 v = (u64)pos(q) >> o;
+    v |= (u64)pos(q + 1) << (RB - o);
+    if (2 * RB - o < 64) v |= (u64)pos(q + 2) << (2 * RB - o);
+    rl[j - J0] = (u32)v;
+  }
+  // 5: R - N, borrow lookahead, store R or R - N
+  u32 bo = 0, eq = 1;
+#pragma unroll
+  for (int j = J0; j < J1; ++j) {
+    const u64 d = (u64)rl[j - J0] - N32[j] - bo;
+    bo = (u32)(d >> 63);
+    eq &= rl[j - J0] == N32[j] ? 1u : 0u;
+  }
+  u32 tb = 0;
+  if constexpr (W == T - 1) {
+    constexpr int b = 32 * L32 + 2 * RB, q = b / RB, o = b % RB;
+    tb = (pos(q) >> o) & 1u;
+  }
+  bp[W * FIX_TILE + lane] = bo | (eq << 1) | (tb << 2);
+  __syncthreads();
+  u32 bin = 0, chain = 0, top = 0;
+#pragma unroll
+  for (int w = 0; w < T; ++w) {
+    const u32 f = bp[w * FIX_TILE + lane];
+    if (w == W) bin = chain;
+    chain = (f & 1u) | ((f >> 1) & 1u & chain);
+    if (w == T - 1) top = (f >> 2) & 1u;
+  }
+  const bool sub = top != 0 || chain == 0;  // R >= N
+  if (store) {
+    u32 b = bin;
+#pragma unroll
+    for (int j = J0; j < J1; ++j) {
+      const u64 d = (u64)rl[j - J0] - N32[j] - b;
+      b = (u32)(d >> 63);
+      ob[(size_t)j * so] = sub ? (u32)d : rl[j - J0];
+    }
+  }
+}
+
+template <int RB, int LP, int TPI, int L32, int W = 0>
+__device__ __forceinline__ void fixed_finish(int wave, const LazyDev* __restrict__ md, u64 (&acc)[LP / TPI],
+                                             u32* lds, int lane, u32* __restrict__ ob, size_t so, bool store) {
+  if constexpr (W < TPI) {
+    if (wave == W)
+      fixed_finish_w<RB, LP, TPI, L32, W>(md, acc, lds, lane, ob, so, store);
+    else
+      fixed_finish<RB, LP, TPI, L32, W + 1>(wave, md, acc, lds, lane, ob, so, store);
+  }
+}
+
 template <int RB, int LP, int TPI, int L32>
 __global__ __launch_bounds__(64 * TPI) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_fixed_mulmod(const LazyDev* __restrict__ md,
                                                            const u32* __restrict__ x, int begin, int count,
                                                            int stride, MultiQuery mq, int out_stride) {
   extern __shared__ uint4 smem4[];
-  typedef Lazy<RB, LP, TPI, L32> E;
-  typedef FixedLds<RB, LP, TPI, L32> S;
-  constexpr int K = E::K, K32 = E::K32, G = E::G, CH = FixedShape<K>::CH;
-  constexpr u32 MASK = E::MASK;
-  u32* xs = reinterpret_cast<u32*>(smem4);  // [LP][FIX_TILE]
-  u32* ex = xs + S::XS;                     // [FIX_TILE][LP]
-  u64* cw = reinterpret_cast<u64*>(ex + FIX_TILE * LP);  // [FIX_TILE][TPI]
+  constexpr int K = LP / TPI, CH = FixedShape<K>::CH;
+  static_assert(LP % TPI == 0, "LP must be a multiple of TPI");
+  u32* lds = reinterpret_cast<u32*>(smem4);
+  u32* xs = lds;  // [LP][FIX_TILE]
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
-  u32* area = ex + wave * (E::SLOTS * G);  // Lazy staging (phases A and D), overlays ex
   const int lane = lane_id();
-  const int g = Grp<TPI>::gid(), r = Grp<TPI>::rank();
   const int tiles = (count + FIX_TILE - 1) / FIX_TILE;
   for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
     const int base = tile * FIX_TILE;
@@ -890,7 +1067,6 @@ __global__ __launch_bounds__(64 * TPI) __attribute__((amdgpu_waves_per_eu(4, 8))
       // B: positions [wave K, wave K + K) of V for entry `lane`
       // the table does not change during the launch: read it through the
       // constant address space, so the wave-uniform loads are scalar
-      typedef __attribute__((address_space(4))) const u32 cu32;
       const cu32* Y = (const cu32*)(mq.op[q] + wave * CH);
       u64 acc[K];
 #pragma unroll
@@ -917,48 +1093,10 @@ __global__ __launch_bounds__(64 * TPI) __attribute__((amdgpu_waves_per_eu(4, 8))
         }
         __builtin_amdgcn_sched_barrier(0);
       }
-      // C: ripple to RB-bit positions; the carry leaves through LDS
-      {
-        u64 c = 0;
-#pragma unroll
-        for (int p = 0; p < K; ++p) {
-          const u64 t = acc[p] + c;
-          ex[lane * LP + wave * K + p] = (u32)t & MASK;
-          c = t >> RB;
-        }
-        cw[lane * TPI + wave] = c;
-      }
-      __syncthreads();
-      // D: TPI lanes per entry; reduction and canonical output
-      const int e = wave * G + g;
-      u64 V[K];
-#pragma unroll
-      for (int p = 0; p < K; ++p) V[p] = ex[e * LP + r * K + p];
-      if (r > 0) V[0] += cw[e * TPI + r - 1];
-      if (r == TPI - 1) V[K - 1] += cw[e * TPI + TPI - 1] << RB;
-      __syncthreads();  // ex is read everywhere before the staging areas (overlaying it) are written
-      E M;  // the modulus slice is loaded here only, so it is not live during B
-      M.init(area, md->N, md->n0);
-      M.template reduce<0>(V);
-      M.template reduce<1>(V);
-      u64 U[K];
-#pragma unroll
-      for (int p = 0; p < K; ++p) U[p] = V[(p + 2) % K];  // undo the two rows' register rotation
-      u32 X[K], w[K32], n32[K32];
-      M.settle(X, U);
-      M.renorm(X);
-      const u32 hi = M.top32(X);
-      M.to32(w, X);
-#pragma unroll
-      for (int k = 0; k < K32; ++k) n32[k] = md->N32[r * K32 + k];
-      cond_sub<K32, TPI>(w, hi, n32);
-      if (base + e < count) {
-        const size_t so = (size_t)opaque(out_stride);
-        u32* ob = mq.out[q] + (size_t)(r * K32) * so + (size_t)begin + base + e;
-#pragma unroll
-        for (int k = 0; k < K32; ++k) ob[(size_t)k * so] = w[k];
-      }
-      __syncthreads();  // staging areas and ex are reused by the next query or tile
+      // C..: reduction, exact positions, canonical 32-bit limbs, store
+      const size_t so = (size_t)opaque(out_stride);
+      fixed_finish<RB, LP, TPI, L32>(wave, md, acc, lds, lane, mq.out[q] + (size_t)begin + base + lane, so,
+                                     base + lane < count);
     }
   }
 }
