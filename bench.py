@@ -615,14 +615,14 @@ def main() -> int:
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     half_ms, half_n = ctx.kernel_time("lazy_modexp")
-    fold_ms, _ = ctx.kernel_time("crt_fold")
-    fin_ms, _ = ctx.kernel_time("crt_finish")
+    fin_ms, fin_n = ctx.kernel_time("crt_finish")
     ctx.set_timing(False)
     elapsed = t1 - t0
     ms_per_step = elapsed / args.steps * 1e3
     modexp_per_step = 2 * count
     value = modexp_per_step * args.steps / elapsed
-    log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step, {value:,.0f} modexp/s")
+    log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step, {value:,.0f} modexp/s "
+        f"(exponentiation {half_ms / max(half_n, 1):.2f} ms, CRT finish {fin_ms / max(fin_n, 1):.2f} ms per step)")
 
     # full-size bit-exact check: CRT decrypt must return the seeded plaintexts
     got = from_il(out_il, H)

@@ -324,7 +324,6 @@ void fill_moddev(ModDev& md, const host::MontConsts& c) {
     md.N[i] = c.N[i];
     md.R1[i] = c.R1[i];
     md.R2[i] = c.R2[i];
-    md.R3[i] = c.R3[i];
   }
 }
 
@@ -459,8 +458,10 @@ LazySet make_lazy_set(const host::MontConsts& c, hipStream_t st) {
     one[0] = 1;
     BN r1 = host::bn_shl_mod(one, rb * lp, c.N);  // R' mod N
     BN r2 = host::bn_shl_mod(r1, rb * lp, c.N);   // R'^2 mod N
+    BN rw = host::bn_shl_mod(r2, 32 * c.L, c.N);  // 2^(32 L32) R'^2 mod N
     to_positions(r1, rb, lp, ld->R1);
     to_positions(r2, rb, lp, ld->R2);
+    to_positions(rw, rb, lp, ld->RW);
     for (int k = 0; k < c.L; ++k) ld->N32[k] = c.N[k];
     ls.d[i] = dev_upload(ld.get(), 1, st);
   }
@@ -522,10 +523,15 @@ size_t lazy_lds_bytes() {
 // One launch exponentiating one or two batches (CRT halves) of `count` items.
 // jobs[j]: input/output interleaved 32-bit buffers, schedule; the modulus
 // block of the chosen configuration is taken from sets[j].
+// in_hi (optional, per job): upper L32 limbs of double-width inputs
 void run_lazy(chx_ctx* ctx, int L32, const LazySet* sets, const ExpSched* const* scheds, const uint32_t* const* in,
-              uint32_t* const* out, int njobs, size_t count, size_t stride) {
+              uint32_t* const* out, int njobs, size_t count, size_t stride,
+              const uint32_t* const* in_hi = nullptr) {
   const int ci = pick_lazy(ctx, L32, count * njobs);
   const LazyCfg cfg = lazy_cfgs(L32)[ci];
+  // a double-width input enters the exponentiation below 4N: R' > 16N keeps
+  // every product of it below 2N
+  if (in_hi && cfg.rb * cfg.lp < 32 * L32 + 4) fail(CHX_EUNSUPPORTED, "lazy configuration too narrow for double-width input");
   int window = scheds[0]->window;
   for (int j = 1; j < njobs; ++j)
     if (scheds[j]->window != window) fail(CHX_EINVAL, "jobs of one launch must share the window width");
@@ -540,6 +546,7 @@ void run_lazy(chx_ctx* ctx, int L32, const LazySet* sets, const ExpSched* const*
     jb[j].sched = scheds[s]->d;
     jb[j].nsched = (int)scheds[s]->h.size();
     jb[j].in = in[s];
+    jb[j].in_hi = in_hi ? in_hi[s] : nullptr;
     jb[j].out = out[s];
     jb[j].table = ctx->slot[S_TAB].as<uint32_t>() + (size_t)s * tab_words;
   }
@@ -642,23 +649,17 @@ void launch_crt(chx_pkey* k, const uint32_t* c, uint32_t* m, size_t count, size_
   const ExpSched& sp = get_sched_p(k, 2, w);
   const ExpSched& sq = get_sched_p(k, 3, w);
   const size_t half_words = (size_t)H * stride;
-  ctx->slot[S_MID0].ensure(2 * half_words * sizeof(uint32_t));
   ctx->slot[S_MID1].ensure(2 * half_words * sizeof(uint32_t));
-  uint32_t* x = ctx->slot[S_MID0].as<uint32_t>();
   uint32_t* u = ctx->slot[S_MID1].as<uint32_t>();
   const int tpi = pick_tpi(ctx, CHX_OP_MULMOD, H, count);
-  record_start(ctx, "crt_fold");
-  CHX_DISPATCH_H(H, tpi, {
-    hipLaunchKernelGGL((k_crt_fold<H, TPI>), grid_for(count, TPI, 2), dim3(BLOCK_THREADS), lds_bytes(H, TPI),
-                       ctx->stream, k->d, c, x, (int)count, (int)stride);
-  });
-  record_stop(ctx);
-  check_launch("k_crt_fold");
+  // both halves read the whole ciphertext (rows [0, H) and [H, 2H)) and reduce
+  // it modulo p^2 and q^2 on entry
   const LazySet sets[2] = {k->lz_p2, k->lz_q2};
   const ExpSched* scheds[2] = {&sp, &sq};
-  const uint32_t* ins[2] = {x, x + half_words};
+  const uint32_t* ins[2] = {c, c};
+  const uint32_t* his[2] = {c + half_words, c + half_words};
   uint32_t* outs[2] = {u, u + half_words};
-  run_lazy(ctx, H, sets, scheds, ins, outs, 2, count, stride);
+  run_lazy(ctx, H, sets, scheds, ins, outs, 2, count, stride, his);
   record_start(ctx, "crt_finish");
   CHX_DISPATCH_H(H, tpi, {
     hipLaunchKernelGGL((k_crt_finish<H, TPI>), grid_for(count, TPI), dim3(BLOCK_THREADS), lds_bytes(H, TPI),

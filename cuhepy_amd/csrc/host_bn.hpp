@@ -119,7 +119,7 @@ struct MontConsts {
   int L = 0;
   uint32_t n0inv = 0;
   int bits = 0;
-  BN N, R1, R2, R3;
+  BN N, R1, R2;
 };
 
 inline MontConsts mont_consts(const BN& N) {
@@ -134,7 +134,6 @@ inline MontConsts mont_consts(const BN& N) {
   if (bn_cmp(one, N) >= 0) throw std::invalid_argument("modulus must exceed 1");
   c.R1 = bn_shl_mod(one, 32 * c.L, N);
   c.R2 = bn_shl_mod(c.R1, 32 * c.L, N);
-  c.R3 = bn_shl_mod(c.R2, 32 * c.L, N);
   return c;
 }
 

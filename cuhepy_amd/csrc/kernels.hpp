@@ -25,7 +25,6 @@ struct ModDev {
   u32 N[LMAX];
   u32 R1[LMAX];  // R mod N      (Montgomery form of 1)
   u32 R2[LMAX];  // R^2 mod N    (to Montgomery form)
-  u32 R3[LMAX];  // R^3 mod N    (Montgomery form of R; folds a 2L-limb input)
 };
 
 // A plain value of up to LMAX limbs (e.g. a constant already in Montgomery form).
@@ -309,32 +308,6 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_encrypt_finish(const Paillier
   M.mul_into(y, c);                  // y*R
   M.mul_into(a, y);                  // (1 + m n) * y
   if (item < count) store_dist<L, TPI>(out, a, item, stride);
-}
-
-// CRT input folding, one half per launch row (blockIdx.y = 0: p^2, 1: q^2):
-// xout[half] = c mod p^2 (canonical) = MontMul(MontMul(c_lo, R^2) + MontMul(c_hi, R^3), 1)
-template <int H, int TPI>
-__global__ __launch_bounds__(BLOCK_THREADS) void k_crt_fold(const PaillierDev* __restrict__ key,
-                                                            const u32* __restrict__ C, u32* __restrict__ xout,
-                                                            int count, int stride) {
-  extern __shared__ uint4 smem[];
-  constexpr int K = H / TPI;
-  const int half = (int)blockIdx.y;
-  const ModDev* sq = half == 0 ? &key->p2 : &key->q2;
-  const int item = group_item<TPI>();
-  const int it = item < count ? item : count - 1;
-  Mont<H, TPI> M;
-  mont_init<H, TPI>(M, sq, smem);
-  u32 lo[K], hi[K], r[K], x[K];
-  load_dist<H, TPI>(lo, C, it, stride);
-  load_dist<H, TPI>(hi, C + (size_t)H * stride, it, stride);
-  load_const<H, TPI>(r, sq->R2);
-  M.mul(x, lo, r);  // c_lo * R
-  load_const<H, TPI>(r, sq->R3);
-  M.mul_into(hi, r);  // c_hi * R^2
-  add_mod<K, TPI>(x, hi, M.N);
-  from_mont<H, TPI>(M, x);
-  if (item < count) store_dist<H, TPI>(xout + (size_t)half * H * stride, x, item, stride);
 }
 
 // CRT finish: m_p = L_p(u_p) h_p mod p, m_q likewise, then Garner

@@ -348,19 +348,23 @@ struct LazyDev {
   u32 N[LPMAX];     // modulus, positions
   u32 R2[LPMAX];    // R'^2 mod N, positions
   u32 R1[LPMAX];    // R' mod N, positions (Montgomery form of 1)
+  u32 RW[LPMAX];    // 2^(32 L32) R'^2 mod N, positions (upper half of a double-width input)
   u32 N32[LMAX];    // modulus, 32-bit limbs (final canonicalisation)
 };
 
 // x^e mod N for a batch; e shared (host-built sliding-window schedule).
 // blockIdx.y selects one of two (modulus, schedule, input, output) sets so the
 // CRT halves run in one launch. Input: 32-bit limbs (any value < 2^(32 L32));
-// output: canonical 32-bit limbs. Window table: positions, per item, in global
-// scratch ([entry][item][position]: a lane's positions are contiguous).
+// with in_hi set, the input is c_lo + 2^(32 L32) c_hi, the upper L32 limbs read
+// from in_hi, so a CRT half reduces the full-width ciphertext itself. Output:
+// canonical 32-bit limbs. Window table: positions, per item, in global scratch
+// ([entry][item][position]: a lane's positions are contiguous).
 struct LazyJob {
   const LazyDev* md;
   const u32* sched;
   int nsched;
   const u32* in;
+  const u32* in_hi;  // nullptr: single-width input
   u32* out;
   u32* table;
 };
@@ -389,6 +393,23 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_modexp(LazyJob j0, LazyJ
 #pragma unroll
     for (int p = 0; p < K; ++p) r2[p] = jb.md->R2[r * K + p];
     M.mul_into(x, r2);  // x * R' (below 2N)
+  }
+  if (jb.in_hi != nullptr) {  // block-uniform
+    // + c_hi 2^(32 L32) R' (below 2N): the sum is below 4N, and the host
+    // admits this form only where R' > 16N, which keeps every product of it
+    // below 2N. Settling the position-wise sum leaves each position at most
+    // 2^RB, and its value below R' leaves nothing above the top position.
+#pragma unroll
+    for (int k = 0; k < K32; ++k) w[k] = jb.in_hi[(size_t)(r * K32 + k) * stride + it];
+    u32 xh[K], rw[K];
+    M.from32(xh, w);
+#pragma unroll
+    for (int p = 0; p < K; ++p) rw[p] = jb.md->RW[r * K + p];
+    M.mul_into(xh, rw);
+    u64 v[K];
+#pragma unroll
+    for (int p = 0; p < K; ++p) v[p] = (u64)x[p] + xh[p];
+    M.settle(x, v);
   }
   // odd powers x, x^3, ..., x^(2^w - 1)
   u32* tab = jb.table;

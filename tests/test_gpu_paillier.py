@@ -76,6 +76,31 @@ def test_seeded_batches_every_variant(ctx, key_len):
         assert limbs_to_ints(key.decrypt_crt(want_c)) == ms, f"crt variant {i}"
 
 
+@pytest.mark.parametrize("key_len", [512, 1024])
+def test_crt_full_width_ciphertext_reduction(ctx, key_len):
+    """Each CRT half reduces the whole ciphertext c = c_lo + 2^(32 H) c_hi
+    modulo p^2 (q^2) on entry: halves of all-ones and zero limbs, multiples of
+    p^2 and q^2 (non-units), and values at the top of [0, n^2)."""
+    from cuhepy_amd.engine import Context, PaillierKey
+    from cuhepy_amd.limbs import limbs_to_ints
+    keys = O.key_gen(key_len, 500 + key_len)
+    n, p, q = keys["pk"]["n"], keys["p"], keys["q"]
+    key = PaillierKey(ctx, n, keys["sk"]["phi"], keys["sk"]["inv"], p, q)
+    n2, half = n * n, 32 * key.half
+    lo = (1 << half) - 1
+    rng = random.Random(key_len)
+    cs = [lo, 1 << half, (1 << half) + lo, p * p, q * q, p * p - 1, q * q + 1, 7 * p * p, n2 - 1,
+          ((n2 - 1) >> half << half) - 1, ((((n2 - 1) >> half) - 1) << half) | (lo >> 1),
+          n2 - 1 - rng.randrange(1 << 64)]
+    cs += [rng.randrange(n2 >> 1, n2) for _ in range(20)]
+    assert all(0 <= c < n2 for c in cs)
+    want = [O.decrypt_crt(c, keys) for c in cs]
+    for tpi in Context.supported_tpi(key.half, op=OP_MODEXP):
+        ctx.set_tuning(OP_MODEXP, key.half, tpi, 0)
+        assert limbs_to_ints(key.decrypt_crt(cs)) == want, f"exp tpi={tpi}"
+    ctx.set_tuning(OP_MODEXP, key.half, 0, 0)
+
+
 def test_decode_hamming_fused(ctx, golden):
     """decrypt + odd-bit popcount on the GPU = plaintext Hamming distance."""
     from cuhepy_amd.engine import Modulus, PaillierKey
