@@ -462,6 +462,19 @@ LazySet make_lazy_set(const host::MontConsts& c, hipStream_t st) {
     to_positions(r1, rb, lp, ld->R1);
     to_positions(r2, rb, lp, ld->R2);
     to_positions(rw, rb, lp, ld->RW);
+    if (rb * lp >= 32 * c.L + rb + 4) {  // Lazy::UNIT: exponentiation modulo N' = kN
+      const uint32_t k = ld->n0;            // -N^-1 mod 2^rb, so kN = -1 mod 2^rb
+      BN nk(c.L + 1, 0u);
+      uint64_t carry = 0;
+      for (int i = 0; i < c.L; ++i) {
+        const uint64_t t = (uint64_t)c.N[i] * k + carry;
+        nk[i] = (uint32_t)t;
+        carry = t >> 32;
+      }
+      nk[c.L] = (uint32_t)carry;
+      if (((nk[0] + 1u) & ((1u << rb) - 1u)) != 0u) fail(CHX_EINVAL, "internal: kN is not -1 mod 2^rb");
+      to_positions(nk, rb, lp, ld->NK);
+    }
     for (int k = 0; k < c.L; ++k) ld->N32[k] = c.N[k];
     ls.d[i] = dev_upload(ld.get(), 1, st);
   }

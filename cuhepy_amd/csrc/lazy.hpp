@@ -68,6 +68,15 @@ struct Lazy {
   static constexpr int FMAX_SQ =
       (int)((TWO64 - 4.0 * (double)(1ull << (64 - RB))) / (3.0 * (double)(1ull << (2 * RB))));
   static constexpr bool SYM = !FOLD && LP <= FMAX_SQ;
+  // Long product chains may run modulo N' = kN with k = -N^-1 mod 2^RB
+  // instead of N: N' = -1 mod 2^RB, so -N'^-1 mod 2^RB = 1 and each row's
+  // digit is the bottom position's low RB bits, with no multiply. Only the
+  // residue mod N matters, and N divides N', so operands need only be
+  // congruent mod N to their Montgomery forms (tables and constants built mod
+  // N serve unchanged) and below 2N'; one last product modulo N itself makes
+  // the result canonical. N' has up to RB more bits than N; UNIT marks the
+  // shapes where R' still exceeds 16 N' (bases below 4N' give products below 2N').
+  static constexpr bool UNIT = RB * LP >= 32 * L32 + RB + 4;
   static constexpr int SLOTS = (LP > L32 ? LP : L32);  // u32 rows of the staging area
   // u32 words of one wave's staging area: squarings stage (x_i, 2 x_i) pairs
   static constexpr int AREA = (SYM && 2 * LP > SLOTS ? 2 * LP : SLOTS) * G;
@@ -176,7 +185,9 @@ struct Lazy {
   }
 
   // b = a*b*R'^-1 mod N (below 2N), in place; a staged in LDS. The result
-  // overwrites b only after the last row has read it.
+  // overwrites b only after the last row has read it. U: the modulus held in
+  // N is = -1 mod 2^RB (digit without multiply, see UNIT).
+  template <bool U = false>
   __device__ __forceinline__ void mul_staged(u32 (&b)[K]) {
     const int g = grp::gid();
     u64 V[K];
@@ -185,7 +196,7 @@ struct Lazy {
 #pragma unroll 1
     for (int blk = 0; blk < TPI; ++blk) {
       const u32* arow = lds + (blk * K) * G + g;
-      row<0>(V, b, arow);
+      row<0, U>(V, b, arow);
       if constexpr (FOLD && FR >= K) {
         if ((blk + 1) % (FR / K) == 0) fold<0>(V);  // wave-uniform
       }
@@ -194,7 +205,7 @@ struct Lazy {
   }
 
   // rows S..K-1 of one block (unrolled by recursion over S)
-  template <int S>
+  template <int S, bool U = false>
   __device__ __forceinline__ void row(u64 (&V)[K], const u32 (&b)[K], const u32* arow) {
     if constexpr (S < K) {
       const u32 a = arow[S * G];
@@ -203,17 +214,21 @@ struct Lazy {
         const int q = (p + S) % K;
         V[q] = (u64)a * b[p] + V[q];
       }
-      reduce<S>(V);
+      reduce<S, U>(V);
       if constexpr (FOLD && FR < K && (S + 1) % FR == 0) fold<(S + 1) % K>(V);
-      row<S + 1>(V, b, arow);
+      row<S + 1, U>(V, b, arow);
     }
   }
 
   // Second half of row S: add m*N for the digit m that clears relative
   // position 0, then shift down one position.
-  template <int S>
+  template <int S, bool U = false>
   __device__ __forceinline__ void reduce(u64 (&V)[K]) {
-    const u32 m = grp::bcast0((u32)V[S] * n0) & maskv;
+    u32 m;
+    if constexpr (U)
+      m = grp::bcast0((u32)V[S]) & maskv;
+    else
+      m = grp::bcast0((u32)V[S] * n0) & maskv;
 #pragma unroll
     for (int p = 0; p < K; ++p) {
       const int q = (p + S) % K;
@@ -240,7 +255,7 @@ struct Lazy {
   // the same code: K(K+1)/2 product MADs per block instead of K^2. Each term
   // is still added in a row <= i + j, before the reduction of its position.
   // A position receives at most 2 x_i x_j + m N_p < 3 * 2^(2 RB) per row.
-  template <int S>
+  template <int S, bool U = false>
   __device__ __forceinline__ void row_sq(u64 (&V)[K], const u32 (&b)[K], const u64* arow) {
     if constexpr (S < K) {
       const u64 pr = arow[S * G];
@@ -251,12 +266,13 @@ struct Lazy {
         const int q = (p + S) % K;
         V[q] = (u64)a2 * b[p] + V[q];
       }
-      reduce<S>(V);
-      row_sq<S + 1>(V, b, arow);
+      reduce<S, U>(V);
+      row_sq<S + 1, U>(V, b, arow);
     }
   }
 
   // b = b^2 R'^-1 mod N (below 2N), in place; b also staged in LDS (stage_sq).
+  template <bool U = false>
   __device__ __forceinline__ void sqr_staged(u32 (&b)[K]) {
     const int g = grp::gid();
     u64 V[K];
@@ -265,28 +281,30 @@ struct Lazy {
 #pragma unroll 1
     for (int blk = 0; blk < TPI; ++blk) {
       const u64* arow = reinterpret_cast<const u64*>(lds) + (blk * K) * G + g;
-      row_sq<0>(V, b, arow);
+      row_sq<0, U>(V, b, arow);
     }
     settle(b, V);
   }
 
   // x = x * y (both below 2N)
+  template <bool U = false>
   __device__ __forceinline__ void mul_into(u32 (&x)[K], const u32 (&y)[K]) {
     stage(y);
     wave_lds_sync();
-    mul_staged(x);
+    mul_staged<U>(x);
     wave_lds_sync();
   }
 
+  template <bool U = false>
   __device__ __forceinline__ void sqr(u32 (&x)[K]) {
     if constexpr (SYM) {
       stage_sq(x);
       wave_lds_sync();
-      sqr_staged(x);
+      sqr_staged<U>(x);
     } else {
       stage(x);
       wave_lds_sync();
-      mul_staged(x);
+      mul_staged<U>(x);
     }
     wave_lds_sync();
   }
@@ -349,6 +367,7 @@ struct LazyDev {
   u32 R2[LPMAX];    // R'^2 mod N, positions
   u32 R1[LPMAX];    // R' mod N, positions (Montgomery form of 1)
   u32 RW[LPMAX];    // 2^(32 L32) R'^2 mod N, positions (upper half of a double-width input)
+  u32 NK[LPMAX];    // N' = kN = -1 mod 2^RB (k = -N^-1 mod 2^RB), positions; Lazy::UNIT shapes only
   u32 N32[LMAX];    // modulus, 32-bit limbs (final canonicalisation)
 };
 
@@ -375,10 +394,16 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_modexp(LazyJob j0, LazyJ
   extern __shared__ uint4 smem4[];
   typedef Lazy<RB, LP, TPI, L32> E;
   constexpr int K = E::K, K32 = E::K32;
+  // U: exponentiate modulo N' = kN (digits without multiply); the result is
+  // taken modulo N by the final product
+  constexpr bool U = E::UNIT;
   const LazyJob jb = blockIdx.y == 0 ? j0 : j1;
   u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * E::AREA;
   E M;
-  M.init(area, jb.md->N, jb.md->n0);
+  if constexpr (U)
+    M.init(area, jb.md->NK, 1u);
+  else
+    M.init(area, jb.md->N, jb.md->n0);
   const int item = group_item<TPI>();
   const int it = item < count ? item : count - 1;
   const int r = Grp<TPI>::rank();
@@ -392,12 +417,12 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_modexp(LazyJob j0, LazyJ
     u32 r2[K];
 #pragma unroll
     for (int p = 0; p < K; ++p) r2[p] = jb.md->R2[r * K + p];
-    M.mul_into(x, r2);  // x * R' (below 2N)
+    M.template mul_into<U>(x, r2);  // = x R' mod N, below 2N' (2N without U)
   }
   if (jb.in_hi != nullptr) {  // block-uniform
-    // + c_hi 2^(32 L32) R' (below 2N): the sum is below 4N, and the host
-    // admits this form only where R' > 16N, which keeps every product of it
-    // below 2N. Settling the position-wise sum leaves each position at most
+    // + c_hi 2^(32 L32) R' (below 2N, or 2N' with U): the sum is below 4N
+    // (4N'), and the host admits this form only where R' > 16N, which keeps
+    // every product of it below 2N (UNIT shapes have R' > 16N' by definition). Settling the position-wise sum leaves each position at most
     // 2^RB, and its value below R' leaves nothing above the top position.
 #pragma unroll
     for (int k = 0; k < K32; ++k) w[k] = jb.in_hi[(size_t)(r * K32 + k) * stride + it];
@@ -405,7 +430,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_modexp(LazyJob j0, LazyJ
     M.from32(xh, w);
 #pragma unroll
     for (int p = 0; p < K; ++p) rw[p] = jb.md->RW[r * K + p];
-    M.mul_into(xh, rw);
+    M.template mul_into<U>(xh, rw);
     u64 v[K];
 #pragma unroll
     for (int p = 0; p < K; ++p) v[p] = (u64)x[p] + xh[p];
@@ -427,13 +452,13 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_modexp(LazyJob j0, LazyJ
     u32 cur[K];
 #pragma unroll
     for (int p = 0; p < K; ++p) cur[p] = x[p];
-    M.sqr(cur);
+    M.template sqr<U>(cur);
     M.stage(cur);
 #pragma unroll
     for (int p = 0; p < K; ++p) cur[p] = x[p];
     wave_lds_sync();
     for (int e = 1; e < tentries; ++e) {
-      M.mul_staged(cur);
+      M.template mul_staged<U>(cur);
       tstore(e, cur);
     }
     wave_lds_sync();
@@ -443,14 +468,16 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_modexp(LazyJob j0, LazyJ
     const u32 op = jb.sched[s];
     const int sq = (int)(op >> 16);
     const int idx = (int)(op & 0xFFFFu);
-    for (int i = 0; i < sq; ++i) M.sqr(x);
+    for (int i = 0; i < sq; ++i) M.template sqr<U>(x);
     if (idx) {
       u32 y[K];
       tload(idx - 1, y);
-      M.mul_into(x, y);
+      M.template mul_into<U>(x, y);
     }
   }
-  // out of Montgomery form: MontMul(x, 1) <= N, then canonical
+  // out of Montgomery form: MontMul(x, 1) <= N, then canonical. With U the
+  // product is taken modulo N itself: x < 2N' < R' gives (x + mN)/R' <= N.
+  if constexpr (U) M.init(area, jb.md->N, jb.md->n0);
   {
     u32 one[K];
 #pragma unroll
@@ -563,9 +590,13 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_lookup_encrypt(const Laz
   extern __shared__ uint4 smem4[];
   typedef Lazy<RB, LP, TPI, L32> E;
   constexpr int K = E::K, K32 = E::K32;
+  constexpr bool U = E::UNIT;  // products modulo N' = kN, the last one modulo N
   u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * E::AREA;
   E M;
-  M.init(area, md->N, md->n0);
+  if constexpr (U)
+    M.init(area, md->NK, 1u);
+  else
+    M.init(area, md->N, md->n0);
   const int item = group_item<TPI>();
   const int it = item < count ? item : count - 1;
   const int r = Grp<TPI>::rank();
@@ -597,7 +628,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_lookup_encrypt(const Laz
     e += r * K;
 #pragma unroll
     for (int p = 0; p < K; ++p) y[p] = e[p];
-    M.mul_into(acc, y);
+    M.template mul_into<U>(acc, y);
   }
   const unsigned char* ni = nidx + (size_t)it * nmult;
   for (int j = 0; j < nmult; ++j) {
@@ -605,8 +636,9 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_lookup_encrypt(const Laz
     const u32* e = ntab + (size_t)(ni[j] % nnoise) * LP + r * K;  // index mod table size, as lcu:933-934
 #pragma unroll
     for (int p = 0; p < K; ++p) y[p] = e[p];
-    M.mul_into(acc, y);
+    M.template mul_into<U>(acc, y);
   }
+  if constexpr (U) M.init(area, md->N, md->n0);  // acc < 2N' < R': MontMul_N(acc, 1) <= N
   {
     u32 one[K];
 #pragma unroll
@@ -645,6 +677,8 @@ __device__ __forceinline__ void modexp_each_item(Lazy<RB, LP, TPI, L32>& M, cons
                                                  int slot) {
   typedef Lazy<RB, LP, TPI, L32> E;
   constexpr int K = E::K, K32 = E::K32;
+  constexpr bool U = E::UNIT;  // products modulo N' = kN, the last one modulo N
+  if constexpr (U) M.init(M.lds, md->NK, 1u);
   const int r = TPI == 1 ? 0 : opaque(Grp<TPI>::rank());
   auto tstore = [&](int e, const u32 (&v)[K]) {
 #pragma unroll
@@ -670,7 +704,7 @@ __device__ __forceinline__ void modexp_each_item(Lazy<RB, LP, TPI, L32>& M, cons
     u32 r2[K];
 #pragma unroll
     for (int p = 0; p < K; ++p) r2[p] = md->R2[r * K + p];
-    M.mul_into(x, r2);
+    M.template mul_into<U>(x, r2);
   }
   {
     u32 cur[K];
@@ -683,20 +717,21 @@ __device__ __forceinline__ void modexp_each_item(Lazy<RB, LP, TPI, L32>& M, cons
     for (int p = 0; p < K; ++p) cur[p] = x[p];
     wave_lds_sync();
     for (int e = 2; e < (1 << w); ++e) {
-      M.mul_staged(cur);
+      M.template mul_staged<U>(cur);
       tstore(e, cur);
     }
     wave_lds_sync();
   }
   tload(digit(nwin - 1), x);
   for (int win = nwin - 2; win >= 0; --win) {
-    for (int i = 0; i < w; ++i) M.sqr(x);
+    for (int i = 0; i < w; ++i) M.template sqr<U>(x);
     const int d = digit(win);
     if (__ballot(d != 0) == 0ull) continue;  // whole wave multiplies by one: skip
     u32 y[K];
     tload(d, y);
-    M.mul_into(x, y);
+    M.template mul_into<U>(x, y);
   }
+  if constexpr (U) M.init(M.lds, md->N, md->n0);  // x < 2N' < R': MontMul_N(x, 1) <= N
   {
     u32 one[K];
 #pragma unroll
