@@ -783,8 +783,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_modexp_each(const LazyDe
 // over gpu_pointwise_mulmod_hex, cu:672-732; paillier_GPU_lookup_client.cu:2312-2329
 // over gpu_pointwise_mulmod_bigint, lcu:1506) multiplies a query ciphertext into every index
 // ciphertext mod n^2. One launch serves up to MQ_MAX queries over one range of
-// the index (k_fixed_mulmod below): the entries are converted to positions once
-// and multiplied by every query's operand table.
+// the index (k_fixed_mulmod below), in (query, tile) work units, query-major.
 constexpr int MQ_MAX = 32;
 struct MultiQuery {
   int nq;
@@ -1106,7 +1105,13 @@ __global__ __launch_bounds__(64 * TPI) __attribute__((amdgpu_waves_per_eu(4, 8))
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
   const int lane = lane_id();
   const int tiles = (count + FIX_TILE - 1) / FIX_TILE;
-  for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+  // Work units in query-major order: the workgroups sweep the range for one
+  // query before the next, so at any time they read one or two operand tables
+  // (97 KB each at 4096-bit moduli) instead of all of them. Each unit converts
+  // its tile's positions again (a small share of the unit's multiply-adds).
+  const int units = tiles * mq.nq;
+  for (int u = blockIdx.x; u < units; u += gridDim.x) {
+    const int q = u / tiles, tile = u - q * tiles;
     const int base = tile * FIX_TILE;
     // A: wave w converts positions [w K, w K + K) of all 64 entries (one
     // entry per lane) straight from the index limbs into xs
@@ -1119,7 +1124,7 @@ __global__ __launch_bounds__(64 * TPI) __attribute__((amdgpu_waves_per_eu(4, 8))
       fixed_positions<RB, LP, TPI, L32>(wave, xb, so, xs + lane);
     }
     __syncthreads();
-    for (int q = 0; q < mq.nq; ++q) {
+    {
       // B: positions [wave K, wave K + K) of V for entry `lane`
       // the table does not change during the launch: read it through the
       // constant address space, so the wave-uniform loads are scalar

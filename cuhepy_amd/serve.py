@@ -10,8 +10,9 @@ many queries at once:
 
 * cyclic scan: the index is cut into ``nchunks`` ranges; every scheduler
   iteration launches ONE kernel that multiplies the next range by the
-  operands of all active queries (``Modulus.mulmod_multi_dev``), so the index
-  range is read once for all of them;
+  operands of all active queries (``Modulus.mulmod_multi_dev``; inside the
+  launch the work runs query by query over the range, so the workgroups share
+  one or two operand tables at a time);
 * continuous batching: a query that arrives while others are on the GPU joins
   at the next iteration, at whatever range the scan is on, and is complete
   after ``nchunks`` iterations (it has then seen every range once); finished
