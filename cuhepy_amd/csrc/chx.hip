@@ -301,12 +301,11 @@ dim3 grid_for(size_t count, int tpi, int rows = 1) {
     else fail(CHX_EUNSUPPORTED, "no kernel built for %d limbs with %d lanes per bignum", l__, t__); \
   } while (0)
 
-// CRT halves: H in {32, 64}
+// CRT halves: H in {32, 64}; lane groups of at least 4 (a p- and a q-half)
 #define CHX_DISPATCH_H(H_, TPI_, CALL)                                                         \
   do {                                                                                         \
     const int h__ = (H_), t__ = (TPI_);                                                        \
-    if (h__ == 32 && t__ == 1) { constexpr int H = 32, TPI = 1; CALL; }                        \
-    else if (h__ == 32 && t__ == 4) { constexpr int H = 32, TPI = 4; CALL; }                   \
+    if (h__ == 32 && t__ == 4) { constexpr int H = 32, TPI = 4; CALL; }                        \
     else if (h__ == 64 && t__ == 4) { constexpr int H = 64, TPI = 4; CALL; }                   \
     else if (h__ == 64 && t__ == 16) { constexpr int H = 64, TPI = 16; CALL; }                 \
     else fail(CHX_EUNSUPPORTED, "no CRT kernel built for %d-limb halves with %d lanes", h__, t__); \
@@ -664,7 +663,8 @@ void launch_crt(chx_pkey* k, const uint32_t* c, uint32_t* m, size_t count, size_
   const size_t half_words = (size_t)H * stride;
   ctx->slot[S_MID1].ensure(2 * half_words * sizeof(uint32_t));
   uint32_t* u = ctx->slot[S_MID1].as<uint32_t>();
-  const int tpi = pick_tpi(ctx, CHX_OP_MULMOD, H, count);
+  // the finish splits each lane group into a p- and a q-half: at least 2 lanes
+  const int tpi = std::max(pick_tpi(ctx, CHX_OP_MULMOD, H, count), 4);
   // both halves read the whole ciphertext (rows [0, H) and [H, 2H)) and reduce
   // it modulo p^2 and q^2 on entry
   const LazySet sets[2] = {k->lz_p2, k->lz_q2};
@@ -1428,22 +1428,36 @@ int chx_paillier_key_create(chx_ctx* ctx, const uint32_t* n, int n_limbs, const 
       if (half_limbs > H) fail(CHX_EINVAL, "factors wider than half the width");
       if (H != 32 && H != 64) fail(CHX_EUNSUPPORTED, "CRT needs 32- or 64-limb halves (width %d)", W);
       BN pb = host::bn_from(p, half_limbs, H), qb = host::bn_from(q, half_limbs, H);
+      // p^2 and q^2 must fit in H limbs, so p and q in H/2 (the CRT finish
+      // works modulo p and q at that width)
+      if (host::bn_bits(pb) > 16 * H || host::bn_bits(qb) > 16 * H)
+        fail(CHX_EINVAL, "p and q must be below 2^%d for %d-limb CRT halves", 16 * H, H);
       // p^2, q^2 at H limbs
       auto sq = [](const BN& x) { return host::bn_sqr_lo(x); };
       host::MontConsts cp2 = host::mont_consts(sq(pb)), cq2 = host::mont_consts(sq(qb));
-      host::MontConsts cp = host::mont_consts(pb), cq = host::mont_consts(qb);
       BN nH = host::bn_from(n, n_limbs, H);
       host::MontConsts cn = host::mont_consts(nH);
       fill_moddev(pd->p2, cp2);
       fill_moddev(pd->q2, cq2);
       k->lz_p2 = make_lazy_set(cp2, ctx->stream);
       k->lz_q2 = make_lazy_set(cq2, ctx->stream);
-      fill_moddev(pd->p, cp);
-      fill_moddev(pd->q, cq);
       fill_moddev(pd->nh, cn);
-      fill_val(pd->hpR, host::to_mont_host(host::bn_from(hp, half_limbs, H), cp));
-      fill_val(pd->hqR, host::to_mont_host(host::bn_from(hq, half_limbs, H), cq));
-      fill_val(pd->qinvR, host::to_mont_host(host::bn_from(qinv, half_limbs, H), cp));
+      const int Hh = H / 2;
+      BN ph = host::bn_from(pb.data(), Hh, Hh), qh = host::bn_from(qb.data(), Hh, Hh);
+      host::MontConsts cph = host::mont_consts(ph), cqh = host::mont_consts(qh);
+      fill_moddev(pd->ph, cph);
+      fill_moddev(pd->qh, cqh);
+      auto below = [&](const uint32_t* v, const BN& m, const char* what) {
+        BN x = host::bn_from(v, half_limbs, H);
+        for (int i = Hh; i < H; ++i)
+          if (x[i]) fail(CHX_EINVAL, "%s must be below its modulus", what);
+        x.resize(Hh);
+        if (host::bn_cmp(x, m) >= 0) fail(CHX_EINVAL, "%s must be below its modulus", what);
+        return x;
+      };
+      fill_val(pd->hpR, host::to_mont_host(below(hp, ph, "h_p"), cph));
+      fill_val(pd->hqR, host::to_mont_host(below(hq, qh, "h_q"), cqh));
+      fill_val(pd->qinvR, host::to_mont_host(below(qinv, ph, "q^-1 mod p"), cph));
       fill_val(pd->qRn, host::to_mont_host(qb, cn));
       BN one(H, 0u);
       one[0] = 1;

@@ -39,11 +39,12 @@ struct PaillierDev {
   ModDev n;       // n, padded to W limbs (exact division by n, arithmetic mod n)
   ValDev nR;      // n * R mod n^2     (m*n = MontMul(m, nR))
   ValDev muR;     // inv * R mod n     (decrypt: m = L(u) * inv mod n)
-  // CRT part (half width H = W/2): p^2, q^2, p, q and n at H limbs
-  ModDev p2, q2, p, q, nh;
-  ValDev hpR;     // h_p * R_H mod p
-  ValDev hqR;     // h_q * R_H mod q
-  ValDev qinvR;   // q^-1 * R_H mod p
+  // CRT part (half width H = W/2): p^2, q^2 and n at H limbs; p and q at
+  // H/2 limbs (p^2 and q^2 fit in H limbs), R_h = 2^(16 H)
+  ModDev p2, q2, nh, ph, qh;
+  ValDev hpR;     // h_p * R_h mod p
+  ValDev hqR;     // h_q * R_h mod q
+  ValDev qinvR;   // q^-1 * R_h mod p
   ValDev qRn;     // q * R_H mod n     (q*h = MontMul_n(h, qRn))
 };
 
@@ -310,39 +311,76 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_encrypt_finish(const Paillier
   if (item < count) store_dist<L, TPI>(out, a, item, stride);
 }
 
-// CRT finish: m_p = L_p(u_p) h_p mod p, m_q likewise, then Garner
-// m = m_q + q * ((m_p - m_q) * q^-1 mod p). U holds u_p, u_q ([2][H][stride]).
+// CRT finish. Each group of TPI lanes handles one item; its two halves of
+// TPI/2 lanes work on the p- and the q-residue at once, at half width (H/2
+// limbs, K limbs per lane as at full width, so the low half's lanes hold
+// exactly the low H/2 limbs of the full-width distribution).
+// u_p = c^(p-1) mod p^2 (U: [2][H][stride], u_p then u_q) is 0 when p | c,
+// else 1 mod p (Fermat), so L_p(u_p) = (u_p - 1) / p is an exact quotient
+// below p, the Hensel division of the low H/2 limbs of u_p - 1; u_p = 0
+// gives floor(-1 / p) = -1, i.e. p - 1 (the floor semantics of scheme.py:86-104).
+// m_p = L_p(u_p) h_p mod p, m_q likewise; then Garner
+// m = m_q + q ((m_p - m_q) q^-1 mod p), q h < n exact at H limbs.
 template <int H, int TPI>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_crt_finish(const PaillierDev* __restrict__ key,
                                                               const u32* __restrict__ U, u32* __restrict__ out,
                                                               int count, int stride) {
   extern __shared__ uint4 smem[];
-  constexpr int K = H / TPI;
+  constexpr int HH = H / 2, TH = TPI / 2, K = H / TPI;
+  static_assert(TPI >= 2 && HH / TH == K, "two half-width halves per group");
   const int item = group_item<TPI>();
   const int it = item < count ? item : count - 1;
+  const int half = Grp<TPI>::rank() / TH;  // 0: p, 1: q
+  Mont<HH, TH> M;
+  mont_init<HH, TH>(M, half == 0 ? &key->ph : &key->qh, smem);
+  const u32* Uh = U + (size_t)half * H * stride;
+  u32 u[K], hi[K];
+  load_dist<HH, TH>(u, Uh, it, stride);
+  load_dist<HH, TH>(hi, Uh + (size_t)HH * stride, it, stride);
+  u32 nz = 0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) nz |= u[k] | hi[k];
+  const bool is_zero = Grp<TH>::group_bits(__ballot(nz != 0)) == 0ull;
+  (void)sub_small<K, TH>(u, 1u);
+  u32 lq[K];
+  M.exact_div(lq, u);
+  u32 nm1[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) nm1[k] = M.N[k];
+  (void)sub_small<K, TH>(nm1, 1u);
+#pragma unroll
+  for (int k = 0; k < K; ++k) lq[k] = is_zero ? nm1[k] : lq[k];
+  u32 c[K], m[K];
+  load_const<HH, TH>(c, half == 0 ? key->hpR.v : key->hqR.v);
+  M.mul(m, lq, c);  // m_p (low half) or m_q (high half), canonical
+  // Garner, computed by both halves modulo p
   u32 mp[K], mq[K];
-  load_dist<H, TPI>(mp, U, it, stride);
-  load_dist<H, TPI>(mq, U + (size_t)H * stride, it, stride);
-  Mont<H, TPI> Mp;
-  mont_init<H, TPI>(Mp, &key->p, smem);
-  paillier_finish<H, TPI>(Mp, &key->p, mp, key->hpR.v);
-  {
-    Mont<H, TPI> Mq;
-    mont_init<H, TPI>(Mq, &key->q, smem);
-    paillier_finish<H, TPI>(Mq, &key->q, mq, key->hqR.v);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const u32 o = (u32)__shfl_xor((int)m[k], TH);
+    mp[k] = half == 0 ? m[k] : o;
+    mq[k] = half == 0 ? o : m[k];
   }
+  Mont<HH, TH> P;
+  mont_init<HH, TH>(P, &key->ph, smem);
   u32 qi[K], h1[K], h2[K];
-  load_const<H, TPI>(qi, key->qinvR.v);
-  Mp.mul(h1, mp, qi);  // m_p * q^-1 mod p
-  Mp.mul(h2, mq, qi);  // m_q * q^-1 mod p   (m_q < R, so the product is reduced)
-  sub_mod<K, TPI>(h1, h2, Mp.N);
+  load_const<HH, TH>(qi, key->qinvR.v);
+  P.mul(h1, mp, qi);  // m_p * q^-1 mod p
+  P.mul(h2, mq, qi);  // m_q * q^-1 mod p   (m_q < R_h, so the product is reduced)
+  sub_mod<K, TH>(h1, h2, P.N);
+  // to the full-width distribution: the high half's lanes hold limbs >= H/2, zero
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    h1[k] = half == 0 ? h1[k] : 0u;
+    mq[k] = half == 0 ? mq[k] : 0u;
+  }
   Mont<H, TPI> Mn;
   mont_init<H, TPI>(Mn, &key->nh, smem);
-  u32 qr[K], m[K];
+  u32 qr[K], res[K];
   load_const<H, TPI>(qr, key->qRn.v);
-  Mn.mul(m, h1, qr);  // q*h  (< n, exact)
-  (void)add_dist<K, TPI>(m, mq);
-  if (item < count) store_dist<H, TPI>(out, m, item, stride);
+  Mn.mul(res, h1, qr);  // q*h  (< n, exact)
+  (void)add_dist<K, TPI>(res, mq);
+  if (item < count) store_dist<H, TPI>(out, res, item, stride);
 }
 
 // ---------------------------------------------------------------------------
