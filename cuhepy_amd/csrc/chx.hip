@@ -175,7 +175,9 @@ struct chx_lkey {
   uint32_t* d_gar = nullptr;   // g_a_inv * R mod n
   uint32_t* d_gtab = nullptr;  // Montgomery-form g table, AoS
   uint32_t* d_ntab = nullptr;  // Montgomery-form noise table, AoS
-  uint32_t* d_ptab = nullptr;  // row-pair table (k_lazy_pair_table), AoS; null if chunks < 2
+  uint32_t* d_ptab = nullptr;  // row-pair table (k_lazy_group_table<2>), AoS; null if chunks < 2
+  uint32_t* d_ttab = nullptr;  // row-triple table (k_lazy_group_table<3>), built on first use
+  bool ttab_failed = false;    // no memory for it: row pairs instead
   LazySet lz_n2;
   BN e_a;
   std::map<int, ExpSched> scheds;
@@ -694,20 +696,35 @@ const ExpSched& get_sched_l(chx_lkey* k, int window) {
   return k->scheds.emplace(window, std::move(s)).first->second;
 }
 
+uint32_t* lazy_group_table(chx_ctx* ctx, const LazySet& ls, const uint32_t* gtab, size_t groups, int R);
+
 void launch_lookup_encrypt(chx_lkey* k, const uint8_t* digits, const uint8_t* nidx, int nmult, uint32_t* c,
                            size_t count, size_t stride) {
   chx_ctx* ctx = k->ctx;
   if (!k->d_gtab || !k->d_ntab) fail(CHX_EINVAL, "lookup key was created without tables (decrypt-only)");
   const int L = k->W;
   const int ci = pick_lazy(ctx, L, count, CHX_OP_LOOKUP);
-  // rows per table product: the context's lookup window (0 = pairs when built)
-  const int rows = ctx->tune_window[CHX_OP_LOOKUP][lidx(L)];
-  const uint32_t* ptab = rows == 1 ? nullptr : k->d_ptab;
+  // rows per table product: the context's lookup window (0 = triples)
+  int rows = ctx->tune_window[CHX_OP_LOOKUP][lidx(L)];
+  if (rows == 0) rows = 3;
+  if (rows == 3 && !k->d_ttab && !k->ttab_failed && k->chunks >= 3) {
+    // 16^3 entries per three rows (212 MB for 256 rows at a 4096-bit n^2);
+    // without the memory for it the pairs serve
+    try {
+      k->d_ttab = lazy_group_table(ctx, k->lz_n2, k->d_gtab, k->chunks / 3, 3);
+    } catch (const ChxError& e) {
+      if (e.code != CHX_ENOMEM) throw;
+      (void)hipGetLastError();  // the failed allocation is not this launch's error
+      k->ttab_failed = true;
+    }
+  }
+  if (rows == 3 && !k->d_ttab) rows = 2;
+  const uint32_t* ptab = rows == 2 ? k->d_ptab : rows == 3 ? k->d_ttab : nullptr;
   record_start(ctx, "lookup_encrypt");
   CHX_DISPATCH_LAZY(L, ci, {
     hipLaunchKernelGGL((k_lazy_lookup_encrypt<RB, LP, TPI, L32>), grid_for(count, TPI), dim3(BLOCK_THREADS),
-                       (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, k->lz_n2.d[ci], k->d_gtab, ptab, k->d_ntab,
-                       digits, k->chunks, nidx, nmult, k->noise, c, (int)count, (int)stride);
+                       (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, k->lz_n2.d[ci], k->d_gtab, ptab, rows,
+                       k->d_ntab, digits, k->chunks, nidx, nmult, k->noise, c, (int)count, (int)stride);
   });
   record_stop(ctx);
   check_launch("k_lazy_lookup_encrypt");
@@ -738,25 +755,31 @@ uint32_t* lazy_table_upload(chx_ctx* ctx, const LazySet& ls, const uint32_t* hos
   return dst;
 }
 
-// row-pair table of `pairs` row pairs from the device g table (both in the
-// positions of configuration 0); returns the buffer (pairs x 256 entries)
-uint32_t* lazy_pair_table(chx_ctx* ctx, const LazySet& ls, const uint32_t* gtab, size_t pairs) {
+// row-group table of `groups` groups of R = 2 or 3 rows from the device g
+// table (both in the positions of configuration 0); returns the buffer
+// (groups x 16^R entries)
+uint32_t* lazy_group_table(chx_ctx* ctx, const LazySet& ls, const uint32_t* gtab, size_t groups, int R) {
   const int L = ls.L32;
   const LazyCfg cfg = lazy_cfgs(L)[0];
   const int ci = 0;
-  const size_t entries = pairs * 256;
+  if (R != 2 && R != 3) fail(CHX_EINVAL, "row groups of %d rows are not built", R);
+  const size_t entries = groups * ((size_t)1 << (4 * R));
   if (entries > (size_t)INT32_MAX / 2) fail(CHX_EINVAL, "too many table rows");
   uint32_t* dst = nullptr;
   hipError_t e = hipMalloc(&dst, entries * cfg.lp * sizeof(uint32_t));
   if (e != hipSuccess) fail(CHX_ENOMEM, "hipMalloc: %s", hipGetErrorString(e));
   CHX_DISPATCH_LAZY(L, ci, {
-    hipLaunchKernelGGL((k_lazy_pair_table<RB, LP, TPI, L32>), grid_for(entries, TPI), dim3(BLOCK_THREADS),
-                       (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, ls.d[ci], gtab, dst, (int)entries);
+    if (R == 2)
+      hipLaunchKernelGGL((k_lazy_group_table<RB, LP, TPI, L32, 2>), grid_for(entries, TPI), dim3(BLOCK_THREADS),
+                         (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, ls.d[ci], gtab, dst, (int)entries);
+    else
+      hipLaunchKernelGGL((k_lazy_group_table<RB, LP, TPI, L32, 3>), grid_for(entries, TPI), dim3(BLOCK_THREADS),
+                         (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, ls.d[ci], gtab, dst, (int)entries);
   });
   hipError_t le = hipGetLastError();
   if (le != hipSuccess) {
     (void)hipFree(dst);
-    fail(CHX_EDEVICE, "k_lazy_pair_table: %s", hipGetErrorString(le));
+    fail(CHX_EDEVICE, "k_lazy_group_table: %s", hipGetErrorString(le));
   }
   hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
   return dst;
@@ -851,7 +874,7 @@ int chx_ctx_set_tuning(chx_ctx* ctx, int op, int limbs, int tpi, int window) {
       if (!ok) fail(CHX_EUNSUPPORTED, "tpi=%d is not built for op %d at %d limbs", tpi, op, limbs);
     }
     if (window < 0 || window > 7) fail(CHX_EINVAL, "window %d out of range [1, 7]", window);
-    if (op == CHX_OP_LOOKUP && window > 2) fail(CHX_EINVAL, "lookup rows per product %d out of range [1, 2]", window);
+    if (op == CHX_OP_LOOKUP && window > 3) fail(CHX_EINVAL, "lookup rows per product %d out of range [1, 3]", window);
     ctx->tune_tpi[op][li] = tpi;
     ctx->tune_window[op][li] = window;
   });
@@ -1637,7 +1660,7 @@ int chx_lookup_key_create(chx_ctx* ctx, const uint32_t* n, int n_limbs, const ui
       // tables to the lazy Montgomery domain on the device (once per key)
       k->d_gtab = lazy_table_upload(ctx, k->lz_n2, g_table, (size_t)chunks * 256);
       k->d_ntab = lazy_table_upload(ctx, k->lz_n2, noise_table, (size_t)noise_entries);
-      if (chunks >= 2) k->d_ptab = lazy_pair_table(ctx, k->lz_n2, k->d_gtab, chunks / 2);
+      if (chunks >= 2) k->d_ptab = lazy_group_table(ctx, k->lz_n2, k->d_gtab, chunks / 2, 2);
     }
     *out = k.release();
   });
@@ -1648,7 +1671,7 @@ int chx_lookup_key_destroy(chx_lkey* k) {
   int rc = guarded([&] {
     hip_check(hipSetDevice(k->ctx->device), "hipSetDevice");
     for (void* p : {(void*)k->d_n2, (void*)k->d_n, (void*)k->d_gar, (void*)k->d_gtab, (void*)k->d_ntab,
-                    (void*)k->d_ptab})
+                    (void*)k->d_ptab, (void*)k->d_ttab})
       if (p) (void)hipFree(p);
     free_lazy_set(k->lz_n2);
     for (auto& s : k->scheds)

@@ -534,33 +534,43 @@ __device__ __forceinline__ int spread4(int v) { return (v & 1) | ((v & 2) << 1) 
 __device__ __forceinline__ int squeeze4(int d) { return (d & 1) | ((d >> 1) & 2) | ((d >> 2) & 4) | ((d >> 3) & 8); }
 constexpr int PACKED_ZERO_BITS = 0xAA;
 
-// Row-pair table for lookup encryption: entry e of pair j is
-//   g_table[2j][spread4(e & 15)] * g_table[2j+1][spread4(e >> 4)] mod n^2
-//   = g^((spread4(e & 15) + 256 spread4(e >> 4)) 2^(16 j)),
-// so one product covers the two 8-bit digits of a packed message (a 16-bit
-// digit, HAC 14.109 with a wider window). In and out: AoS positions in lazy
-// Montgomery form (the product of two such values is the form of the
-// product). One group per entry; entries = pairs * 256.
-template <int RB, int LP, int TPI, int L32>
-__global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_pair_table(const LazyDev* __restrict__ md,
-                                                                   const u32* __restrict__ gtab,
-                                                                   u32* __restrict__ out, int entries) {
+// Row-group table for lookup encryption, R = 2 or 3 rows per entry: entry e
+// of group j is
+//   prod_{t < R} g_table[R j + t][spread4((e >> 4t) & 15)] mod n^2
+//   = g^(sum_t spread4((e >> 4t) & 15) 256^t 2^(8 R j)),
+// so one product covers R 8-bit digits of a packed message (a wider digit,
+// HAC 14.109 with a wider window). In and out: AoS positions in lazy
+// Montgomery form (the product of such values is the form of the product).
+// One group per entry; entries = groups * 16^R.
+template <int RB, int LP, int TPI, int L32, int R>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_group_table(const LazyDev* __restrict__ md,
+                                                                    const u32* __restrict__ gtab,
+                                                                    u32* __restrict__ out, int entries) {
+  static_assert(R == 2 || R == 3, "2 or 3 rows per entry");
   extern __shared__ uint4 smem4[];
   typedef Lazy<RB, LP, TPI, L32> E;
-  constexpr int K = E::K;
+  constexpr int K = E::K, SH = 4 * R;
   u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * E::AREA;
   E M;
   M.init(area, md->N, md->n0);
   const int item = group_item<TPI>();
   const int it = item < entries ? item : entries - 1;
   const int r = Grp<TPI>::rank();
-  const int j = it >> 8, e = it & 255;
-  const u32* lo = gtab + ((size_t)(2 * j) * 256 + spread4(e & 15)) * LP + r * K;
-  const u32* hi = gtab + ((size_t)(2 * j + 1) * 256 + spread4(e >> 4)) * LP + r * K;
-  u32 x[K], y[K];
+  const int j = it >> SH, e = it & ((1 << SH) - 1);
+  u32 x[K];
+  {
+    const u32* src = gtab + ((size_t)(R * j) * 256 + spread4(e & 15)) * LP + r * K;
 #pragma unroll
-  for (int p = 0; p < K; ++p) x[p] = lo[p], y[p] = hi[p];
-  M.mul_into(x, y);
+    for (int p = 0; p < K; ++p) x[p] = src[p];
+  }
+#pragma unroll
+  for (int t = 1; t < R; ++t) {
+    const u32* src = gtab + ((size_t)(R * j + t) * 256 + spread4((e >> (4 * t)) & 15)) * LP + r * K;
+    u32 y[K];
+#pragma unroll
+    for (int p = 0; p < K; ++p) y[p] = src[p];
+    M.mul_into(x, y);
+  }
   if (item < entries) {
 #pragma unroll
     for (int p = 0; p < K; ++p) out[(size_t)item * LP + r * K + p] = x[p];
@@ -572,15 +582,16 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_pair_table(const LazyDev
 // Tables are AoS positions in lazy Montgomery form. Rows where every item of
 // the wave has digit 0 are skipped (g_table[i][0] = 1); other items with
 // digit 0 multiply by their row's j = 0 entry (the Montgomery one), so the
-// wave stays convergent and the product is unchanged. With a row-pair table
-// (ptab, k_lazy_pair_table), rows 2j and 2j+1 take one product from it when
-// every item of the wave has both digits in the packing alphabet; otherwise
-// the wave multiplies the two rows one at a time. Either way the result is
-// the same residue.
+// wave stays convergent and the product is unchanged. With a row-group table
+// of R = prow rows per entry (ptab, k_lazy_group_table), rows R j .. R j + R - 1
+// take one product from it when every item of the wave has all R digits in
+// the packing alphabet; otherwise the wave multiplies the rows one at a time
+// (so do rows past the last whole group). Either way the result is the same
+// residue.
 template <int RB, int LP, int TPI, int L32>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_lookup_encrypt(const LazyDev* __restrict__ md,
                                                                        const u32* __restrict__ gtab,
-                                                                       const u32* __restrict__ ptab,
+                                                                       const u32* __restrict__ ptab, int prow,
                                                                        const u32* __restrict__ ntab,
                                                                        const unsigned char* __restrict__ digits,
                                                                        int chunks,
@@ -605,17 +616,22 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_lookup_encrypt(const Laz
   for (int p = 0; p < K; ++p) acc[p] = md->R1[r * K + p];
   const unsigned char* dg = digits + (size_t)it * chunks;
   for (int i = 0; i < chunks;) {
-    // one step: a row pair (even i, wave-uniform test) or a single row
+    // one step: a row group (i a multiple of prow, wave-uniform test) or a single row
     const int d0 = dg[i];
     const u32* e = gtab + ((size_t)i * 256 + d0) * LP;
     bool act;
     int step = 1;
-    if (ptab != nullptr && (i & 1) == 0 && i + 1 < chunks) {
-      const int d1 = dg[i + 1];
-      if (__ballot(((d0 | d1) & PACKED_ZERO_BITS) != 0) == 0ull) {
-        step = 2;
-        e = ptab + ((size_t)(i >> 1) * 256 + (squeeze4(d0) | (squeeze4(d1) << 4))) * LP;
-        act = __ballot((d0 | d1) != 0) != 0ull;
+    if (ptab != nullptr && i % prow == 0 && i + prow <= chunks) {
+      int dor = d0, idx = squeeze4(d0);
+      for (int t = 1; t < prow; ++t) {
+        const int dt = dg[i + t];
+        dor |= dt;
+        idx |= squeeze4(dt) << (4 * t);
+      }
+      if (__ballot((dor & PACKED_ZERO_BITS) != 0) == 0ull) {
+        step = prow;
+        e = ptab + ((size_t)(i / prow) * ((size_t)1 << (4 * prow)) + idx) * LP;
+        act = __ballot(dor != 0) != 0ull;
       } else {
         act = __ballot(d0 != 0) != 0ull;
       }
