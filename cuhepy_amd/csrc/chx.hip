@@ -178,6 +178,7 @@ struct chx_lkey {
   uint32_t* d_ptab = nullptr;  // row-pair table (k_lazy_group_table<2>), AoS; null if chunks < 2
   uint32_t* d_ttab = nullptr;  // row-triple table (k_lazy_group_table<3>), built on first use
   bool ttab_failed = false;    // no memory for it: row pairs instead
+  uint32_t* d_nptab = nullptr;  // noise-pair table (k_lazy_noise_pairs), noise^2 entries
   LazySet lz_n2;
   BN e_a;
   std::map<int, ExpSched> scheds;
@@ -724,7 +725,7 @@ void launch_lookup_encrypt(chx_lkey* k, const uint8_t* digits, const uint8_t* ni
   CHX_DISPATCH_LAZY(L, ci, {
     hipLaunchKernelGGL((k_lazy_lookup_encrypt<RB, LP, TPI, L32>), grid_for(count, TPI), dim3(BLOCK_THREADS),
                        (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, k->lz_n2.d[ci], k->d_gtab, ptab, rows,
-                       k->d_ntab, digits, k->chunks, nidx, nmult, k->noise, c, (int)count, (int)stride);
+                       k->d_ntab, k->d_nptab, digits, k->chunks, nidx, nmult, k->noise, c, (int)count, (int)stride);
   });
   record_stop(ctx);
   check_launch("k_lazy_lookup_encrypt");
@@ -752,6 +753,29 @@ uint32_t* lazy_table_upload(chx_ctx* ctx, const LazySet& ls, const uint32_t* hos
   check_launch("k_lazy_table");
   hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
   (void)hipFree(src);
+  return dst;
+}
+
+// noise-pair table from the device noise table (nn entries, positions of
+// configuration 0); returns the buffer (nn^2 entries)
+uint32_t* lazy_noise_pairs(chx_ctx* ctx, const LazySet& ls, const uint32_t* ntab, int nn) {
+  const int L = ls.L32;
+  const LazyCfg cfg = lazy_cfgs(L)[0];
+  const int ci = 0;
+  const size_t entries = (size_t)nn * nn;
+  uint32_t* dst = nullptr;
+  hipError_t e = hipMalloc(&dst, entries * cfg.lp * sizeof(uint32_t));
+  if (e != hipSuccess) fail(CHX_ENOMEM, "hipMalloc: %s", hipGetErrorString(e));
+  CHX_DISPATCH_LAZY(L, ci, {
+    hipLaunchKernelGGL((k_lazy_noise_pairs<RB, LP, TPI, L32>), grid_for(entries, TPI), dim3(BLOCK_THREADS),
+                       (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, ls.d[ci], ntab, nn, dst);
+  });
+  hipError_t le = hipGetLastError();
+  if (le != hipSuccess) {
+    (void)hipFree(dst);
+    fail(CHX_EDEVICE, "k_lazy_noise_pairs: %s", hipGetErrorString(le));
+  }
+  hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
   return dst;
 }
 
@@ -1661,6 +1685,7 @@ int chx_lookup_key_create(chx_ctx* ctx, const uint32_t* n, int n_limbs, const ui
       k->d_gtab = lazy_table_upload(ctx, k->lz_n2, g_table, (size_t)chunks * 256);
       k->d_ntab = lazy_table_upload(ctx, k->lz_n2, noise_table, (size_t)noise_entries);
       if (chunks >= 2) k->d_ptab = lazy_group_table(ctx, k->lz_n2, k->d_gtab, chunks / 2, 2);
+      k->d_nptab = lazy_noise_pairs(ctx, k->lz_n2, k->d_ntab, noise_entries);
     }
     *out = k.release();
   });
@@ -1671,7 +1696,7 @@ int chx_lookup_key_destroy(chx_lkey* k) {
   int rc = guarded([&] {
     hip_check(hipSetDevice(k->ctx->device), "hipSetDevice");
     for (void* p : {(void*)k->d_n2, (void*)k->d_n, (void*)k->d_gar, (void*)k->d_gtab, (void*)k->d_ntab,
-                    (void*)k->d_ptab, (void*)k->d_ttab})
+                    (void*)k->d_ptab, (void*)k->d_ttab, (void*)k->d_nptab})
       if (p) (void)hipFree(p);
     free_lazy_set(k->lz_n2);
     for (auto& s : k->scheds)

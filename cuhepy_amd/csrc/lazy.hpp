@@ -577,6 +577,35 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_group_table(const LazyDe
   }
 }
 
+// Noise-pair table: entry a * nn + b = noise[a] * noise[b] mod n^2 (nn
+// entries in the noise table), so two noise factors of an encryption take one
+// product. AoS positions in lazy Montgomery form, one group per entry.
+template <int RB, int LP, int TPI, int L32>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_noise_pairs(const LazyDev* __restrict__ md,
+                                                                    const u32* __restrict__ ntab, int nn,
+                                                                    u32* __restrict__ out) {
+  extern __shared__ uint4 smem4[];
+  typedef Lazy<RB, LP, TPI, L32> E;
+  constexpr int K = E::K;
+  u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * E::AREA;
+  E M;
+  M.init(area, md->N, md->n0);
+  const int entries = nn * nn;
+  const int item = group_item<TPI>();
+  const int it = item < entries ? item : entries - 1;
+  const int r = Grp<TPI>::rank();
+  const u32* a = ntab + (size_t)(it / nn) * LP + r * K;
+  const u32* b = ntab + (size_t)(it % nn) * LP + r * K;
+  u32 x[K], y[K];
+#pragma unroll
+  for (int p = 0; p < K; ++p) x[p] = a[p], y[p] = b[p];
+  M.mul_into(x, y);
+  if (item < entries) {
+#pragma unroll
+    for (int p = 0; p < K; ++p) out[(size_t)item * LP + r * K + p] = x[p];
+  }
+}
+
 // Lookup-table encryption (lookup.py:213-252, lcu:839-948) in the lazy domain:
 //   c = prod_i g_table[i][d_i] * prod_k noise[idx_k] mod n^2
 // Tables are AoS positions in lazy Montgomery form. Rows where every item of
@@ -587,12 +616,14 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_group_table(const LazyDe
 // take one product from it when every item of the wave has all R digits in
 // the packing alphabet; otherwise the wave multiplies the rows one at a time
 // (so do rows past the last whole group). Either way the result is the same
-// residue.
+// residue. With a noise-pair table (nptab, k_lazy_noise_pairs) the noise
+// factors go two per product.
 template <int RB, int LP, int TPI, int L32>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_lookup_encrypt(const LazyDev* __restrict__ md,
                                                                        const u32* __restrict__ gtab,
                                                                        const u32* __restrict__ ptab, int prow,
                                                                        const u32* __restrict__ ntab,
+                                                                       const u32* __restrict__ nptab,
                                                                        const unsigned char* __restrict__ digits,
                                                                        int chunks,
                                                                        const unsigned char* __restrict__ nidx,
@@ -647,9 +678,18 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_lookup_encrypt(const Laz
     M.template mul_into<U>(acc, y);
   }
   const unsigned char* ni = nidx + (size_t)it * nmult;
-  for (int j = 0; j < nmult; ++j) {
+  for (int j = 0; j < nmult;) {
+    // index mod table size, as lcu:933-934
+    const int a = ni[j] % nnoise;
+    const u32* e;
+    if (nptab != nullptr && j + 1 < nmult) {
+      e = nptab + ((size_t)a * nnoise + ni[j + 1] % nnoise) * LP + r * K;
+      j += 2;
+    } else {
+      e = ntab + (size_t)a * LP + r * K;
+      j += 1;
+    }
     u32 y[K];
-    const u32* e = ntab + (size_t)(ni[j] % nnoise) * LP + r * K;  // index mod table size, as lcu:933-934
 #pragma unroll
     for (int p = 0; p < K; ++p) y[p] = e[p];
     M.template mul_into<U>(acc, y);

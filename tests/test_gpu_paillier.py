@@ -219,6 +219,23 @@ def test_lookup_row_groups_packed_and_mixed(ctx):
             got = limbs_to_ints(key.encrypt_digits(digits, idx))
             assert got == want, f"rows per product {rows}"
         assert limbs_to_ints(key.decrypt(got)) == ms
+    # noise factors go two per product: an odd count leaves one single. The
+    # oracle takes the reference's 14 factors; dividing its result by the
+    # noise entries left out gives the expected value for fewer.
+    ctx.set_tuning(OP_LOOKUP, key.width, 0, 0)
+    n2, nt = keys["pk"]["n_squared"], keys["noise_table"]
+    idx14 = np.array([[rng.randrange(256) for _ in range(14)] for _ in packed], dtype=np.uint8)
+    full = [O.lookup_encrypt(m, keys["pk"], table, nt, keys["message_chunks"], ix.tolist())
+            for m, ix in zip(packed, idx14)]
+    digits = plaintexts_to_digits(ints_to_limbs(packed, key.width), key.chunks)
+    for nm in (13, 1):
+        want = []
+        for c, ix in zip(full, idx14):
+            for j in ix[nm:]:
+                c = c * pow(nt[int(j) % len(nt)], -1, n2) % n2
+            want.append(c)
+        got = limbs_to_ints(key.encrypt_digits(digits, np.ascontiguousarray(idx14[:, :nm])))
+        assert got == want, f"{nm} noise factors"
 
 
 def test_plain_encrypt_decrypt_1024_messages():
