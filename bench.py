@@ -285,8 +285,8 @@ def lookup_leg(ctx, args, peak: float) -> dict:
 
     def timed(rows: int) -> tuple[float, float]:
         """(kernel seconds per launch, wall seconds per batch) with `rows`
-        8-bit table rows per product (1 = single rows, 2 = pairs, 0 = the
-        default, row triples)."""
+        8-bit table rows per product (1 = single rows, 2 = pairs, 3 = triples,
+        0 = the default, row quads)."""
         ctx.set_tuning(OP_LOOKUP, W, 0, rows)
         key.encrypt_digits_dev(d_dig, d_idx, 14, out, count, stride)  # warm
         torch.cuda.synchronize()
@@ -303,13 +303,14 @@ def lookup_leg(ctx, args, peak: float) -> dict:
 
     single_s, _ = timed(1)
     pairs_s, _ = timed(2)
-    launch_s, wall = timed(0)  # row triples (the default); its output is checked below
+    triples_s, _ = timed(3)
+    launch_s, wall = timed(0)  # row quads (the default); its output is checked below
     ctx.set_tuning(OP_LOOKUP, W, 0, 0)
     nnz = int((digits != 0).sum())
     mulmods = nnz + 13 * count
     macs = mulmods * 2 * W * W
     achieved = macs / launch_s / 1e12
-    executed = _lookup_products(digits, 64 // Context.supported_tpi(W, op=OP_LOOKUP)[0], 14, 3)
+    executed = _lookup_products(digits, 64 // Context.supported_tpi(W, op=OP_LOOKUP)[0], 14, 4)
     executed_tmac = executed * 2 * W * W / launch_s / 1e12
     # round trip: every ciphertext decrypts to its plaintext (GPU decrypt)
     aos = torch.empty((count, W), dtype=torch.int32, device=dev)
@@ -330,11 +331,11 @@ def lookup_leg(ctx, args, peak: float) -> dict:
     cpu_s = time.perf_counter() - t
     got = limbs_to_ints(c_aos[sample])
     f_bad = sum(int(a != b) for a, b in zip(got, want))
-    log(f"lookup: {count} vectors in {launch_s * 1e3:.2f} ms ({count / launch_s:,.0f} vectors/s); row triples "
+    log(f"lookup: {count} vectors in {launch_s * 1e3:.2f} ms ({count / launch_s:,.0f} vectors/s); row quads "
         f"execute {executed / count:.1f} products per vector: {executed_tmac:.2f} TMAC/s = "
         f"{executed_tmac / peak:.1%} of the issue rate (the reference's one product per nonzero digit, "
-        f"{mulmods / count:.1f} per vector, would need {achieved:.2f} TMAC/s for this rate); pairs "
-        f"{pairs_s * 1e3:.2f} ms, single rows {single_s * 1e3:.2f} ms; round-trip "
+        f"{mulmods / count:.1f} per vector, would need {achieved:.2f} TMAC/s for this rate); triples "
+        f"{triples_s * 1e3:.2f} ms, pairs {pairs_s * 1e3:.2f} ms, single rows {single_s * 1e3:.2f} ms; round-trip "
         f"mismatches {rt_bad}; oracle mismatches {f_bad}/{len(sample)}; CPU {len(sample) / cpu_s:.1f} vectors/s")
     if rt_bad or f_bad:
         raise RuntimeError(f"lookup encryption wrong: round trip {rt_bad}, oracle {f_bad}")
@@ -353,9 +354,11 @@ def lookup_leg(ctx, args, peak: float) -> dict:
         "frac_of_peak": round(executed_tmac / peak, 4),
         "reference_algorithm": {"mulmods": mulmods, "macs": macs, "mulmod_per_s": round(mulmods / launch_s, 1),
                                 "equivalent_tmac_s": round(achieved, 4)},
-        "row_groups": {"rows_per_product": 3,
+        "row_groups": {"rows_per_product": 4,
+                       "triples_kernel_ms": round(triples_s * 1e3, 3),
                        "pairs_kernel_ms": round(pairs_s * 1e3, 3),
                        "single_rows_kernel_ms": round(single_s * 1e3, 3),
+                       "speedup_over_triples": round(triples_s / launch_s, 3),
                        "speedup_over_pairs": round(pairs_s / launch_s, 3),
                        "speedup_over_single_rows": round(single_s / launch_s, 3)},
         "wall_ms_per_batch": round(wall * 1e3, 3),

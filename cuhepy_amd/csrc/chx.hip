@@ -178,6 +178,8 @@ struct chx_lkey {
   uint32_t* d_ptab = nullptr;  // row-pair table (k_lazy_group_table<2>), AoS; null if chunks < 2
   uint32_t* d_ttab = nullptr;  // row-triple table (k_lazy_group_table<3>), built on first use
   bool ttab_failed = false;    // no memory for it: row pairs instead
+  uint32_t* d_qtab = nullptr;  // row-quad table (k_lazy_group_table<4>), built on first use
+  bool qtab_failed = false;    // no memory for it: triples instead
   uint32_t* d_nptab = nullptr;  // noise-pair table (k_lazy_noise_pairs), noise^2 entries
   LazySet lz_n2;
   BN e_a;
@@ -705,22 +707,31 @@ void launch_lookup_encrypt(chx_lkey* k, const uint8_t* digits, const uint8_t* ni
   if (!k->d_gtab || !k->d_ntab) fail(CHX_EINVAL, "lookup key was created without tables (decrypt-only)");
   const int L = k->W;
   const int ci = pick_lazy(ctx, L, count, CHX_OP_LOOKUP);
-  // rows per table product: the context's lookup window (0 = triples)
+  // rows per table product: the context's lookup window (0 = quads). The
+  // triple and quad tables (16^R entries per R rows: 212 MB and 2.55 GB for
+  // 256 rows at a 4096-bit n^2) are built on first use; without the memory
+  // for one, the next smaller group serves.
   int rows = ctx->tune_window[CHX_OP_LOOKUP][lidx(L)];
-  if (rows == 0) rows = 3;
-  if (rows == 3 && !k->d_ttab && !k->ttab_failed && k->chunks >= 3) {
-    // 16^3 entries per three rows (212 MB for 256 rows at a 4096-bit n^2);
-    // without the memory for it the pairs serve
+  if (rows == 0) rows = 4;
+  auto ensure = [&](uint32_t*& tab, bool& failed, int R) {
+    if (tab || failed || k->chunks < R) return;
     try {
-      k->d_ttab = lazy_group_table(ctx, k->lz_n2, k->d_gtab, k->chunks / 3, 3);
+      tab = lazy_group_table(ctx, k->lz_n2, k->d_gtab, k->chunks / R, R);
     } catch (const ChxError& e) {
       if (e.code != CHX_ENOMEM) throw;
       (void)hipGetLastError();  // the failed allocation is not this launch's error
-      k->ttab_failed = true;
+      failed = true;
     }
+  };
+  if (rows == 4) {
+    ensure(k->d_qtab, k->qtab_failed, 4);
+    if (!k->d_qtab) rows = 3;
   }
-  if (rows == 3 && !k->d_ttab) rows = 2;
-  const uint32_t* ptab = rows == 2 ? k->d_ptab : rows == 3 ? k->d_ttab : nullptr;
+  if (rows == 3) {
+    ensure(k->d_ttab, k->ttab_failed, 3);
+    if (!k->d_ttab) rows = 2;
+  }
+  const uint32_t* ptab = rows == 2 ? k->d_ptab : rows == 3 ? k->d_ttab : rows == 4 ? k->d_qtab : nullptr;
   record_start(ctx, "lookup_encrypt");
   CHX_DISPATCH_LAZY(L, ci, {
     hipLaunchKernelGGL((k_lazy_lookup_encrypt<RB, LP, TPI, L32>), grid_for(count, TPI), dim3(BLOCK_THREADS),
@@ -786,7 +797,7 @@ uint32_t* lazy_group_table(chx_ctx* ctx, const LazySet& ls, const uint32_t* gtab
   const int L = ls.L32;
   const LazyCfg cfg = lazy_cfgs(L)[0];
   const int ci = 0;
-  if (R != 2 && R != 3) fail(CHX_EINVAL, "row groups of %d rows are not built", R);
+  if (R < 2 || R > 4) fail(CHX_EINVAL, "row groups of %d rows are not built", R);
   const size_t entries = groups * ((size_t)1 << (4 * R));
   if (entries > (size_t)INT32_MAX / 2) fail(CHX_EINVAL, "too many table rows");
   uint32_t* dst = nullptr;
@@ -795,6 +806,9 @@ uint32_t* lazy_group_table(chx_ctx* ctx, const LazySet& ls, const uint32_t* gtab
   CHX_DISPATCH_LAZY(L, ci, {
     if (R == 2)
       hipLaunchKernelGGL((k_lazy_group_table<RB, LP, TPI, L32, 2>), grid_for(entries, TPI), dim3(BLOCK_THREADS),
+                         (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, ls.d[ci], gtab, dst, (int)entries);
+    else if (R == 4)
+      hipLaunchKernelGGL((k_lazy_group_table<RB, LP, TPI, L32, 4>), grid_for(entries, TPI), dim3(BLOCK_THREADS),
                          (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, ls.d[ci], gtab, dst, (int)entries);
     else
       hipLaunchKernelGGL((k_lazy_group_table<RB, LP, TPI, L32, 3>), grid_for(entries, TPI), dim3(BLOCK_THREADS),
@@ -898,7 +912,7 @@ int chx_ctx_set_tuning(chx_ctx* ctx, int op, int limbs, int tpi, int window) {
       if (!ok) fail(CHX_EUNSUPPORTED, "tpi=%d is not built for op %d at %d limbs", tpi, op, limbs);
     }
     if (window < 0 || window > 7) fail(CHX_EINVAL, "window %d out of range [1, 7]", window);
-    if (op == CHX_OP_LOOKUP && window > 3) fail(CHX_EINVAL, "lookup rows per product %d out of range [1, 3]", window);
+    if (op == CHX_OP_LOOKUP && window > 4) fail(CHX_EINVAL, "lookup rows per product %d out of range [1, 4]", window);
     ctx->tune_tpi[op][li] = tpi;
     ctx->tune_window[op][li] = window;
   });
@@ -1696,7 +1710,7 @@ int chx_lookup_key_destroy(chx_lkey* k) {
   int rc = guarded([&] {
     hip_check(hipSetDevice(k->ctx->device), "hipSetDevice");
     for (void* p : {(void*)k->d_n2, (void*)k->d_n, (void*)k->d_gar, (void*)k->d_gtab, (void*)k->d_ntab,
-                    (void*)k->d_ptab, (void*)k->d_ttab, (void*)k->d_nptab})
+                    (void*)k->d_ptab, (void*)k->d_ttab, (void*)k->d_qtab, (void*)k->d_nptab})
       if (p) (void)hipFree(p);
     free_lazy_set(k->lz_n2);
     for (auto& s : k->scheds)

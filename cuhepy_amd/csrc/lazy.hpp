@@ -546,7 +546,7 @@ template <int RB, int LP, int TPI, int L32, int R>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_group_table(const LazyDev* __restrict__ md,
                                                                     const u32* __restrict__ gtab,
                                                                     u32* __restrict__ out, int entries) {
-  static_assert(R == 2 || R == 3, "2 or 3 rows per entry");
+  static_assert(R >= 2 && R <= 4, "2 to 4 rows per entry");
   extern __shared__ uint4 smem4[];
   typedef Lazy<RB, LP, TPI, L32> E;
   constexpr int K = E::K, SH = 4 * R;

@@ -70,9 +70,9 @@ int chx_ctx_sync(chx_ctx* ctx);
 /* lanes per bignum (tpi) and window width for `op` at `limbs`; 0 = automatic.
  * For CHX_OP_LOOKUP the window is the number of 8-bit table rows one product
  * covers: 1 = single rows, 2 = row pairs (the key's pair table), 3 = row
- * triples (a 16^3-entry table per three rows, built on first use; pairs when
- * its memory cannot be allocated); groups are used for waves whose digits are
- * all in the bit-packing alphabet; 0 = 3. */
+ * triples, 4 = row quads (16^R-entry tables per R rows, built on first use;
+ * the next smaller group when the memory for one cannot be allocated); groups
+ * are used for waves whose digits are all in the bit-packing alphabet; 0 = 4. */
 int chx_ctx_set_tuning(chx_ctx* ctx, int op, int limbs, int tpi, int window);
 /* lanes-per-bignum values built for `limbs` (returns count, fills up to cap);
  * op CHX_OP_MODEXP lists the exponentiation engine's values, other ops the

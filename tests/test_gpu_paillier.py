@@ -187,11 +187,11 @@ def test_lookup_seeded_every_variant(ctx):
 
 
 def test_lookup_row_groups_packed_and_mixed(ctx):
-    """Packed embeddings take the row-group tables (one product per two or
-    three 8-bit digits; 128 rows leave two single rows after 42 triples);
+    """Packed embeddings take the row-group tables (one product per two, three
+    or four 8-bit digits; 128 rows leave two single rows after 42 triples);
     waves with a message outside the packing alphabet fall back to single
-    rows. All must equal the oracle: triples (window 0/3), pairs (2), single
-    rows (1)."""
+    rows. All must equal the oracle: quads (window 0/4), triples (3), pairs
+    (2), single rows (1)."""
     from cuhepy_amd.engine import LookupKey
     from cuhepy_amd.limbs import ints_to_limbs, limbs_to_ints
     from cuhepy_amd.paillier.packing import plaintexts_to_digits
@@ -214,7 +214,7 @@ def test_lookup_row_groups_packed_and_mixed(ctx):
         digits = plaintexts_to_digits(ints_to_limbs(ms, key.width), key.chunks)
         if ms is packed:
             assert not (digits & 0xAA).any()  # the packing alphabet: the pair path applies
-        for rows in (0, 3, 2, 1):
+        for rows in (0, 4, 3, 2, 1):
             ctx.set_tuning(OP_LOOKUP, key.width, 0, rows)
             got = limbs_to_ints(key.encrypt_digits(digits, idx))
             assert got == want, f"rows per product {rows}"
