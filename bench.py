@@ -222,9 +222,10 @@ def _lookup_products(digits, items_per_wave: int, nmult: int, rows: int) -> int:
     `rows` table rows per product: per wave, one per row group with a nonzero
     digit when every digit of the group is in the packing alphabet, else one
     per row with a nonzero digit (rows past the last whole group go singly);
-    plus the noise factors, two per product (noise-pair table), and the final
-    conversion of every item (all items of a wave run every product the wave
-    runs)."""
+    the first of them is a copy (the accumulator starts at the Montgomery
+    one); plus the noise factors, two per product (noise-pair table), the last
+    of which also leaves the Montgomery form (no conversion product unless
+    there is no noise). All items of a wave run every product the wave runs."""
     import numpy as np
     count, chunks = digits.shape
     waves = -(-count // items_per_wave)
@@ -241,10 +242,11 @@ def _lookup_products(digits, items_per_wave: int, nmult: int, rows: int) -> int:
         per_wave += np.where(packed, nz_grp.astype(np.int64), nz_rows)
     for j in range(whole, chunks):
         per_wave += (d[:, :, j] != 0).any(axis=1)
+    per_wave -= per_wave > 0  # the first active product is a copy
     # every item of a wave takes part in every product the wave runs
     real = np.full(waves, items_per_wave, dtype=np.int64)
     real[-1] = count - (waves - 1) * items_per_wave
-    return int((per_wave * real).sum()) + (nmult // 2 + nmult % 2 + 1) * count
+    return int((per_wave * real).sum()) + (nmult // 2 + nmult % 2 + (nmult == 0)) * count
 
 
 def lookup_leg(ctx, args, peak: float) -> dict:

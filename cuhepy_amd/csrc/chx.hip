@@ -181,6 +181,8 @@ struct chx_lkey {
   uint32_t* d_qtab = nullptr;  // row-quad table (k_lazy_group_table<4>), built on first use
   bool qtab_failed = false;    // no memory for it: triples instead
   uint32_t* d_nptab = nullptr;  // noise-pair table (k_lazy_noise_pairs), noise^2 entries
+  uint32_t* d_ntab_plain = nullptr;   // the noise table as plain residues (k_lazy_plain)
+  uint32_t* d_nptab_plain = nullptr;  // the noise-pair table as plain residues
   LazySet lz_n2;
   BN e_a;
   std::map<int, ExpSched> scheds;
@@ -736,7 +738,8 @@ void launch_lookup_encrypt(chx_lkey* k, const uint8_t* digits, const uint8_t* ni
   CHX_DISPATCH_LAZY(L, ci, {
     hipLaunchKernelGGL((k_lazy_lookup_encrypt<RB, LP, TPI, L32>), grid_for(count, TPI), dim3(BLOCK_THREADS),
                        (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, k->lz_n2.d[ci], k->d_gtab, ptab, rows,
-                       k->d_ntab, k->d_nptab, digits, k->chunks, nidx, nmult, k->noise, c, (int)count, (int)stride);
+                       k->d_ntab, k->d_nptab, k->d_ntab_plain, k->d_nptab_plain, digits, k->chunks, nidx, nmult,
+                       k->noise, c, (int)count, (int)stride);
   });
   record_stop(ctx);
   check_launch("k_lazy_lookup_encrypt");
@@ -790,7 +793,29 @@ uint32_t* lazy_noise_pairs(chx_ctx* ctx, const LazySet& ls, const uint32_t* ntab
   return dst;
 }
 
-// row-group table of `groups` groups of R = 2 or 3 rows from the device g
+// plain residues (MontMul(e, 1)) of a device table in lazy Montgomery form
+// (positions of configuration 0); returns the buffer (entries x LP)
+uint32_t* lazy_plain_copy(chx_ctx* ctx, const LazySet& ls, const uint32_t* in, size_t entries) {
+  const int L = ls.L32;
+  const LazyCfg cfg = lazy_cfgs(L)[0];
+  const int ci = 0;
+  uint32_t* dst = nullptr;
+  hipError_t e = hipMalloc(&dst, entries * cfg.lp * sizeof(uint32_t));
+  if (e != hipSuccess) fail(CHX_ENOMEM, "hipMalloc: %s", hipGetErrorString(e));
+  CHX_DISPATCH_LAZY(L, ci, {
+    hipLaunchKernelGGL((k_lazy_plain<RB, LP, TPI, L32>), grid_for(entries, TPI), dim3(BLOCK_THREADS),
+                       (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, ls.d[ci], in, dst, (int)entries);
+  });
+  hipError_t le = hipGetLastError();
+  if (le != hipSuccess) {
+    (void)hipFree(dst);
+    fail(CHX_EDEVICE, "k_lazy_plain: %s", hipGetErrorString(le));
+  }
+  hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  return dst;
+}
+
+// row-group table of `groups` groups of R = 2 to 4 rows from the device g
 // table (both in the positions of configuration 0); returns the buffer
 // (groups x 16^R entries)
 uint32_t* lazy_group_table(chx_ctx* ctx, const LazySet& ls, const uint32_t* gtab, size_t groups, int R) {
@@ -1700,6 +1725,8 @@ int chx_lookup_key_create(chx_ctx* ctx, const uint32_t* n, int n_limbs, const ui
       k->d_ntab = lazy_table_upload(ctx, k->lz_n2, noise_table, (size_t)noise_entries);
       if (chunks >= 2) k->d_ptab = lazy_group_table(ctx, k->lz_n2, k->d_gtab, chunks / 2, 2);
       k->d_nptab = lazy_noise_pairs(ctx, k->lz_n2, k->d_ntab, noise_entries);
+      k->d_ntab_plain = lazy_plain_copy(ctx, k->lz_n2, k->d_ntab, (size_t)noise_entries);
+      k->d_nptab_plain = lazy_plain_copy(ctx, k->lz_n2, k->d_nptab, (size_t)noise_entries * noise_entries);
     }
     *out = k.release();
   });
@@ -1710,7 +1737,8 @@ int chx_lookup_key_destroy(chx_lkey* k) {
   int rc = guarded([&] {
     hip_check(hipSetDevice(k->ctx->device), "hipSetDevice");
     for (void* p : {(void*)k->d_n2, (void*)k->d_n, (void*)k->d_gar, (void*)k->d_gtab, (void*)k->d_ntab,
-                    (void*)k->d_ptab, (void*)k->d_ttab, (void*)k->d_qtab, (void*)k->d_nptab})
+                    (void*)k->d_ptab, (void*)k->d_ttab, (void*)k->d_qtab, (void*)k->d_nptab,
+                    (void*)k->d_ntab_plain, (void*)k->d_nptab_plain})
       if (p) (void)hipFree(p);
     free_lazy_set(k->lz_n2);
     for (auto& s : k->scheds)

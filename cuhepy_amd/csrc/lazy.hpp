@@ -606,6 +606,33 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_noise_pairs(const LazyDe
   }
 }
 
+// Plain residues of a table in lazy Montgomery form: out[e] = MontMul(in[e],
+// 1) = the entry's value v (at most N) instead of v R'. AoS positions, one
+// group per entry.
+template <int RB, int LP, int TPI, int L32>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_plain(const LazyDev* __restrict__ md,
+                                                              const u32* __restrict__ in, u32* __restrict__ out,
+                                                              int entries) {
+  extern __shared__ uint4 smem4[];
+  typedef Lazy<RB, LP, TPI, L32> E;
+  constexpr int K = E::K;
+  u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * E::AREA;
+  E M;
+  M.init(area, md->N, md->n0);
+  const int item = group_item<TPI>();
+  const int it = item < entries ? item : entries - 1;
+  const int r = Grp<TPI>::rank();
+  u32 x[K], one[K];
+#pragma unroll
+  for (int p = 0; p < K; ++p) x[p] = in[(size_t)it * LP + r * K + p], one[p] = 0;
+  one[0] = r == 0 ? 1u : 0u;
+  M.mul_into(x, one);
+  if (item < entries) {
+#pragma unroll
+    for (int p = 0; p < K; ++p) out[(size_t)item * LP + r * K + p] = x[p];
+  }
+}
+
 // Lookup-table encryption (lookup.py:213-252, lcu:839-948) in the lazy domain:
 //   c = prod_i g_table[i][d_i] * prod_k noise[idx_k] mod n^2
 // Tables are AoS positions in lazy Montgomery form. Rows where every item of
@@ -617,13 +644,19 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_noise_pairs(const LazyDe
 // the packing alphabet; otherwise the wave multiplies the rows one at a time
 // (so do rows past the last whole group). Either way the result is the same
 // residue. With a noise-pair table (nptab, k_lazy_noise_pairs) the noise
-// factors go two per product.
+// factors go two per product. The first product of a wave with an active row
+// is a copy (the accumulator starts at the Montgomery one). With plain-residue
+// copies of the noise tables (ntab_plain, nptab_plain: v instead of v R'),
+// the last noise product also leaves the Montgomery form, so no separate
+// conversion product is needed.
 template <int RB, int LP, int TPI, int L32>
-__global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_lookup_encrypt(const LazyDev* __restrict__ md,
+__global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_lazy_lookup_encrypt(const LazyDev* __restrict__ md,
                                                                        const u32* __restrict__ gtab,
                                                                        const u32* __restrict__ ptab, int prow,
                                                                        const u32* __restrict__ ntab,
                                                                        const u32* __restrict__ nptab,
+                                                                       const u32* __restrict__ ntab_plain,
+                                                                       const u32* __restrict__ nptab_plain,
                                                                        const unsigned char* __restrict__ digits,
                                                                        int chunks,
                                                                        const unsigned char* __restrict__ nidx,
@@ -645,6 +678,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_lookup_encrypt(const Laz
   u32 acc[K];
 #pragma unroll
   for (int p = 0; p < K; ++p) acc[p] = md->R1[r * K + p];
+  bool first = true;  // wave-uniform: acc still holds the Montgomery one
   const unsigned char* dg = digits + (size_t)it * chunks;
   for (int i = 0; i < chunks;) {
     // one step: a row group (i a multiple of prow, wave-uniform test) or a single row
@@ -675,27 +709,46 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_lookup_encrypt(const Laz
     e += r * K;
 #pragma unroll
     for (int p = 0; p < K; ++p) y[p] = e[p];
-    M.template mul_into<U>(acc, y);
+    if (first) {
+#pragma unroll
+      for (int p = 0; p < K; ++p) acc[p] = y[p];
+      first = false;
+    } else {
+      M.template mul_into<U>(acc, y);
+    }
   }
   const unsigned char* ni = nidx + (size_t)it * nmult;
+  const bool plain = ntab_plain != nullptr && (nptab == nullptr || nptab_plain != nullptr);
+  bool out_of_form = false;
   for (int j = 0; j < nmult;) {
     // index mod table size, as lcu:933-934
     const int a = ni[j] % nnoise;
-    const u32* e;
-    if (nptab != nullptr && j + 1 < nmult) {
-      e = nptab + ((size_t)a * nnoise + ni[j + 1] % nnoise) * LP + r * K;
-      j += 2;
-    } else {
-      e = ntab + (size_t)a * LP + r * K;
-      j += 1;
-    }
+    const bool pair = nptab != nullptr && j + 1 < nmult;
+    const int step = pair ? 2 : 1;
+    const bool last = plain && j + step >= nmult;
+    const u32* e = pair ? (last ? nptab_plain : nptab) + ((size_t)a * nnoise + ni[j + 1] % nnoise) * LP
+                        : (last ? ntab_plain : ntab) + (size_t)a * LP;
+    j += step;
     u32 y[K];
+    e += r * K;
 #pragma unroll
     for (int p = 0; p < K; ++p) y[p] = e[p];
-    M.template mul_into<U>(acc, y);
+    if (last) {
+      // acc (below 2N', or 2N) times a plain residue v (at most N), modulo N
+      // itself: the result acc v R'^-1 = the plain product, below 2N
+      if constexpr (U) M.init(area, md->N, md->n0);
+      M.mul_into(acc, y);
+      out_of_form = true;
+    } else if (first) {
+#pragma unroll
+      for (int p = 0; p < K; ++p) acc[p] = y[p];
+      first = false;
+    } else {
+      M.template mul_into<U>(acc, y);
+    }
   }
-  if constexpr (U) M.init(area, md->N, md->n0);  // acc < 2N' < R': MontMul_N(acc, 1) <= N
-  {
+  if (!out_of_form) {
+    if constexpr (U) M.init(area, md->N, md->n0);  // acc < 2N' < R': MontMul_N(acc, 1) <= N
     u32 one[K];
 #pragma unroll
     for (int p = 0; p < K; ++p) one[p] = 0;
