@@ -101,6 +101,28 @@ def test_crt_full_width_ciphertext_reduction(ctx, key_len):
     ctx.set_tuning(OP_MODEXP, key.half, 0, 0)
 
 
+def test_crt_rejects_factors_too_wide_for_the_halves(ctx):
+    """p^2 and q^2 must fit in the CRT half width (H limbs, so p, q below
+    2^(16 H)): a 1100-bit p with a 948-bit q (n of 2048 bits, H = 64) is
+    refused instead of being truncated; the same n without factors still
+    decrypts through the full-width path."""
+    from cuhepy_amd.engine import PaillierKey
+    from cuhepy_amd.limbs import limbs_to_ints
+    rng = random.Random(77)
+    p, q = O.get_prime(1100, rng), O.get_prime(948, rng)
+    n = p * q
+    phi = (p - 1) * (q - 1)
+    inv = pow(phi, -1, n)
+    assert 4090 < (n * n).bit_length() <= 4096
+    with pytest.raises(ValueError):
+        PaillierKey(ctx, n, phi, inv, p, q)
+    key = PaillierKey(ctx, n, phi, inv)
+    pk = {"n": n, "n_squared": n * n, "g": n + 1}
+    m, r = rng.randrange(n), O.random_r(n, rng)
+    c = O.encrypt(m, pk, r)
+    assert limbs_to_ints(key.decrypt([c])) == [m]
+
+
 def test_decode_hamming_fused(ctx, golden):
     """decrypt + odd-bit popcount on the GPU = plaintext Hamming distance."""
     from cuhepy_amd.engine import Modulus, PaillierKey
