@@ -1,0 +1,63 @@
+"""CPU model of the CRT finish (k_crt_finish in cuhepy_amd/csrc/kernels.hpp)
+against the oracle's CRT decryption (oracle/paillier_oracle.py decrypt_crt).
+
+The kernel relies on u_p = c^(p-1) mod p^2 being either 0 (p | c) or 1 mod p
+(Fermat), so that L_p(u_p) = floor((u_p - 1) / p) is the exact quotient of
+u_p - 1 by p, below p: the Hensel division of the low H/2 limbs of u_p - 1
+(multiplication by p^-1 mod 2^(16 H)) gives it, and u_p = 0 gives -1, which
+is p - 1 mod p. Then m_p = L_p h_p mod p, m_q likewise, and Garner
+m = m_q + q ((m_p - m_q) q^-1 mod p). Checked on units and non-units of both
+factors at 1024- and 2048-bit n; a planted mistake (the zero case taken as a
+quotient of 0) must fail on a non-unit.
+"""
+
+import random
+
+import pytest
+
+from cuhepy_amd.limbs import width_for_bits
+from oracle import paillier_oracle as O
+
+
+def finish_model(c, keys, half_bits, zero_as_minus_one=True):
+    p, q = keys["p"], keys["q"]
+    R = 1 << half_bits
+    assert p < R and q < R  # the factors fit the half width (the host checks it)
+    cp = O.crt_params(p, q)
+
+    def half(f, h):
+        u = pow(c % (f * f), f - 1, f * f)
+        assert u == 0 or u % f == 1
+        if u == 0:
+            lq = f - 1 if zero_as_minus_one else 0
+        else:
+            lq = ((u - 1) % R) * pow(f, -1, R) % R  # Hensel quotient of the low half
+            assert lq * f == u - 1
+        return lq * h % f
+
+    mp, mq = half(p, cp["hp"]), half(q, cp["hq"])
+    h = (mp - mq) * cp["q_inv_p"] % p
+    return mq + q * h
+
+
+@pytest.mark.parametrize("key_len", [512, 1024])
+def test_crt_finish_model_matches_oracle(key_len):
+    keys = O.key_gen(key_len, 900 + key_len)
+    n, p, q = keys["pk"]["n"], keys["p"], keys["q"]
+    n2 = n * n
+    half_bits = 8 * width_for_bits(n2.bit_length())  # H/2 limbs of 32 bits, H = W/2
+    assert half_bits == {512: 512, 1024: 1024}[key_len]
+    rng = random.Random(key_len)
+    cs = [0, 1, p, q, n, p * p, q * q, 3 * p, 5 * q, n2 - 1, n2 - p]
+    cs += [rng.randrange(n2) for _ in range(10)]
+    cs += [p * rng.randrange(1, q * n) % n2 for _ in range(3)]  # non-units in p
+    for c in cs:
+        assert finish_model(c, keys, half_bits) == O.decrypt_crt(c, keys), hex(c)[:20]
+
+
+def test_crt_finish_model_zero_case_is_minus_one():
+    keys = O.key_gen(512, 901)
+    p, q = keys["p"], keys["q"]
+    c = p * p  # u_p = 0
+    assert finish_model(c, keys, 512) == O.decrypt_crt(c, keys)
+    assert finish_model(c, keys, 512, zero_as_minus_one=False) != O.decrypt_crt(c, keys)
