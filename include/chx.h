@@ -204,6 +204,10 @@ int chx_paillier_decode_hamming(chx_ctx* ctx, const chx_pkey* k, const uint32_t*
  * entry: g_table[chunks][256] (lookup.py:104-110, lcu:550-588) and
  * noise_table[256] (lookup.py:113-117); they are converted to Montgomery form
  * on the device and kept resident in HBM (ensure_device_tables_, lcu:2625-2652).
+ * The key also builds, on the device, a row-pair table (chunks/2 x 256
+ * entries) and a noise-pair table with plain-residue copies (noise^2 entries;
+ * 40 MB each at a 4096-bit n^2); row-triple and row-quad tables are built by
+ * the first encryption that uses them (see chx_ctx_set_tuning).
  * chunks = 0 (tables NULL) creates a decrypt-only key. */
 int chx_lookup_key_create(chx_ctx* ctx, const uint32_t* n, int n_limbs, const uint32_t* a, int a_limbs,
                           const uint32_t* g_a_inv, const uint32_t* g_table, int chunks,
