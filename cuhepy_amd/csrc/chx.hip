@@ -1509,52 +1509,57 @@ int chx_paillier_key_create(chx_ctx* ctx, const uint32_t* n, int n_limbs, const 
     // n * R mod n^2: n < n^2, shift n by 32W bits mod n^2
     fill_val(pd->nR, host::to_mont_host(nW, k->n2));
     fill_val(pd->muR, host::to_mont_host(invW, k->n));
-    if (p && q && hp && hq && qinv && half_limbs > 0) {
-      const int H = W / 2;
-      if (half_limbs > H) fail(CHX_EINVAL, "factors wider than half the width");
-      if (H != 32 && H != 64) fail(CHX_EUNSUPPORTED, "CRT needs 32- or 64-limb halves (width %d)", W);
-      BN pb = host::bn_from(p, half_limbs, H), qb = host::bn_from(q, half_limbs, H);
-      // p^2 and q^2 must fit in H limbs, so p and q in H/2 (the CRT finish
-      // works modulo p and q at that width)
-      if (host::bn_bits(pb) > 16 * H || host::bn_bits(qb) > 16 * H)
-        fail(CHX_EINVAL, "p and q must be below 2^%d for %d-limb CRT halves", 16 * H, H);
-      // p^2, q^2 at H limbs
-      auto sq = [](const BN& x) { return host::bn_sqr_lo(x); };
-      host::MontConsts cp2 = host::mont_consts(sq(pb)), cq2 = host::mont_consts(sq(qb));
-      BN nH = host::bn_from(n, n_limbs, H);
-      host::MontConsts cn = host::mont_consts(nH);
-      fill_moddev(pd->p2, cp2);
-      fill_moddev(pd->q2, cq2);
-      k->lz_p2 = make_lazy_set(cp2, ctx->stream);
-      k->lz_q2 = make_lazy_set(cq2, ctx->stream);
-      fill_moddev(pd->nh, cn);
-      const int Hh = H / 2;
-      BN ph = host::bn_from(pb.data(), Hh, Hh), qh = host::bn_from(qb.data(), Hh, Hh);
-      host::MontConsts cph = host::mont_consts(ph), cqh = host::mont_consts(qh);
-      fill_moddev(pd->ph, cph);
-      fill_moddev(pd->qh, cqh);
-      auto below = [&](const uint32_t* v, const BN& m, const char* what) {
-        BN x = host::bn_from(v, half_limbs, H);
-        for (int i = Hh; i < H; ++i)
-          if (x[i]) fail(CHX_EINVAL, "%s must be below its modulus", what);
-        x.resize(Hh);
-        if (host::bn_cmp(x, m) >= 0) fail(CHX_EINVAL, "%s must be below its modulus", what);
-        return x;
-      };
-      fill_val(pd->hpR, host::to_mont_host(below(hp, ph, "h_p"), cph));
-      fill_val(pd->hqR, host::to_mont_host(below(hq, qh, "h_q"), cqh));
-      fill_val(pd->qinvR, host::to_mont_host(below(qinv, ph, "q^-1 mod p"), cph));
-      fill_val(pd->qRn, host::to_mont_host(qb, cn));
-      BN one(H, 0u);
-      one[0] = 1;
-      k->e_pm1 = pb;
-      host::bn_sub(k->e_pm1, one);
-      k->e_qm1 = qb;
-      host::bn_sub(k->e_qm1, one);
-      k->crt = true;
+    try {  // a failure below frees what was already allocated on the device
+      if (p && q && hp && hq && qinv && half_limbs > 0) {
+        const int H = W / 2;
+        if (half_limbs > H) fail(CHX_EINVAL, "factors wider than half the width");
+        if (H != 32 && H != 64) fail(CHX_EUNSUPPORTED, "CRT needs 32- or 64-limb halves (width %d)", W);
+        BN pb = host::bn_from(p, half_limbs, H), qb = host::bn_from(q, half_limbs, H);
+        // p^2 and q^2 must fit in H limbs, so p and q in H/2 (the CRT finish
+        // works modulo p and q at that width)
+        if (host::bn_bits(pb) > 16 * H || host::bn_bits(qb) > 16 * H)
+          fail(CHX_EINVAL, "p and q must be below 2^%d for %d-limb CRT halves", 16 * H, H);
+        // p^2, q^2 at H limbs
+        auto sq = [](const BN& x) { return host::bn_sqr_lo(x); };
+        host::MontConsts cp2 = host::mont_consts(sq(pb)), cq2 = host::mont_consts(sq(qb));
+        BN nH = host::bn_from(n, n_limbs, H);
+        host::MontConsts cn = host::mont_consts(nH);
+        fill_moddev(pd->p2, cp2);
+        fill_moddev(pd->q2, cq2);
+        k->lz_p2 = make_lazy_set(cp2, ctx->stream);
+        k->lz_q2 = make_lazy_set(cq2, ctx->stream);
+        fill_moddev(pd->nh, cn);
+        const int Hh = H / 2;
+        BN ph = host::bn_from(pb.data(), Hh, Hh), qh = host::bn_from(qb.data(), Hh, Hh);
+        host::MontConsts cph = host::mont_consts(ph), cqh = host::mont_consts(qh);
+        fill_moddev(pd->ph, cph);
+        fill_moddev(pd->qh, cqh);
+        auto below = [&](const uint32_t* v, const BN& m, const char* what) {
+          BN x = host::bn_from(v, half_limbs, H);
+          for (int i = Hh; i < H; ++i)
+            if (x[i]) fail(CHX_EINVAL, "%s must be below its modulus", what);
+          x.resize(Hh);
+          if (host::bn_cmp(x, m) >= 0) fail(CHX_EINVAL, "%s must be below its modulus", what);
+          return x;
+        };
+        fill_val(pd->hpR, host::to_mont_host(below(hp, ph, "h_p"), cph));
+        fill_val(pd->hqR, host::to_mont_host(below(hq, qh, "h_q"), cqh));
+        fill_val(pd->qinvR, host::to_mont_host(below(qinv, ph, "q^-1 mod p"), cph));
+        fill_val(pd->qRn, host::to_mont_host(qb, cn));
+        BN one(H, 0u);
+        one[0] = 1;
+        k->e_pm1 = pb;
+        host::bn_sub(k->e_pm1, one);
+        k->e_qm1 = qb;
+        host::bn_sub(k->e_qm1, one);
+        k->crt = true;
+      }
+      k->lz_n2 = make_lazy_set(k->n2, ctx->stream);
+      k->d = dev_upload(pd.get(), 1, ctx->stream);
+    } catch (...) {
+      (void)chx_paillier_key_destroy(k.release());
+      throw;
     }
-    k->lz_n2 = make_lazy_set(k->n2, ctx->stream);
-    k->d = dev_upload(pd.get(), 1, ctx->stream);
     *out = k.release();
   });
 }
@@ -1711,22 +1716,27 @@ int chx_lookup_key_create(chx_ctx* ctx, const uint32_t* n, int n_limbs, const ui
     k->n2 = host::mont_consts(n2);
     k->n = host::mont_consts(nW);
     k->e_a = host::bn_from(a, a_limbs, std::max(a_limbs, 1));
-    ModDev md;
-    fill_moddev(md, k->n2);
-    k->d_n2 = dev_upload(&md, 1, ctx->stream);
-    fill_moddev(md, k->n);
-    k->d_n = dev_upload(&md, 1, ctx->stream);
-    k->lz_n2 = make_lazy_set(k->n2, ctx->stream);
-    BN gar = host::to_mont_host(host::bn_from(g_a_inv, n_limbs, W), k->n);
-    k->d_gar = dev_upload(gar.data(), W, ctx->stream);
-    if (tables) {
-      // tables to the lazy Montgomery domain on the device (once per key)
-      k->d_gtab = lazy_table_upload(ctx, k->lz_n2, g_table, (size_t)chunks * 256);
-      k->d_ntab = lazy_table_upload(ctx, k->lz_n2, noise_table, (size_t)noise_entries);
-      if (chunks >= 2) k->d_ptab = lazy_group_table(ctx, k->lz_n2, k->d_gtab, chunks / 2, 2);
-      k->d_nptab = lazy_noise_pairs(ctx, k->lz_n2, k->d_ntab, noise_entries);
-      k->d_ntab_plain = lazy_plain_copy(ctx, k->lz_n2, k->d_ntab, (size_t)noise_entries);
-      k->d_nptab_plain = lazy_plain_copy(ctx, k->lz_n2, k->d_nptab, (size_t)noise_entries * noise_entries);
+    try {  // a failure below frees what was already allocated on the device
+      ModDev md;
+      fill_moddev(md, k->n2);
+      k->d_n2 = dev_upload(&md, 1, ctx->stream);
+      fill_moddev(md, k->n);
+      k->d_n = dev_upload(&md, 1, ctx->stream);
+      k->lz_n2 = make_lazy_set(k->n2, ctx->stream);
+      BN gar = host::to_mont_host(host::bn_from(g_a_inv, n_limbs, W), k->n);
+      k->d_gar = dev_upload(gar.data(), W, ctx->stream);
+      if (tables) {
+        // tables to the lazy Montgomery domain on the device (once per key)
+        k->d_gtab = lazy_table_upload(ctx, k->lz_n2, g_table, (size_t)chunks * 256);
+        k->d_ntab = lazy_table_upload(ctx, k->lz_n2, noise_table, (size_t)noise_entries);
+        if (chunks >= 2) k->d_ptab = lazy_group_table(ctx, k->lz_n2, k->d_gtab, chunks / 2, 2);
+        k->d_nptab = lazy_noise_pairs(ctx, k->lz_n2, k->d_ntab, noise_entries);
+        k->d_ntab_plain = lazy_plain_copy(ctx, k->lz_n2, k->d_ntab, (size_t)noise_entries);
+        k->d_nptab_plain = lazy_plain_copy(ctx, k->lz_n2, k->d_nptab, (size_t)noise_entries * noise_entries);
+      }
+    } catch (...) {
+      (void)chx_lookup_key_destroy(k.release());
+      throw;
     }
     *out = k.release();
   });
