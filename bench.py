@@ -549,7 +549,7 @@ def main() -> int:
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; every unit of work is independent and "
-                                   "the multi-GPU split is a host-side scatter (see DESIGN.md)",
+                                   "the multi-GPU split is a host-side scatter of those units",
                           "n_gpus": args.gpus}))
         return 2
 
