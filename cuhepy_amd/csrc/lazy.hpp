@@ -755,13 +755,16 @@ __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(3
     one[0] = r == 0 ? 1u : 0u;
     M.mul_into(acc, one);
   }
+  // canonical: acc is below 2N (below N after MontMul(acc, 1)); 2N can pass
+  // 2^(32 L32) when N is near it, so the bit above the limbs takes part
   u32 w[K32];
   M.renorm(acc);
+  const u32 hi = M.top32(acc);
   M.to32(w, acc);
   u32 n32[K32];
 #pragma unroll
   for (int k = 0; k < K32; ++k) n32[k] = md->N32[r * K32 + k];
-  cond_sub<K32, TPI>(w, 0u, n32);
+  cond_sub<K32, TPI>(w, hi, n32);
   if (item < count) {
 #pragma unroll
     for (int k = 0; k < K32; ++k) out[(size_t)(r * K32 + k) * stride + item] = w[k];

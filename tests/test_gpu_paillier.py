@@ -260,6 +260,41 @@ def test_lookup_row_groups_packed_and_mixed(ctx):
         assert got == want, f"{nm} noise factors"
 
 
+def test_lookup_modulus_at_the_top_of_the_width(ctx):
+    """n = 2^2048 - 159, so n^2 lies just below 2^4096 (128 limbs): the last
+    product of the lookup encryption (a plain noise factor; R' = 2^4104 leaves
+    little room above n^2) lands at or above 2^4096 for about 0.1% of the
+    items, before the final subtraction, whose carry word must then be the
+    bit above the limbs. 10,000 items give ~10 such items. The encryption
+    formula holds for any odd n, any g and any number of 8-bit rows: 8 rows
+    (messages below 2^64, packed and arbitrary) keep the oracle cheap; quads
+    and single rows against it."""
+    from cuhepy_amd.engine import LookupKey
+    from cuhepy_amd.limbs import ints_to_limbs, limbs_to_ints
+    from cuhepy_amd.paillier.packing import plaintexts_to_digits
+    n = (1 << 2048) - 159
+    n2 = n * n
+    assert n2.bit_length() == 4096 and n2 > (1 << 4096) - (1 << 4060)
+    rng = random.Random(4096)
+    g = rng.randrange(2, n2)
+    table = O.precompute_g_table(g, n, 32)  # 8 rows
+    noise = [rng.randrange(1, n2) for _ in range(256)]
+    key = LookupKey(ctx, n, 1, 1, [table[i] for i in range(len(table))], noise)
+    assert key.width == 128 and key.chunks == 8
+    spread = lambda v: sum(((v >> b) & 1) << (2 * b) for b in range(32))  # noqa: E731
+    ms = [spread(rng.getrandbits(32)) for _ in range(4000)]  # packed: every digit in the alphabet
+    ms += [rng.getrandbits(64) for _ in range(3000)] + [0] * 3000
+    idx = np.array([[rng.randrange(256) for _ in range(14)] for _ in ms], dtype=np.uint8)
+    pk = {"n": n, "n_squared": n2}
+    want = [O.lookup_encrypt(m, pk, table, noise, 8, ix.tolist()) for m, ix in zip(ms, idx)]
+    digits = plaintexts_to_digits(ints_to_limbs(ms, key.width), key.chunks)
+    for rows in (0, 1):
+        ctx.set_tuning(OP_LOOKUP, key.width, 0, rows)
+        got = limbs_to_ints(key.encrypt_digits(digits, idx))
+        bad = [i for i, (x, y) in enumerate(zip(got, want)) if x != y]
+        assert not bad, f"rows per product {rows}: {len(bad)} items differ, first {bad[:4]}"
+
+
 def test_plain_encrypt_decrypt_1024_messages():
     """The reference's CPU-runnable case: key_len 1024-bit n (512-bit primes,
     n^2 2048-bit), keypair from seeded primes, 1024 messages m uniform in [0, n)
