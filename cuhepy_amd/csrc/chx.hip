@@ -594,6 +594,30 @@ void launch_mulmod(chx_ctx* ctx, const ModDev* md, int L, const uint32_t* a, con
   check_launch("k_mulmod");
 }
 
+// Pointwise multiply of a modulus handle: the lazy engine for batches of
+// kLazyMulmodMin or more (two lazy products per item), the canonical engine
+// below that or when the context forces its lane count.
+constexpr size_t kLazyMulmodMin = 8192;
+
+void launch_mulmod_mod(chx_ctx* ctx, const chx_mod* m, const uint32_t* a, const uint32_t* b, uint32_t* out,
+                       size_t count, size_t stride) {
+  const int L = m->hc.L;
+  const int li = lidx(L);
+  if (li < 0 || count < kLazyMulmodMin || ctx->tune_tpi[CHX_OP_MULMOD][li] != 0) {
+    launch_mulmod(ctx, m->d, L, a, b, out, count, stride);
+    return;
+  }
+  const int ci = pick_lazy(ctx, L, count);
+  record_start(ctx, "mulmod");
+  CHX_DISPATCH_LAZY(L, ci, {
+    hipLaunchKernelGGL((k_lazy_mulmod<RB, LP, TPI, L32>), grid_for(count, TPI), dim3(BLOCK_THREADS),
+                       (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, m->lz.d[ci], a, b, out, (int)count,
+                       (int)stride);
+  });
+  record_stop(ctx);
+  check_launch("k_lazy_mulmod");
+}
+
 void launch_modexp(chx_ctx* ctx, const chx_mod* m, const uint32_t* base, uint32_t* out, size_t count,
                    size_t stride, const ExpSched& s) {
   const ExpSched* sp = &s;
@@ -1012,7 +1036,7 @@ int chx_mulmod(chx_ctx* ctx, const chx_mod* m, const uint32_t* a, const uint32_t
     uint32_t* da = upload_il(ctx, S_IN0, a, count, L, stride);
     uint32_t* db = upload_il(ctx, S_IN1, b, count, L, stride);
     ctx->slot[S_OUT].ensure(stride * L * 4);
-    launch_mulmod(ctx, m->d, L, da, db, ctx->slot[S_OUT].as<uint32_t>(), count, stride);
+    launch_mulmod_mod(ctx, m, da, db, ctx->slot[S_OUT].as<uint32_t>(), count, stride);
     download_il(ctx, ctx->slot[S_OUT].as<uint32_t>(), out, count, L, stride);
   });
 }
@@ -1023,7 +1047,7 @@ int chx_mulmod_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* a, const uint
     if (!m || !a || !b || !out) fail(CHX_EINVAL, "null argument");
     if (count == 0) return;
     check_stride(count, stride);
-    launch_mulmod(ctx, m->d, m->hc.L, a, b, out, count, stride);
+    launch_mulmod_mod(ctx, m, a, b, out, count, stride);
   });
 }
 

@@ -497,6 +497,55 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_modexp(LazyJob j0, LazyJ
   }
 }
 
+// Pointwise out[i] = a[i] b[i] mod N for any a, b < 2^(32 L32), interleaved
+// [L32][stride] (the Hamming evaluator's ciphertext multiply,
+// encode_hamming_server: cu:672-732, lcu:1505-1561). t = MontMul(a, b) =
+// a b R'^-1 (modulo N' = kN on UNIT shapes) is below a b / R' + N' < R';
+// MontMul(t, R'^2 mod N), modulo N itself, is then below t N / R' + N < 2N,
+// and one conditional subtraction (with the bit above the limbs, as 2N may
+// pass 2^(32 L32)) makes it canonical.
+template <int RB, int LP, int TPI, int L32>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_mulmod(const LazyDev* __restrict__ md,
+                                                               const u32* __restrict__ A, const u32* __restrict__ B,
+                                                               u32* __restrict__ out, int count, int stride) {
+  extern __shared__ uint4 smem4[];
+  typedef Lazy<RB, LP, TPI, L32> E;
+  constexpr int K = E::K, K32 = E::K32;
+  constexpr bool U = E::UNIT;
+  u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * E::AREA;
+  E M;
+  if constexpr (U)
+    M.init(area, md->NK, 1u);
+  else
+    M.init(area, md->N, md->n0);
+  const int item = group_item<TPI>();
+  const int it = item < count ? item : count - 1;
+  const int r = Grp<TPI>::rank();
+  u32 w[K32], x[K], y[K];
+#pragma unroll
+  for (int k = 0; k < K32; ++k) w[k] = A[(size_t)(r * K32 + k) * stride + it];
+  M.from32(x, w);
+#pragma unroll
+  for (int k = 0; k < K32; ++k) w[k] = B[(size_t)(r * K32 + k) * stride + it];
+  M.from32(y, w);
+  M.template mul_into<U>(x, y);
+  if constexpr (U) M.init(area, md->N, md->n0);
+#pragma unroll
+  for (int p = 0; p < K; ++p) y[p] = md->R2[r * K + p];
+  M.mul_into(x, y);
+  M.renorm(x);
+  const u32 hi = M.top32(x);
+  M.to32(w, x);
+  u32 n32[K32];
+#pragma unroll
+  for (int k = 0; k < K32; ++k) n32[k] = md->N32[r * K32 + k];
+  cond_sub<K32, TPI>(w, hi, n32);
+  if (item < count) {
+#pragma unroll
+    for (int k = 0; k < K32; ++k) out[(size_t)(r * K32 + k) * stride + item] = w[k];
+  }
+}
+
 // Lookup tables to the lazy Montgomery domain: out[e] = in[e] * R' mod N
 // (below 2N) as positions; in: AoS 32-bit limbs (L32 per entry), out: AoS
 // positions (LP per entry). One group per entry.

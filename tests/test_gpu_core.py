@@ -58,6 +58,28 @@ def test_mulmod_every_variant(ctx, limbs):
 
 
 @pytest.mark.parametrize("limbs", [32, 64, 128])
+def test_mulmod_lazy_large_batches(ctx, limbs):
+    """Batches of 8,192 or more items go to the lazy-engine multiply (two lazy
+    products, the last modulo N, then a subtraction that takes the bit above
+    the limbs): full-width, smaller and near-2^(32 L) moduli, operands up to
+    2^(32 L) - 1, both lazy configurations of the width."""
+    from cuhepy_amd.engine import Context, Modulus
+    from cuhepy_amd.limbs import ints_to_limbs, limbs_to_ints
+    rng = random.Random(300 + limbs)
+    R = 1 << (32 * limbs)
+    count = 9000
+    for tpi in Context.supported_tpi(limbs, op=OP_MODEXP):
+        ctx.set_tuning(OP_MODEXP, limbs, tpi, 0)
+        for n in (_mod(rng, limbs, True), _mod(rng, limbs, False), R - 2 * rng.randrange(1, 1 << 20) - 1):
+            mod = Modulus(ctx, n, limbs)
+            a = [0, 1, n - 1, R - 1, n] + [rng.randrange(R) for _ in range(count - 5)]
+            b = [n - 1, R - 1, n - 1, R - 1, 2] + [rng.randrange(R) for _ in range(count - 5)]
+            got = limbs_to_ints(mod.mulmod(ints_to_limbs(a, limbs), ints_to_limbs(b, limbs)))
+            bad = [i for i, (g, x, y) in enumerate(zip(got, a, b)) if g != x * y % n]
+            assert not bad, f"tpi={tpi} n bits {n.bit_length()}: first bad index {bad[:5]}"
+
+
+@pytest.mark.parametrize("limbs", [32, 64, 128])
 def test_modexp_every_variant(ctx, limbs):
     from cuhepy_amd.engine import Modulus
     from cuhepy_amd.limbs import ints_to_limbs, limbs_to_ints
