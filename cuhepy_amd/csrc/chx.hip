@@ -1005,7 +1005,7 @@ int chx_mod_create(chx_ctx* ctx, const uint32_t* n, int limbs, chx_mod** out) {
       m->d = dev_upload(&md, 1, ctx->stream);
       m->lz = make_lazy_set(m->hc, ctx->stream);
     } catch (...) {
-      delete m;
+      (void)chx_mod_destroy(m);  // frees what was already allocated on the device
       throw;
     }
     *out = m;
