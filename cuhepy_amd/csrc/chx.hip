@@ -727,8 +727,10 @@ const ExpSched& get_sched_l(chx_lkey* k, int window) {
 
 uint32_t* lazy_group_table(chx_ctx* ctx, const LazySet& ls, const uint32_t* gtab, size_t groups, int R);
 
+// any_packed = false: the caller knows that no item has all its digits in the
+// packing alphabet, so no wave can use a row group and none is built.
 void launch_lookup_encrypt(chx_lkey* k, const uint8_t* digits, const uint8_t* nidx, int nmult, uint32_t* c,
-                           size_t count, size_t stride) {
+                           size_t count, size_t stride, bool any_packed = true) {
   chx_ctx* ctx = k->ctx;
   if (!k->d_gtab || !k->d_ntab) fail(CHX_EINVAL, "lookup key was created without tables (decrypt-only)");
   const int L = k->W;
@@ -739,6 +741,7 @@ void launch_lookup_encrypt(chx_lkey* k, const uint8_t* digits, const uint8_t* ni
   // for one, the next smaller group serves.
   int rows = ctx->tune_window[CHX_OP_LOOKUP][lidx(L)];
   if (rows == 0) rows = 4;
+  if (!any_packed) rows = 1;
   auto ensure = [&](uint32_t*& tab, bool& failed, int R) {
     if (tab || failed || k->chunks < R) return;
     try {
@@ -1796,6 +1799,16 @@ int chx_lookup_encrypt(chx_ctx* ctx, const chx_lkey* kc, const uint8_t* digits, 
     const int L = k->W;
     const size_t stride = batch_stride(count);
     const size_t dbytes = count * (size_t)k->chunks, nbytes = count * (size_t)nmult;
+    // arbitrary plaintexts (e.g. decrypted ones being re-encrypted) have no
+    // item whose digits all lie in the packing alphabet: then the row-group
+    // tables would go unused, and single rows serve without building them
+    bool any_packed = false;
+    for (size_t i = 0; i < count && !any_packed; ++i) {
+      const uint8_t* d = digits + i * (size_t)k->chunks;
+      int j = 0;
+      while (j < k->chunks && (d[j] & PACKED_ZERO_BITS) == 0) ++j;
+      any_packed = j == k->chunks;
+    }
     ctx->slot[S_IN0].ensure(dbytes + 16);
     ctx->slot[S_IN1].ensure(nbytes + 16);
     hip_check(hipMemcpyAsync(ctx->slot[S_IN0].p, digits, dbytes, hipMemcpyHostToDevice, ctx->stream), "H2D");
@@ -1803,7 +1816,7 @@ int chx_lookup_encrypt(chx_ctx* ctx, const chx_lkey* kc, const uint8_t* digits, 
       hip_check(hipMemcpyAsync(ctx->slot[S_IN1].p, noise_idx, nbytes, hipMemcpyHostToDevice, ctx->stream), "H2D");
     ctx->slot[S_OUT].ensure(stride * L * 4);
     launch_lookup_encrypt(k, ctx->slot[S_IN0].as<uint8_t>(), ctx->slot[S_IN1].as<uint8_t>(), nmult,
-                          ctx->slot[S_OUT].as<uint32_t>(), count, stride);
+                          ctx->slot[S_OUT].as<uint32_t>(), count, stride, any_packed);
     download_il(ctx, ctx->slot[S_OUT].as<uint32_t>(), c, count, L, stride);
   });
 }
