@@ -175,7 +175,8 @@ struct chx_lkey {
   uint32_t* d_gar = nullptr;   // g_a_inv * R mod n
   uint32_t* d_gtab = nullptr;  // Montgomery-form g table, AoS
   uint32_t* d_ntab = nullptr;  // Montgomery-form noise table, AoS
-  uint32_t* d_ptab = nullptr;  // row-pair table (k_lazy_group_table<2>), AoS; null if chunks < 2
+  uint32_t* d_ptab = nullptr;  // row-pair table (k_lazy_group_table<2>), built on first use
+  bool ptab_failed = false;    // no memory for it: single rows instead
   uint32_t* d_ttab = nullptr;  // row-triple table (k_lazy_group_table<3>), built on first use
   bool ttab_failed = false;    // no memory for it: row pairs instead
   uint32_t* d_qtab = nullptr;  // row-quad table (k_lazy_group_table<4>), built on first use
@@ -736,7 +737,7 @@ void launch_lookup_encrypt(chx_lkey* k, const uint8_t* digits, const uint8_t* ni
   const int L = k->W;
   const int ci = pick_lazy(ctx, L, count, CHX_OP_LOOKUP);
   // rows per table product: the context's lookup window (0 = quads). The
-  // triple and quad tables (16^R entries per R rows: 212 MB and 2.55 GB for
+  // group tables (16^R entries per R rows: 19.9 MB, 212 MB and 2.55 GB for
   // 256 rows at a 4096-bit n^2) are built on first use; without the memory
   // for one, the next smaller group serves.
   int rows = ctx->tune_window[CHX_OP_LOOKUP][lidx(L)];
@@ -759,6 +760,10 @@ void launch_lookup_encrypt(chx_lkey* k, const uint8_t* digits, const uint8_t* ni
   if (rows == 3) {
     ensure(k->d_ttab, k->ttab_failed, 3);
     if (!k->d_ttab) rows = 2;
+  }
+  if (rows == 2) {
+    ensure(k->d_ptab, k->ptab_failed, 2);
+    if (!k->d_ptab) rows = 1;
   }
   const uint32_t* ptab = rows == 2 ? k->d_ptab : rows == 3 ? k->d_ttab : rows == 4 ? k->d_qtab : nullptr;
   record_start(ctx, "lookup_encrypt");
@@ -1756,7 +1761,6 @@ int chx_lookup_key_create(chx_ctx* ctx, const uint32_t* n, int n_limbs, const ui
         // tables to the lazy Montgomery domain on the device (once per key)
         k->d_gtab = lazy_table_upload(ctx, k->lz_n2, g_table, (size_t)chunks * 256);
         k->d_ntab = lazy_table_upload(ctx, k->lz_n2, noise_table, (size_t)noise_entries);
-        if (chunks >= 2) k->d_ptab = lazy_group_table(ctx, k->lz_n2, k->d_gtab, chunks / 2, 2);
         k->d_nptab = lazy_noise_pairs(ctx, k->lz_n2, k->d_ntab, noise_entries);
         k->d_ntab_plain = lazy_plain_copy(ctx, k->lz_n2, k->d_ntab, (size_t)noise_entries);
         k->d_nptab_plain = lazy_plain_copy(ctx, k->lz_n2, k->d_nptab, (size_t)noise_entries * noise_entries);

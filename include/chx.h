@@ -69,10 +69,10 @@ void* chx_ctx_get_stream(chx_ctx* ctx);
 int chx_ctx_sync(chx_ctx* ctx);
 /* lanes per bignum (tpi) and window width for `op` at `limbs`; 0 = automatic.
  * For CHX_OP_LOOKUP the window is the number of 8-bit table rows one product
- * covers: 1 = single rows, 2 = row pairs (the key's pair table), 3 = row
- * triples, 4 = row quads (16^R-entry tables per R rows, built on first use;
- * the next smaller group when the memory for one cannot be allocated); groups
- * are used for waves whose digits are all in the bit-packing alphabet; 0 = 4. */
+ * covers: 1 = single rows, 2 = row pairs, 3 = row triples, 4 = row quads
+ * (16^R-entry tables per R rows, built on first use; the next smaller group
+ * when the memory for one cannot be allocated); groups are used for waves
+ * whose digits are all in the bit-packing alphabet; 0 = 4. */
 int chx_ctx_set_tuning(chx_ctx* ctx, int op, int limbs, int tpi, int window);
 /* lanes-per-bignum values built for `limbs` (returns count, fills up to cap);
  * op CHX_OP_MODEXP lists the exponentiation engine's values, other ops the
@@ -204,10 +204,10 @@ int chx_paillier_decode_hamming(chx_ctx* ctx, const chx_pkey* k, const uint32_t*
  * entry: g_table[chunks][256] (lookup.py:104-110, lcu:550-588) and
  * noise_table[256] (lookup.py:113-117); they are converted to Montgomery form
  * on the device and kept resident in HBM (ensure_device_tables_, lcu:2625-2652).
- * The key also builds, on the device, a row-pair table (chunks/2 x 256
- * entries) and a noise-pair table with plain-residue copies (noise^2 entries;
- * 40 MB each at a 4096-bit n^2); row-triple and row-quad tables are built by
- * the first encryption that uses them (see chx_ctx_set_tuning).
+ * The key also builds, on the device, a noise-pair table with plain-residue
+ * copies (noise^2 entries; 40 MB each at a 4096-bit n^2); the row-group
+ * tables (pairs, triples, quads) are built by the first encryption that uses
+ * them (see chx_ctx_set_tuning).
  * chunks = 0 (tables NULL) creates a decrypt-only key. */
 int chx_lookup_key_create(chx_ctx* ctx, const uint32_t* n, int n_limbs, const uint32_t* a, int a_limbs,
                           const uint32_t* g_a_inv, const uint32_t* g_table, int chunks,
