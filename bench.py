@@ -346,6 +346,7 @@ def lookup_leg(ctx, args, peak: float) -> dict:
     t = time.perf_counter()
     client.encrypt(bits[:sl].tolist())
     client_s = time.perf_counter() - t
+    reenc = _reencrypt_block(ctx, args, key, out, d_dig, d_idx, m, count, stride, W)
     return {
         "workload": f"{count} binary vectors x 512 bits (Bernoulli 0.5), key_len 1024 (n^2 4096-bit), alpha 280, "
                     "14 noise factors per ciphertext",
@@ -367,7 +368,63 @@ def lookup_leg(ctx, args, peak: float) -> dict:
         "client_path_vectors_per_s": round(sl / client_s, 1),
         "bit_exact": {"round_trip_vectors": count, "oracle_vectors": len(sample), "mismatches": 0},
         "cpu_port_single_thread_vectors_s": round(len(sample) / cpu_s, 2),
+        "reencrypt": reenc,
     }
+
+
+def _reencrypt_block(ctx, args, key, c_il, d_dig, d_idx, m, count: int, stride: int, W: int) -> dict:
+    """Fused re-encryption (decrypt_then_reencrypt, lcu:1053-1135) of the
+    lookup leg's ciphertexts from its key to a second key_len-1024 key: the
+    decryption c^a mod n^2 (a 280-bit), the device digit extraction and the
+    row-quad encryption under the second key, all on the device. Checked over
+    every vector: it equals the second key's encryption of the plaintexts'
+    digits with the same noise indices, and decrypts to the plaintexts."""
+    import numpy as np
+    import torch
+
+    from cuhepy_amd.paillier._lookup_gpu_ext import PaillierGPULookupClient
+
+    dev = torch.device("cuda:0")
+    t = time.perf_counter()
+    kb = PaillierGPULookupClient(embed_len=512, key_len=1024, rng=random.Random(args.seed + 7))._device_key()
+    log(f"reencrypt: second key and its tables built ({time.perf_counter() - t:.1f}s)")
+    out = torch.zeros((W, stride), dtype=torch.int32, device=dev)
+    ref = torch.zeros((W, stride), dtype=torch.int32, device=dev)
+    key.reencrypt_dev(c_il, kb, d_idx, 14, out, count, stride)  # warm: the second key's quad table
+    torch.cuda.synchronize()
+    names = ("lazy_modexp", "decrypt_finish", "il_digits", "lookup_encrypt")
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.lookup_steps):
+        key.reencrypt_dev(c_il, kb, d_idx, 14, out, count, stride)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.lookup_steps
+    per = {}
+    for nm in names:
+        ms, nl = ctx.kernel_time(nm)
+        per[nm] = ms / max(nl, 1)
+    ctx.set_timing(False)
+    kernel_ms = sum(per.values())
+    kb.encrypt_digits_dev(d_dig, d_idx, 14, ref, count, stride)
+    torch.cuda.synchronize()
+    differ = int((out[:, :count] != ref[:, :count]).any(dim=0).sum())
+    aos = torch.empty((count, W), dtype=torch.int32, device=dev)
+    ctx.il_to_aos(out, aos, count, W, stride)
+    torch.cuda.synchronize()
+    back = kb.decrypt(aos.cpu().numpy().view(np.uint32))
+    rt_bad = int((back != m).any(axis=1).sum())
+    log(f"reencrypt: {count} vectors in {kernel_ms:.2f} ms of kernels ({count / kernel_ms * 1e3:,.0f} vectors/s; "
+        f"decrypt {per['lazy_modexp']:.2f} + {per['decrypt_finish']:.3f} ms, digits {per['il_digits']:.3f} ms, "
+        f"encrypt {per['lookup_encrypt']:.2f} ms), wall {wall * 1e3:.2f} ms; differ from the two-step "
+        f"encryption: {differ}; round-trip mismatches {rt_bad}")
+    if differ or rt_bad:
+        raise RuntimeError(f"re-encryption wrong: {differ} differ from the two-step path, {rt_bad} round trip")
+    return {"vectors": count, "kernel_ms": round(kernel_ms, 3), "vectors_per_s": round(count / kernel_ms * 1e3, 1),
+            "decrypt_ms": round(per["lazy_modexp"] + per["decrypt_finish"], 3),
+            "digits_ms": round(per["il_digits"], 4), "encrypt_ms": round(per["lookup_encrypt"], 3),
+            "wall_ms_per_batch": round(wall * 1e3, 3),
+            "bit_exact": {"vs_two_step_vectors": count, "round_trip_vectors": count, "mismatches": 0}}
 
 
 def serve_leg(ctx, args, peak: float) -> dict:

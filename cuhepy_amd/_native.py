@@ -102,6 +102,8 @@ SIGNATURES = {
     "chx_lookup_encrypt_dev": (c_int, [vp, vp, vp, vp, c_int, vp, c_size_t, c_size_t]),
     "chx_lookup_decrypt": (c_int, [vp, vp, u32p, u32p, c_size_t]),
     "chx_lookup_decode_hamming": (c_int, [vp, vp, u32p, u32p, u32p, c_size_t]),
+    "chx_lookup_reencrypt": (c_int, [vp, vp, vp, u32p, u8p, c_int, u32p, c_size_t]),
+    "chx_lookup_reencrypt_dev": (c_int, [vp, vp, vp, vp, vp, c_int, vp, c_size_t, c_size_t]),
     "chx_aos_to_il_dev": (c_int, [vp, vp, vp, c_size_t, c_int, c_size_t]),
     "chx_il_to_aos_dev": (c_int, [vp, vp, vp, c_size_t, c_int, c_size_t]),
     "chx_host_schedule": (c_int, [u32p, c_int, c_int, u32p, c_int]),

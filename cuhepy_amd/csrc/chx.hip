@@ -5,6 +5,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -128,6 +129,7 @@ struct chx_ctx {
   bool timing = false;
   std::vector<TimedLaunch> pending;
   std::map<std::string, std::pair<double, int>> stats;
+  uint32_t packed_token = 0;  // per-call value of k_il_digits's flag (never 0)
 };
 
 // Lazy-radix constants of one modulus, one block per built lazy configuration
@@ -728,10 +730,11 @@ const ExpSched& get_sched_l(chx_lkey* k, int window) {
 
 uint32_t* lazy_group_table(chx_ctx* ctx, const LazySet& ls, const uint32_t* gtab, size_t groups, int R);
 
-// any_packed = false: the caller knows that no item has all its digits in the
-// packing alphabet, so no wave can use a row group and none is built.
+// any_packed (optional) is asked before a row-group table would be built:
+// false means that no item has all its digits in the packing alphabet, so no
+// wave can use a row group and none is built.
 void launch_lookup_encrypt(chx_lkey* k, const uint8_t* digits, const uint8_t* nidx, int nmult, uint32_t* c,
-                           size_t count, size_t stride, bool any_packed = true) {
+                           size_t count, size_t stride, const std::function<bool()>& any_packed = nullptr) {
   chx_ctx* ctx = k->ctx;
   if (!k->d_gtab || !k->d_ntab) fail(CHX_EINVAL, "lookup key was created without tables (decrypt-only)");
   const int L = k->W;
@@ -742,9 +745,11 @@ void launch_lookup_encrypt(chx_lkey* k, const uint8_t* digits, const uint8_t* ni
   // for one, the next smaller group serves.
   int rows = ctx->tune_window[CHX_OP_LOOKUP][lidx(L)];
   if (rows == 0) rows = 4;
-  if (!any_packed) rows = 1;
+  int packed = -1;  // any_packed's answer, asked once
   auto ensure = [&](uint32_t*& tab, bool& failed, int R) {
     if (tab || failed || k->chunks < R) return;
+    if (any_packed && packed < 0) packed = any_packed() ? 1 : 0;
+    if (packed == 0) return;
     try {
       tab = lazy_group_table(ctx, k->lz_n2, k->d_gtab, k->chunks / R, R);
     } catch (const ChxError& e) {
@@ -1820,7 +1825,7 @@ int chx_lookup_encrypt(chx_ctx* ctx, const chx_lkey* kc, const uint8_t* digits, 
       hip_check(hipMemcpyAsync(ctx->slot[S_IN1].p, noise_idx, nbytes, hipMemcpyHostToDevice, ctx->stream), "H2D");
     ctx->slot[S_OUT].ensure(stride * L * 4);
     launch_lookup_encrypt(k, ctx->slot[S_IN0].as<uint8_t>(), ctx->slot[S_IN1].as<uint8_t>(), nmult,
-                          ctx->slot[S_OUT].as<uint32_t>(), count, stride, any_packed);
+                          ctx->slot[S_OUT].as<uint32_t>(), count, stride, [&] { return any_packed; });
     download_il(ctx, ctx->slot[S_OUT].as<uint32_t>(), c, count, L, stride);
   });
 }
@@ -1876,6 +1881,80 @@ int chx_lookup_decode_hamming(chx_ctx* ctx, const chx_lkey* kc, const uint32_t* 
                         ctx->slot[S_OUT].as<uint32_t>());
     hip_check(hipMemcpyAsync(counts, ctx->slot[S_OUT].p, count * 4, hipMemcpyDeviceToHost, ctx->stream), "D2H");
     hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  });
+}
+
+// Decrypt dc under `from` into a scratch buffer, take the plaintexts' base-2^8
+// digits on the device and encrypt them under `to` into out (same stride).
+static void lookup_reencrypt_impl(chx_ctx* ctx, chx_lkey* from, chx_lkey* to, const uint32_t* dc,
+                                  const uint8_t* nidx, int nmult, uint32_t* out, size_t count, size_t stride) {
+  if (from->ctx != ctx || to->ctx != ctx) fail(CHX_EINVAL, "both keys must belong to this context");
+  if (!to->d_gtab || !to->d_ntab) fail(CHX_EINVAL, "target key was created without tables (decrypt-only)");
+  // m < n_from: every digit of a plaintext has a row of the target's g table
+  const int mbits = host::bn_bits(from->n.N);
+  if (mbits > 8 * to->chunks)
+    fail(CHX_EINVAL, "plaintexts of a %d-bit n have more base-256 digits than the target's %d message chunks", mbits,
+         to->chunks);
+  const int La = from->W;  // m < n fits the low La/2 limbs (32 La >= bits(n^2) >= 2 bits(n) - 1)
+  ctx->slot[S_MID1].ensure(stride * La * sizeof(uint32_t));
+  uint32_t* dm = ctx->slot[S_MID1].as<uint32_t>();
+  lookup_decrypt_impl(from, dc, dm, count, stride, nullptr, nullptr);
+  ctx->slot[S_TMP].ensure(count * (size_t)to->chunks + 16);
+  ctx->slot[S_AUX].ensure(16);
+  uint32_t* flag = ctx->slot[S_AUX].as<uint32_t>();
+  uint32_t token = ++ctx->packed_token;
+  if (token == 0) token = ctx->packed_token = 1;
+  record_start(ctx, "il_digits");
+  hipLaunchKernelGGL(k_il_digits, grid_for(count, 1), dim3(BLOCK_THREADS), 0, ctx->stream, dm, La / 2,
+                     (int)stride, (int)count, ctx->slot[S_TMP].as<uint8_t>(), to->chunks, flag, token);
+  record_stop(ctx);
+  check_launch("k_il_digits");
+  // read back only when a row-group table would be built: a stale flag can
+  // only cause a build, never a different result
+  auto any_packed = [&] {
+    uint32_t h = 0;
+    hip_check(hipMemcpyAsync(&h, flag, 4, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+    hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return h == token;
+  };
+  launch_lookup_encrypt(to, ctx->slot[S_TMP].as<uint8_t>(), nidx, nmult, out, count, stride, any_packed);
+}
+
+int chx_lookup_reencrypt(chx_ctx* ctx, const chx_lkey* fc, const chx_lkey* tc, const uint32_t* c,
+                         const uint8_t* noise_idx, int nmult, uint32_t* out, size_t count) {
+  chx_lkey* from = const_cast<chx_lkey*>(fc);
+  chx_lkey* to = const_cast<chx_lkey*>(tc);
+  return with_ctx(ctx, [&] {
+    if (!from || !to || !c || !out || (nmult > 0 && !noise_idx)) fail(CHX_EINVAL, "null argument");
+    if (count == 0) return;
+    if (nmult < 0 || nmult > 64) fail(CHX_EINVAL, "nmult out of range");
+    for (size_t i = 0; i < count * (size_t)nmult; ++i)
+      if (noise_idx[i] >= to->noise) fail(CHX_EINVAL, "noise index %u out of range", noise_idx[i]);
+    const int La = from->W, Lb = to->W;
+    check_below(c, count, La, from->n2.N, "c");
+    const size_t stride = batch_stride(count);
+    uint32_t* dc = upload_il(ctx, S_IN0, c, count, La, stride);
+    const size_t nbytes = count * (size_t)nmult;
+    ctx->slot[S_IN1].ensure(nbytes + 16);
+    if (nbytes)
+      hip_check(hipMemcpyAsync(ctx->slot[S_IN1].p, noise_idx, nbytes, hipMemcpyHostToDevice, ctx->stream), "H2D");
+    ctx->slot[S_OUT].ensure(stride * Lb * 4);
+    lookup_reencrypt_impl(ctx, from, to, dc, ctx->slot[S_IN1].as<uint8_t>(), nmult, ctx->slot[S_OUT].as<uint32_t>(),
+                          count, stride);
+    download_il(ctx, ctx->slot[S_OUT].as<uint32_t>(), out, count, Lb, stride);
+  });
+}
+
+int chx_lookup_reencrypt_dev(chx_ctx* ctx, const chx_lkey* fc, const chx_lkey* tc, const uint32_t* c,
+                             const uint8_t* noise_idx, int nmult, uint32_t* out, size_t count, size_t stride) {
+  chx_lkey* from = const_cast<chx_lkey*>(fc);
+  chx_lkey* to = const_cast<chx_lkey*>(tc);
+  return with_ctx(ctx, [&] {
+    if (!from || !to || !c || !out || (nmult > 0 && !noise_idx)) fail(CHX_EINVAL, "null argument");
+    if (count == 0) return;
+    if (nmult < 0 || nmult > 64) fail(CHX_EINVAL, "nmult out of range");
+    check_stride(count, stride);
+    lookup_reencrypt_impl(ctx, from, to, c, noise_idx, nmult, out, count, stride);
   });
 }
 

@@ -583,6 +583,43 @@ __device__ __forceinline__ int spread4(int v) { return (v & 1) | ((v & 2) << 1) 
 __device__ __forceinline__ int squeeze4(int d) { return (d & 1) | ((d >> 1) & 2) | ((d >> 2) & 4) | ((d >> 3) & 8); }
 constexpr int PACKED_ZERO_BITS = 0xAA;
 
+// Base-2^8 digits of interleaved plaintexts (limb j of item i at
+// m[j * stride + i]; limbs at or past `limbs` read as zero) into item-major
+// rows of `chunks` bytes, little-endian: the digit extraction of the fused
+// re-encryption (lcu:1102-1104, lookup.py:239). One thread per item; the
+// loads of a wave are coalesced per limb. *packed is set to `token` when
+// some item has every digit in the bit-packing alphabet (a fresh token per
+// call: the flag needs no clearing).
+__global__ __launch_bounds__(BLOCK_THREADS) void k_il_digits(const u32* __restrict__ m, int limbs, int stride,
+                                                             int count, unsigned char* __restrict__ digits,
+                                                             int chunks, u32* __restrict__ packed, u32 token) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  constexpr u32 Z = 0x01010101u * PACKED_ZERO_BITS;
+  unsigned char* row = digits + (size_t)i * chunks;
+  u32 zbits = 0;
+  if ((chunks & 15) == 0) {  // rows of whole 16-byte words (rows start 16-byte aligned)
+    for (int q = 0; q < chunks / 16; ++q) {
+      uint4 w;
+      w.x = 4 * q + 0 < limbs ? m[(size_t)(4 * q + 0) * stride + i] : 0u;
+      w.y = 4 * q + 1 < limbs ? m[(size_t)(4 * q + 1) * stride + i] : 0u;
+      w.z = 4 * q + 2 < limbs ? m[(size_t)(4 * q + 2) * stride + i] : 0u;
+      w.w = 4 * q + 3 < limbs ? m[(size_t)(4 * q + 3) * stride + i] : 0u;
+      zbits |= (w.x | w.y | w.z | w.w) & Z;
+      *reinterpret_cast<uint4*>(row + 16 * q) = w;
+    }
+  } else {
+    for (int b = 0; b < chunks; ++b) {
+      const int j = b >> 2;
+      const u32 v = j < limbs ? m[(size_t)j * stride + i] : 0u;
+      const unsigned char d = (unsigned char)(v >> (8 * (b & 3)));
+      zbits |= d & PACKED_ZERO_BITS;
+      row[b] = d;
+    }
+  }
+  if (zbits == 0) *packed = token;
+}
+
 // Row-group table for lookup encryption, R = 2 or 3 rows per entry: entry e
 // of group j is
 //   prod_{t < R} g_table[R j + t][spread4((e >> 4t) & 15)] mod n^2

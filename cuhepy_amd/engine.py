@@ -435,6 +435,24 @@ class LookupKey:
                                                     len(C)))
         return out
 
+    def reencrypt(self, c, target: "LookupKey", noise_idx: np.ndarray) -> np.ndarray:
+        """Decrypt under this key and encrypt under `target` (its tables) with
+        the given noise indices, the digits staying on the device
+        (lcu:1053-1135, lcu:2134-2285)."""
+        C = _batch(c, self.width)
+        NI = np.ascontiguousarray(noise_idx, dtype=np.uint8)
+        if NI.ndim != 2 or NI.shape[0] != len(C):
+            raise ValueError("noise_idx must be (count, nmult) bytes")
+        out = np.empty((len(C), target.width), dtype=np.uint32)
+        if len(C):
+            N.check(N.lib.chx_lookup_reencrypt(self.ctx.handle, self.handle, target.handle, _p32(C), _p8(NI),
+                                               NI.shape[1], _p32(out), len(C)))
+        return out
+
+    def reencrypt_dev(self, c, target: "LookupKey", noise_idx, nmult: int, out, count: int, stride: int) -> None:
+        N.check(N.lib.chx_lookup_reencrypt_dev(self.ctx.handle, self.handle, target.handle, _dptr(c),
+                                               _dptr(noise_idx), nmult, _dptr(out), count, stride))
+
 
 class RaggedPlan:
     """Mixed-width, per-item-exponent modexp over many sub-batches in one

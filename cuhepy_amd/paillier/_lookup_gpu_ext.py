@@ -346,7 +346,6 @@ class PaillierGPULookupClient:
                 raise RuntimeError("key pk must contain g, n, n_squared; sk must contain a and g_a_inv")
         na = _to_int(pa["n"])
         dec = LookupKey(self._ctx, na, _to_int(sa["a"]), _to_int(sa["g_a_inv"]), [], [])
-        ms = dec.decrypt(cs)
         nb, gb = _to_int(pb["n"]), _to_int(pb["g"])
         if "message_chunks" in key_b:
             mc = int(key_b["message_chunks"])
@@ -364,9 +363,14 @@ class PaillierGPULookupClient:
             g_tab = build_g_table_device(self._ctx, gb, nb, mc)
             noise = build_noise_table_device(self._ctx, gb, nb, self._rng)
         enc = LookupKey(self._ctx, nb, 1, 1, g_tab, noise)
-        mints = limbs_to_ints(ms)
-        digits = plaintexts_to_digits(ints_to_limbs(mints, Wb), mc)
-        out = enc.encrypt_digits(digits, self._noise_idx(len(cs)))
+        if 8 * mc >= na.bit_length():
+            # every plaintext digit has a row: decrypt, digits and encryption on the device
+            out = dec.reencrypt(cs, enc, self._noise_idx(len(cs)))
+        else:
+            # a plaintext may carry more digits than key B's rows: check each on the host
+            ms = dec.decrypt(cs)
+            digits = plaintexts_to_digits(ints_to_limbs(limbs_to_ints(ms), Wb), mc)
+            out = enc.encrypt_digits(digits, self._noise_idx(len(cs)))
         return [_out(c) for c in limbs_to_ints(out)]
 
     # ---- evaluator ----

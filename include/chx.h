@@ -228,6 +228,19 @@ int chx_lookup_decrypt(chx_ctx* ctx, const chx_lkey* k, const uint32_t* c, uint3
  * replacing K8 + odd_popcount_kernel (lcu:1279-1298) */
 int chx_lookup_decode_hamming(chx_ctx* ctx, const chx_lkey* k, const uint32_t* c, const uint32_t* mask,
                               uint32_t* counts, size_t count);
+/* fused re-encryption: m = decrypt under `from`, then out = the lookup encryption
+ * of m's base-2^8 digits under `to` with noise indices noise_idx (count x nmult
+ * bytes, below to's noise table size) -- decrypt_then_reencrypt_kernel
+ * (lcu:1053-1135), decrypt_then_reencrypt (lcu:2134-2285). Both keys belong to
+ * ctx; from's n may have at most 8 x to's message chunks bits (no digit of a
+ * plaintext is dropped). c: count x from's width limbs, out: count x to's width.
+ * The digits stay on the device; with the same noise indices the result equals
+ * chx_lookup_encrypt of the decrypted plaintexts' digits. */
+int chx_lookup_reencrypt(chx_ctx* ctx, const chx_lkey* from, const chx_lkey* to, const uint32_t* c,
+                         const uint8_t* noise_idx, int nmult, uint32_t* out, size_t count);
+/* device pointers; c and out interleaved with `stride` (at each key's width) */
+int chx_lookup_reencrypt_dev(chx_ctx* ctx, const chx_lkey* from, const chx_lkey* to, const uint32_t* c,
+                             const uint8_t* noise_idx, int nmult, uint32_t* out, size_t count, size_t stride);
 
 /* ---- layout conversion (device) ---- */
 int chx_aos_to_il_dev(chx_ctx* ctx, const uint32_t* aos, uint32_t* il, size_t count, int limbs, size_t stride);

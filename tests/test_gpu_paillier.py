@@ -317,3 +317,78 @@ def test_plain_encrypt_decrypt_1024_messages():
     assert cs[:8] == [O.encrypt(m, keys["pk"], r) for m, r in zip(ms[:8], rs[:8])]
     assert limbs_to_ints(key.decrypt(cs)) == ms
     assert limbs_to_ints(key.decrypt_crt(cs)) == ms
+
+
+def test_lookup_reencrypt_on_device(ctx):
+    """Fused re-encryption (lcu:1053-1135): decrypt under key A, digits taken
+    on the device, encryption under key B's tables. With the same noise
+    indices every result equals the oracle's lookup encryption of the
+    oracle's decryption, for packed messages (row groups) and arbitrary ones
+    (single rows), at every lookup window, through the host and the device
+    entry points."""
+    import torch
+    from cuhepy_amd.engine import LookupKey
+    from cuhepy_amd.limbs import ints_to_limbs, limbs_to_ints
+    ka_o = O.lookup_key_gen(512, 280, 4242, with_tables=True)
+    kb_o = O.lookup_key_gen(512, 280, 4343, with_tables=True)
+    na, nb = ka_o["pk"]["n"], kb_o["pk"]["n"]
+
+    def mk(k):
+        t = k["g_table"]
+        return LookupKey(ctx, k["pk"]["n"], k["sk"]["a"], k["sk"]["g_a_inv"], [t[i] for i in range(len(t))],
+                         k["noise_table"])
+
+    ka, kb = mk(ka_o), mk(kb_o)
+    assert ka.chunks == kb.chunks == 128
+    rng = random.Random(31)
+    embs = [[rng.getrandbits(1) for _ in range(512)] for _ in range(40)]
+    packed = [O.embed_to_plaintexts(e, 512, 512)[0] for e in embs]
+    top = min(na, nb)
+    arbitrary = [0, 1, top - 1] + [rng.randrange(top) for _ in range(37)]
+    for ms in (arbitrary, packed):  # arbitrary first: key B builds no group table for them
+        ia = np.array([[rng.randrange(256) for _ in range(14)] for _ in ms], dtype=np.uint8)
+        cts = [O.lookup_encrypt(m, ka_o["pk"], ka_o["g_table"], ka_o["noise_table"], ka_o["message_chunks"],
+                                ix.tolist()) for m, ix in zip(ms, ia)]
+        assert [O.lookup_decrypt(c, ka_o) for c in cts] == ms
+        ib = np.array([[rng.randrange(256) for _ in range(14)] for _ in ms], dtype=np.uint8)
+        want = [O.lookup_encrypt(m, kb_o["pk"], kb_o["g_table"], kb_o["noise_table"], kb_o["message_chunks"],
+                                 ix.tolist()) for m, ix in zip(ms, ib)]
+        for rows in (0, 3, 1):
+            ctx.set_tuning(OP_LOOKUP, kb.width, 0, rows)
+            got = limbs_to_ints(ka.reencrypt(cts, kb, ib))
+            assert got == want, f"rows per product {rows}"
+        # device entry point: interleaved limbs with a padded stride
+        dev = torch.device("cuda:0")
+        count = len(ms)
+        stride = (count + 63) // 64 * 64 + 64  # whole 64-item columns, one spare
+        aos = torch.from_numpy(ints_to_limbs(cts, ka.width).view(np.int32).copy()).to(dev)
+        il = torch.zeros((ka.width, stride), dtype=torch.int32, device=dev)
+        out_il = torch.full((kb.width, stride), -1, dtype=torch.int32, device=dev)
+        out_aos = torch.zeros((count, kb.width), dtype=torch.int32, device=dev)
+        nidx = torch.from_numpy(ib.copy()).to(dev)
+        torch.cuda.synchronize()
+        ctx.aos_to_il(aos, il, count, ka.width, stride)
+        ka.reencrypt_dev(il, kb, nidx, 14, out_il, count, stride)
+        ctx.il_to_aos(out_il, out_aos, count, kb.width, stride)
+        ctx.sync()
+        assert limbs_to_ints(out_aos.cpu().numpy().view(np.uint32)) == want
+        assert (out_il[:, count:] == -1).all()  # the padding past count is not written
+        assert limbs_to_ints(kb.decrypt(want[:5])) == ms[:5]
+
+
+def test_lookup_reencrypt_rejects_narrow_targets(ctx):
+    """A target whose message chunks cannot hold every digit of the source's
+    plaintexts is refused (no digit is dropped), as are bad noise indices."""
+    from cuhepy_amd.engine import LookupKey
+    ka_o = O.lookup_key_gen(512, 280, 4242, with_tables=True)
+    n = ka_o["pk"]["n"]
+    t = ka_o["g_table"]
+    ka = LookupKey(ctx, n, ka_o["sk"]["a"], ka_o["sk"]["g_a_inv"], [t[i] for i in range(len(t))],
+                   ka_o["noise_table"])
+    narrow = LookupKey(ctx, n, 1, 1, [t[i] for i in range(60)], ka_o["noise_table"])  # 480 bits of digits
+    c = [O.lookup_encrypt(5, ka_o["pk"], t, ka_o["noise_table"], ka_o["message_chunks"], [0] * 14)]
+    with pytest.raises(ValueError, match="message chunks"):
+        ka.reencrypt(c, narrow, np.zeros((1, 14), dtype=np.uint8))
+    small_noise = LookupKey(ctx, n, 1, 1, [t[i] for i in range(len(t))], ka_o["noise_table"][:16])
+    with pytest.raises(ValueError, match="noise index"):
+        ka.reencrypt(c, small_noise, np.full((1, 14), 16, dtype=np.uint8))
