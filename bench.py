@@ -382,6 +382,7 @@ def _reencrypt_block(ctx, args, key, c_il, d_dig, d_idx, m, count: int, stride: 
     import numpy as np
     import torch
 
+    from cuhepy_amd.engine import LookupKey
     from cuhepy_amd.paillier._lookup_gpu_ext import PaillierGPULookupClient
 
     dev = torch.device("cuda:0")
@@ -412,8 +413,22 @@ def _reencrypt_block(ctx, args, key, c_il, d_dig, d_idx, m, count: int, stride: 
     aos = torch.empty((count, W), dtype=torch.int32, device=dev)
     ctx.il_to_aos(out, aos, count, W, stride)
     torch.cuda.synchronize()
-    back = kb.decrypt(aos.cpu().numpy().view(np.uint32))
+    cts_b = aos.cpu().numpy().view(np.uint32)
+    dec_ms = {}
+    for form, k in (("half_width_crt", kb), ("full_width", LookupKey(ctx, kb.n, kb.a, kb.g_a_inv, [], []))):
+        assert k.crt == (form == "half_width_crt")
+        ctx.reset_timing()
+        ctx.set_timing(True)
+        got = k.decrypt(cts_b)
+        ctx.set_timing(False)
+        dec_ms[form] = sum(ctx.kernel_time(nm)[0] for nm in ("lazy_modexp", "decrypt_finish"))
+        if form == "half_width_crt":
+            back = got
+        elif not np.array_equal(got, back):
+            raise RuntimeError("half-width and full-width lookup decryptions differ")
     rt_bad = int((back != m).any(axis=1).sum())
+    log(f"lookup decrypt of {count} ciphertexts: half width (CRT) {dec_ms['half_width_crt']:.2f} ms, "
+        f"full width {dec_ms['full_width']:.2f} ms of kernels, results equal")
     log(f"reencrypt: {count} vectors in {kernel_ms:.2f} ms of kernels ({count / kernel_ms * 1e3:,.0f} vectors/s; "
         f"decrypt {per['lazy_modexp']:.2f} + {per['decrypt_finish']:.3f} ms, digits {per['il_digits']:.3f} ms, "
         f"encrypt {per['lookup_encrypt']:.2f} ms), wall {wall * 1e3:.2f} ms; differ from the two-step "
@@ -423,8 +438,10 @@ def _reencrypt_block(ctx, args, key, c_il, d_dig, d_idx, m, count: int, stride: 
     return {"vectors": count, "kernel_ms": round(kernel_ms, 3), "vectors_per_s": round(count / kernel_ms * 1e3, 1),
             "decrypt_ms": round(per["lazy_modexp"] + per["decrypt_finish"], 3),
             "digits_ms": round(per["il_digits"], 4), "encrypt_ms": round(per["lookup_encrypt"], 3),
-            "wall_ms_per_batch": round(wall * 1e3, 3),
-            "bit_exact": {"vs_two_step_vectors": count, "round_trip_vectors": count, "mismatches": 0}}
+            "wall_ms_per_batch": round(wall * 1e3, 3), "decrypt_form": "half_width_crt" if key.crt else "full_width",
+            "lookup_decrypt_kernel_ms": {k: round(v, 3) for k, v in dec_ms.items()},
+            "bit_exact": {"vs_two_step_vectors": count, "round_trip_vectors": count, "mismatches": 0,
+                          "half_width_equals_full_width": True}}
 
 
 def serve_leg(ctx, args, peak: float) -> dict:

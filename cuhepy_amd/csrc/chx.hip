@@ -189,6 +189,10 @@ struct chx_lkey {
   LazySet lz_n2;
   BN e_a;
   std::map<int, ExpSched> scheds;
+  // half-width decryption (chx_lookup_key_set_crt): c^a mod p^2 and mod q^2
+  bool crt = false;
+  LazySet lz_p2, lz_q2;
+  uint32_t* d_crt = nullptr;  // [e_p R | e_q R] mod n^2, W limbs each
 };
 
 namespace {
@@ -1786,13 +1790,72 @@ int chx_lookup_key_destroy(chx_lkey* k) {
                     (void*)k->d_ptab, (void*)k->d_ttab, (void*)k->d_qtab, (void*)k->d_nptab,
                     (void*)k->d_ntab_plain, (void*)k->d_nptab_plain})
       if (p) (void)hipFree(p);
+    if (k->d_crt) (void)hipFree(k->d_crt);
     free_lazy_set(k->lz_n2);
+    free_lazy_set(k->lz_p2);
+    free_lazy_set(k->lz_q2);
     for (auto& s : k->scheds)
       if (s.second.d) (void)hipFree(s.second.d);
   });
   delete k;
   return rc;
 }
+
+int chx_lookup_key_set_crt(chx_ctx* ctx, chx_lkey* k, const uint32_t* p, const uint32_t* q, int half_limbs,
+                           const uint32_t* ep, const uint32_t* eq) {
+  return with_ctx(ctx, [&] {
+    if (!k || !p || !q || !ep || !eq) fail(CHX_EINVAL, "null argument");
+    if (k->ctx != ctx) fail(CHX_EINVAL, "key belongs to another context");
+    if (k->crt) fail(CHX_EINVAL, "key already has CRT factors");
+    const int W = k->W, H = W / 2;
+    if (H != 32 && H != 64) fail(CHX_EUNSUPPORTED, "CRT needs 32- or 64-limb halves (width %d)", W);
+    if (half_limbs <= 0 || half_limbs > H) fail(CHX_EINVAL, "factors wider than half the width");
+    BN pb = host::bn_from(p, half_limbs, H), qb = host::bn_from(q, half_limbs, H);
+    // the halves exponentiate modulo p^2 and q^2 at H limbs
+    if (host::bn_bits(pb) > 16 * H || host::bn_bits(qb) > 16 * H)
+      fail(CHX_EINVAL, "p and q must be below 2^%d for %d-limb CRT halves", 16 * H, H);
+    if (host::bn_bits(pb) < 2 || host::bn_bits(qb) < 2 || !(pb[0] & 1u) || !(qb[0] & 1u))
+      fail(CHX_EINVAL, "p and q must be odd and above 1");
+    BN pq(W, 0u);  // p * q at W limbs
+    for (int i = 0; i < H; ++i) {
+      uint64_t c = 0;
+      for (int j = 0; j < H; ++j) {
+        const uint64_t t = (uint64_t)pb[i] * qb[j] + pq[i + j] + c;
+        pq[i + j] = (uint32_t)t;
+        c = t >> 32;
+      }
+      pq[i + H] = (uint32_t)c;
+    }
+    if (host::bn_cmp(pq, k->n.N) != 0) fail(CHX_EINVAL, "p * q != n");
+    BN p2 = host::bn_sqr_lo(pb), q2 = host::bn_sqr_lo(qb);
+    BN e1 = host::bn_from(ep, W, W), e2 = host::bn_from(eq, W, W);
+    if (host::bn_cmp(e1, k->n2.N) >= 0 || host::bn_cmp(e2, k->n2.N) >= 0)
+      fail(CHX_EINVAL, "CRT idempotents must be below n^2");
+    BN one(H, 0u), zero(H, 0u);
+    one[0] = 1;
+    if (host::bn_cmp(host::bn_mod(e1, p2), one) != 0 || host::bn_cmp(host::bn_mod(e1, q2), zero) != 0 ||
+        host::bn_cmp(host::bn_mod(e2, q2), one) != 0 || host::bn_cmp(host::bn_mod(e2, p2), zero) != 0)
+      fail(CHX_EINVAL, "e_p must be 1 mod p^2 and 0 mod q^2, e_q the converse");
+    std::vector<uint32_t> er(2 * (size_t)W);
+    BN t1 = host::to_mont_host(e1, k->n2), t2 = host::to_mont_host(e2, k->n2);
+    std::copy(t1.begin(), t1.end(), er.begin());
+    std::copy(t2.begin(), t2.end(), er.begin() + W);
+    try {  // a failure below leaves the key without CRT factors
+      k->lz_p2 = make_lazy_set(host::mont_consts(p2), ctx->stream);
+      k->lz_q2 = make_lazy_set(host::mont_consts(q2), ctx->stream);
+      k->d_crt = dev_upload(er.data(), 2 * W, ctx->stream);
+    } catch (...) {
+      free_lazy_set(k->lz_p2);
+      free_lazy_set(k->lz_q2);
+      if (k->d_crt) (void)hipFree(k->d_crt);
+      k->d_crt = nullptr;
+      throw;
+    }
+    k->crt = true;
+  });
+}
+
+int chx_lookup_key_has_crt(const chx_lkey* k) { return k && k->crt ? 1 : 0; }
 
 int chx_lookup_width(const chx_lkey* k) { return k ? k->W : 0; }
 
@@ -1841,10 +1904,41 @@ int chx_lookup_encrypt_dev(chx_ctx* ctx, const chx_lkey* kc, const uint8_t* digi
   });
 }
 
+// Half-width form: c^a mod p^2 and mod q^2 in one launch (both halves read the
+// whole ciphertext and reduce it on entry), then k_lookup_crt_finish
+// recombines c^a mod n^2 and applies L and g_a_inv as the full-width finish.
+static void launch_lookup_crt(chx_lkey* k, const uint32_t* dc, uint32_t* dm, size_t count, size_t stride,
+                              const uint32_t* mask, uint32_t* pop) {
+  chx_ctx* ctx = k->ctx;
+  const int L = k->W, H = L / 2;
+  const ExpSched& s = get_sched_l(k, pick_window(ctx, CHX_OP_MODEXP, H, host::bn_bits(k->e_a)));
+  const size_t half_words = (size_t)H * stride;
+  ctx->slot[S_MID0].ensure(2 * half_words * sizeof(uint32_t));
+  uint32_t* u = ctx->slot[S_MID0].as<uint32_t>();
+  const LazySet sets[2] = {k->lz_p2, k->lz_q2};
+  const ExpSched* scheds[2] = {&s, &s};
+  const uint32_t* ins[2] = {dc, dc};
+  const uint32_t* his[2] = {dc + half_words, dc + half_words};
+  uint32_t* outs[2] = {u, u + half_words};
+  run_lazy(ctx, H, sets, scheds, ins, outs, 2, count, stride, his);
+  const int tpi = pick_tpi(ctx, CHX_OP_MULMOD, L, count);
+  record_start(ctx, "decrypt_finish");
+  CHX_DISPATCH(L, tpi, {
+    hipLaunchKernelGGL((k_lookup_crt_finish<L, TPI>), grid_for(count, TPI), dim3(BLOCK_THREADS), lds_bytes(L, TPI),
+                       ctx->stream, k->d_n2, k->d_crt, k->d_n, k->d_gar, u, dm, (int)count, (int)stride, mask, pop);
+  });
+  record_stop(ctx);
+  check_launch("k_lookup_crt_finish");
+}
+
 static void lookup_decrypt_impl(chx_lkey* k, const uint32_t* dc, uint32_t* dm, size_t count, size_t stride,
                                 const uint32_t* mask, uint32_t* pop) {
   chx_ctx* ctx = k->ctx;
   const int L = k->W;
+  if (k->crt) {
+    launch_lookup_crt(k, dc, dm, count, stride, mask, pop);
+    return;
+  }
   const ExpSched& s = get_sched_l(k, pick_window(ctx, CHX_OP_MODEXP, L, host::bn_bits(k->e_a)));
   launch_decrypt(ctx, k->lz_n2, k->d_n, k->d_gar, L, dc, dm, count, stride, s, mask, pop);
 }

@@ -287,6 +287,52 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_decrypt_finish(const ModDev* 
   }
 }
 
+// Lookup decryption from half-width residues (the CRT form of
+// paillier/lookup.py:257-278). U holds x_p = c^a mod p^2 in rows [0, H) and
+// x_q = c^a mod q^2 in rows [H, 2H), H = L/2. With the idempotents
+// e_p (1 mod p^2, 0 mod q^2) and e_q (0 mod p^2, 1 mod q^2),
+// x = x_p e_p + x_q e_q mod n^2 is c^a mod n^2 for every c in [0, n^2), unit
+// or not; eR = [e_p R | e_q R] mod n^2 (L limbs each). Then as
+// k_decrypt_finish: m = L(x) * g_a_inv mod n, optionally the odd-bit count.
+template <int L, int TPI>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_lookup_crt_finish(const ModDev* __restrict__ n2m,
+                                                                     const u32* __restrict__ eR,
+                                                                     const ModDev* __restrict__ nm,
+                                                                     const u32* __restrict__ cR,
+                                                                     const u32* __restrict__ U, u32* __restrict__ out,
+                                                                     int count, int stride,
+                                                                     const u32* __restrict__ mask,
+                                                                     u32* __restrict__ popcnt) {
+  extern __shared__ uint4 smem[];
+  constexpr int K = L / TPI, H = L / 2;
+  const int item = group_item<TPI>();
+  const int it = item < count ? item : count - 1;
+  const int r0 = Grp<TPI>::rank() * K;
+  u32 xp[K], xq[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {  // half-width residues zero-extended to L limbs
+    const int l = r0 + k;
+    xp[k] = l < H ? U[limb_index<L>(l, it, stride)] : 0u;
+    xq[k] = l < H ? U[limb_index<L>(H + l, it, stride)] : 0u;
+  }
+  Mont<L, TPI> M2;
+  mont_init<L, TPI>(M2, n2m, smem);
+  u32 e[K], x[K], y[K];
+  load_const<L, TPI>(e, eR);
+  M2.mul(x, xp, e);  // x_p e_p mod n^2 (x_p < R: the product is canonical)
+  load_const<L, TPI>(e, eR + L);
+  M2.mul(y, xq, e);  // x_q e_q mod n^2
+  add_mod<K, TPI>(x, y, M2.N);
+  Mont<L, TPI> Mn;
+  mont_init<L, TPI>(Mn, nm, smem);
+  paillier_finish<L, TPI>(Mn, nm, x, cR);
+  if (out != nullptr && item < count) store_dist<L, TPI>(out, x, item, stride);
+  if (popcnt != nullptr) {
+    const u32 c = group_popcount<L, TPI>(x, mask);
+    if (item < count && Grp<TPI>::rank() == 0) popcnt[item] = c;
+  }
+}
+
 // c = (1 + m*n) * y mod n^2 for y = r^n mod n^2 (cu:522-546 form of scheme.py:63-83)
 template <int L, int TPI>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_encrypt_finish(const PaillierDev* __restrict__ key,

@@ -402,6 +402,26 @@ class LookupKey:
         except Exception:
             pass
 
+    def set_crt(self, p: int, q: int) -> None:
+        """Decrypt at half width from now on: c^a mod p^2 and mod q^2,
+        recombined to c^a mod n^2 with the idempotents e_p, e_q, so every
+        c in [0, n^2) decrypts as the full-width form (lookup.py:257-278).
+        Raises (key unchanged) when p q != n or the width has no CRT form."""
+        if p * q != self.n:
+            raise ValueError("p * q != n")
+        p2, q2 = p * p, q * q
+        ep = q2 * pow(q2, -1, p2) % self.n2
+        eq = p2 * pow(p2, -1, q2) % self.n2
+        hl = max(limb_count(p), limb_count(q))
+        P, Q = ints_to_limbs([p], hl), ints_to_limbs([q], hl)
+        E = ints_to_limbs([ep, eq], self.width)
+        N.check(N.lib.chx_lookup_key_set_crt(self.ctx.handle, self.handle, _p32(P), _p32(Q), hl, _p32(E[0:1]),
+                                             _p32(E[1:2])))
+
+    @property
+    def crt(self) -> bool:
+        return bool(N.lib.chx_lookup_key_has_crt(self.handle))
+
     def encrypt_digits(self, digits: np.ndarray, noise_idx: np.ndarray) -> np.ndarray:
         D = np.ascontiguousarray(digits, dtype=np.uint8)
         NI = np.ascontiguousarray(noise_idx, dtype=np.uint8)

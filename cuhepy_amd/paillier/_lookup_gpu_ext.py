@@ -59,6 +59,18 @@ def _to_int(v) -> int:
     return int(v)
 
 
+def _with_crt(key: LookupKey, phi) -> LookupKey:
+    """Give `key` half-width decryption when phi = (p-1)(q-1) yields n's
+    factors and the width has a CRT form (n^2 of 2048 or 4096 bits); the
+    results are those of the full-width form for every ciphertext."""
+    if phi is None or key.width not in (64, 128):
+        return key
+    f = kg.factor_from_phi(key.n, _to_int(phi))
+    if f is not None and max(f).bit_length() <= 8 * key.width:  # p^2, q^2 fit half the width
+        key.set_crt(*f)
+    return key
+
+
 def build_g_table_device(ctx: Context, g: int, n: int, message_chunks: int) -> np.ndarray:
     """g_table[i][j] = (g^(2^(8i)))^j mod n^2 as a (chunks, 256, W) limb array.
 
@@ -156,7 +168,7 @@ class PaillierGPULookupClient:
         if self._dev is None:
             g_tab, noise = self._ensure_tables()
             k = self._keys
-            self._dev = LookupKey(self._ctx, k["n"], k["a"], k["g_a_inv"], g_tab, noise)
+            self._dev = _with_crt(LookupKey(self._ctx, k["n"], k["a"], k["g_a_inv"], g_tab, noise), k["phi"])
         return self._dev
 
     @property
@@ -313,7 +325,7 @@ class PaillierGPULookupClient:
             return self._dev
         k = self._keys
         if getattr(self, "_dec_only", None) is None or self._dec_only.n != k["n"]:
-            self._dec_only = LookupKey(self._ctx, k["n"], k["a"], k["g_a_inv"], [], [])
+            self._dec_only = _with_crt(LookupKey(self._ctx, k["n"], k["a"], k["g_a_inv"], [], []), k["phi"])
         return self._dec_only
 
     def decode_hamming_client(self, ciphers_batch: Sequence[Sequence]) -> list[int]:
@@ -345,7 +357,7 @@ class PaillierGPULookupClient:
             if not all(x in d for x in names):
                 raise RuntimeError("key pk must contain g, n, n_squared; sk must contain a and g_a_inv")
         na = _to_int(pa["n"])
-        dec = LookupKey(self._ctx, na, _to_int(sa["a"]), _to_int(sa["g_a_inv"]), [], [])
+        dec = _with_crt(LookupKey(self._ctx, na, _to_int(sa["a"]), _to_int(sa["g_a_inv"]), [], []), sa.get("phi"))
         nb, gb = _to_int(pb["n"]), _to_int(pb["g"])
         if "message_chunks" in key_b:
             mc = int(key_b["message_chunks"])

@@ -214,6 +214,20 @@ int chx_lookup_key_create(chx_ctx* ctx, const uint32_t* n, int n_limbs, const ui
                           const uint32_t* noise_table, int noise_entries, chx_lkey** out);
 int chx_lookup_key_destroy(chx_lkey* k);
 int chx_lookup_width(const chx_lkey* k);
+/* Half-width decryption for a lookup key: attach n's factors p, q (half_limbs
+ * limbs each, p q = n, p^2 and q^2 below 2^(16 W)) and the CRT idempotents
+ * ep = q^2 (q^-2 mod p^2), eq = p^2 (p^-2 mod q^2) (W limbs each, W =
+ * chx_lookup_width). From then on chx_lookup_decrypt, _decode_hamming and
+ * _reencrypt compute c^a mod p^2 and mod q^2 and recombine c^a mod n^2 before
+ * the floor division L, so every c in [0, n^2) gives the result of
+ * PaillierLookup.decrypt (lookup.py:257-278), which exponentiates at full
+ * width; the reference has no CRT form. The secret key's phi yields p and q
+ * (p + q = n - phi + 1). Needs a 2048- or 4096-bit width; refused with
+ * CHX_EUNSUPPORTED otherwise, and the key keeps the full-width form. */
+int chx_lookup_key_set_crt(chx_ctx* ctx, chx_lkey* k, const uint32_t* p, const uint32_t* q, int half_limbs,
+                           const uint32_t* ep, const uint32_t* eq);
+/* 1 when chx_lookup_key_set_crt has succeeded on k, else 0 */
+int chx_lookup_key_has_crt(const chx_lkey* k);
 /* c[i] = prod_j g_table[j][digits[i][j]] * prod_t noise[noise_idx[i][t]] mod n^2
  * -- encrypt_kernel (lcu:839-948), encrypt_cts_kernel (lcu:951-1011).
  * digits: count x chunks bytes; noise_idx: count x nmult bytes. */

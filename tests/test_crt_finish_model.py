@@ -61,3 +61,37 @@ def test_crt_finish_model_zero_case_is_minus_one():
     c = p * p  # u_p = 0
     assert finish_model(c, keys, 512) == O.decrypt_crt(c, keys)
     assert finish_model(c, keys, 512, zero_as_minus_one=False) != O.decrypt_crt(c, keys)
+
+
+def lookup_crt_model(c, keys, swap_idempotents=False):
+    """k_lookup_crt_finish's recombination: x = x_p e_p + x_q e_q mod n^2 from
+    x_p = c^a mod p^2, x_q = c^a mod q^2, then L and g_a_inv as lookup.py:257-278."""
+    p, q, n = keys["p"], keys["q"], keys["pk"]["n"]
+    n2, a = n * n, keys["sk"]["a"]
+    p2, q2 = p * p, q * q
+    ep = q2 * pow(q2, -1, p2) % n2
+    eq = p2 * pow(p2, -1, q2) % n2
+    if swap_idempotents:
+        ep, eq = eq, ep
+    xp, xq = pow(c % p2, a, p2), pow(c % q2, a, q2)
+    x = (xp * ep + xq * eq) % n2
+    return O.L(x, n) * keys["sk"]["g_a_inv"] % n
+
+
+@pytest.mark.parametrize("key_len", [512, 1024])
+def test_lookup_crt_model_matches_oracle(key_len):
+    """The half-width lookup decryption equals the oracle's full-width formula
+    for valid ciphertexts and for non-units (multiples of p^2, q^2, p, 0); a
+    planted mistake (idempotents swapped) fails."""
+    keys = O.lookup_key_gen(key_len, 280, 900 + key_len, with_tables=False)
+    n, p, q = keys["pk"]["n"], keys["p"], keys["q"]
+    n2, g, g_n = n * n, keys["pk"]["g"], keys["pk"]["g_n"]
+    assert max(p, q) < 1 << (16 * width_for_bits(n2.bit_length()))  # p^2, q^2 fit the half width
+    rng = random.Random(key_len)
+    ms = [rng.randrange(n) for _ in range(4)]
+    cs = [pow(g, m, n2) * pow(g_n, O.lookup_random_r(n, rng), n2) % n2 for m in ms]
+    cs += [0, 1, p * p, 3 * q * q, 7 * p, n2 - 1, rng.randrange(n2)]
+    want = [O.lookup_decrypt(c, keys) for c in cs]
+    assert want[:4] == ms
+    assert [lookup_crt_model(c, keys) for c in cs] == want
+    assert [lookup_crt_model(c, keys, swap_idempotents=True) for c in cs] != want

@@ -194,6 +194,7 @@ def test_lookup_reencrypt(clients):
     cts = a.encrypt_cts(ms)
     out = a.decrypt_then_reencrypt(cts, ka, kb)
     assert b.decrypt_cts(out) == ms
+    assert a._decrypt_key().crt and b._decrypt_key().crt  # phi gives the factors: half-width decryption
 
 
 def test_readme_pair_distance_256():

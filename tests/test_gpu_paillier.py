@@ -392,3 +392,67 @@ def test_lookup_reencrypt_rejects_narrow_targets(ctx):
     small_noise = LookupKey(ctx, n, 1, 1, [t[i] for i in range(len(t))], ka_o["noise_table"][:16])
     with pytest.raises(ValueError, match="noise index"):
         ka.reencrypt(c, small_noise, np.full((1, 14), 16, dtype=np.uint8))
+
+
+@pytest.mark.parametrize("key_len", [512, 1024])
+def test_lookup_crt_decrypt_matches_full_width(ctx, key_len):
+    """Half-width lookup decryption (c^a mod p^2 and mod q^2, recombined to
+    c^a mod n^2 before L) against the oracle's full-width formula
+    (lookup.py:257-278) and the key's own full-width form: valid ciphertexts
+    and arbitrary c in [0, n^2) -- zero, one, multiples of p^2 and q^2
+    (non-units), all-ones halves, the top of the range -- at every lane count
+    of the half-width exponentiation and of the finish; the fused odd-bit
+    count as well."""
+    from cuhepy_amd.engine import Context, LookupKey
+    from cuhepy_amd.limbs import limbs_to_ints
+    keys = O.lookup_key_gen(key_len, 280, 900 + key_len, with_tables=False)
+    n, n2, p, q = keys["pk"]["n"], keys["pk"]["n_squared"], keys["p"], keys["q"]
+    g, g_n = keys["pk"]["g"], keys["pk"]["g_n"]
+    a, gai = keys["sk"]["a"], keys["sk"]["g_a_inv"]
+    full = LookupKey(ctx, n, a, gai, [], [])
+    crt = LookupKey(ctx, n, a, gai, [], [])
+    assert not crt.crt
+    crt.set_crt(p, q)
+    assert crt.crt and not full.crt
+    W = crt.width
+    half = 16 * W  # bits per half
+    rng = random.Random(key_len)
+    ms = [0, 1, n - 1] + [rng.randrange(n) for _ in range(21)]
+    valid = [pow(g, m, n2) * pow(g_n, O.lookup_random_r(n, rng), n2) % n2 for m in ms]
+    lo = (1 << half) - 1
+    odd = [0, 1, p * p, q * q, 5 * p * p, 3 * q * q, lo, (1 << half) + lo, n2 - 1, n2 - 2, n
+           ] + [rng.randrange(n2) for _ in range(13)]
+    cs = valid + odd
+    assert all(0 <= c < n2 for c in cs)
+    want = [O.lookup_decrypt(c, keys) for c in cs]
+    assert want[:len(ms)] == ms
+    assert limbs_to_ints(full.decrypt(cs)) == want
+    exp_tpis = Context.supported_tpi(W // 2, op=OP_MODEXP)
+    fin_tpis = Context.supported_tpi(W, op=OP_MULMOD)
+    for i in range(max(len(exp_tpis), len(fin_tpis))):
+        ctx.set_tuning(OP_MODEXP, W // 2, exp_tpis[i % len(exp_tpis)], 0)
+        ctx.set_tuning(OP_MULMOD, W, fin_tpis[i % len(fin_tpis)], 0)
+        assert limbs_to_ints(crt.decrypt(cs)) == want, f"variant {i}"
+    mask = sum(1 << b for b in range(1, 32 * W, 2))
+    counts = crt.decode_hamming(cs, mask)
+    assert [int(x) for x in counts] == [bin(m & mask).count("1") for m in want]
+
+
+def test_lookup_crt_refuses_bad_factors(ctx):
+    """set_crt checks p q = n and leaves the key in its full-width form when
+    refused; a second call is refused too."""
+    from cuhepy_amd.engine import LookupKey
+    from cuhepy_amd.limbs import limbs_to_ints
+    keys = O.lookup_key_gen(512, 280, 1905, with_tables=False)
+    n, p, q = keys["pk"]["n"], keys["p"], keys["q"]
+    key = LookupKey(ctx, n, keys["sk"]["a"], keys["sk"]["g_a_inv"], [], [])
+    with pytest.raises(ValueError):
+        key.set_crt(p + 2, q)
+    assert not key.crt
+    c = random.Random(5).randrange(n * n)
+    assert limbs_to_ints(key.decrypt([c])) == [O.lookup_decrypt(c, keys)]
+    key.set_crt(q, p)
+    assert key.crt
+    with pytest.raises(ValueError):
+        key.set_crt(p, q)
+    assert limbs_to_ints(key.decrypt([c])) == [O.lookup_decrypt(c, keys)]
