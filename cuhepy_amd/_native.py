@@ -98,6 +98,8 @@ SIGNATURES = {
                                       POINTER(vp)]),
     "chx_lookup_key_destroy": (c_int, [vp]),
     "chx_lookup_width": (c_int, [vp]),
+    "chx_paillier_key_set_idempotents": (c_int, [vp, vp, u32p, u32p]),
+    "chx_paillier_key_has_idempotents": (c_int, [vp]),
     "chx_lookup_key_set_crt": (c_int, [vp, vp, u32p, u32p, c_int, u32p, u32p]),
     "chx_lookup_key_has_crt": (c_int, [vp]),
     "chx_lookup_encrypt": (c_int, [vp, vp, u8p, u8p, c_int, u32p, c_size_t]),

@@ -164,6 +164,8 @@ struct chx_pkey {
   LazySet lz_n2, lz_p2, lz_q2;
   BN e_n, e_phi, e_pm1, e_qm1;
   std::map<std::pair<int, int>, ExpSched> scheds;  // (which, window) -> schedule
+  BN p2h, q2h;                // p^2, q^2 at H limbs (CRT keys)
+  uint32_t* d_crt = nullptr;  // [e_p R | e_q R] mod n^2 (chx_paillier_key_set_idempotents)
 };
 
 struct chx_lkey {
@@ -670,6 +672,34 @@ void launch_decrypt(chx_ctx* ctx, const LazySet& lz, const ModDev* nm, const uin
   });
   record_stop(ctx);
   check_launch("k_decrypt_finish");
+}
+
+// m = L(u) * cR mod n for u = c^e mod n^2 computed at half width: c^e mod
+// p^2 and mod q^2 in one launch (both halves read the whole ciphertext and
+// reduce it on entry), then k_lookup_crt_finish recombines u exactly with the
+// idempotents eR = [e_p R | e_q R] mod n^2 and finishes as k_decrypt_finish,
+// so the result equals launch_decrypt's for every c in [0, n^2).
+void launch_exact_crt(chx_ctx* ctx, const LazySet& p2, const LazySet& q2, const ExpSched& s, const ModDev* n2m,
+                      const uint32_t* eR, const ModDev* nm, const uint32_t* cR, int L, const uint32_t* c,
+                      uint32_t* out, size_t count, size_t stride, const uint32_t* mask, uint32_t* pop) {
+  const int H = L / 2;
+  const size_t half_words = (size_t)H * stride;
+  ctx->slot[S_MID0].ensure(2 * half_words * sizeof(uint32_t));
+  uint32_t* u = ctx->slot[S_MID0].as<uint32_t>();
+  const LazySet sets[2] = {p2, q2};
+  const ExpSched* scheds[2] = {&s, &s};
+  const uint32_t* ins[2] = {c, c};
+  const uint32_t* his[2] = {c + half_words, c + half_words};
+  uint32_t* outs[2] = {u, u + half_words};
+  run_lazy(ctx, H, sets, scheds, ins, outs, 2, count, stride, his);
+  const int tpi = pick_tpi(ctx, CHX_OP_MULMOD, L, count);
+  record_start(ctx, "decrypt_finish");
+  CHX_DISPATCH(L, tpi, {
+    hipLaunchKernelGGL((k_lookup_crt_finish<L, TPI>), grid_for(count, TPI), dim3(BLOCK_THREADS), lds_bytes(L, TPI),
+                       ctx->stream, n2m, eR, nm, cR, u, out, (int)count, (int)stride, mask, pop);
+  });
+  record_stop(ctx);
+  check_launch("k_lookup_crt_finish");
 }
 
 void launch_encrypt(chx_pkey* k, const uint32_t* m, const uint32_t* r, uint32_t* c, size_t count,
@@ -1567,6 +1597,8 @@ int chx_paillier_key_create(chx_ctx* ctx, const uint32_t* n, int n_limbs, const 
         host::MontConsts cn = host::mont_consts(nH);
         fill_moddev(pd->p2, cp2);
         fill_moddev(pd->q2, cq2);
+        k->p2h = cp2.N;
+        k->q2h = cq2.N;
         k->lz_p2 = make_lazy_set(cp2, ctx->stream);
         k->lz_q2 = make_lazy_set(cq2, ctx->stream);
         fill_moddev(pd->nh, cn);
@@ -1610,6 +1642,7 @@ int chx_paillier_key_destroy(chx_pkey* k) {
   int rc = guarded([&] {
     hip_check(hipSetDevice(k->ctx->device), "hipSetDevice");
     if (k->d) (void)hipFree(k->d);
+    if (k->d_crt) (void)hipFree(k->d_crt);
     free_lazy_set(k->lz_n2);
     free_lazy_set(k->lz_p2);
     free_lazy_set(k->lz_q2);
@@ -1621,6 +1654,36 @@ int chx_paillier_key_destroy(chx_pkey* k) {
 }
 
 int chx_paillier_width(const chx_pkey* k) { return k ? k->W : 0; }
+
+// Checked CRT idempotents -> [e_p R | e_q R] mod n^2 in device memory
+static uint32_t* upload_idempotents(chx_ctx* ctx, const host::MontConsts& n2, const BN& p2, const BN& q2,
+                                    const uint32_t* ep, const uint32_t* eq) {
+  const int W = n2.L, H = (int)p2.size();
+  BN e1 = host::bn_from(ep, W, W), e2 = host::bn_from(eq, W, W);
+  if (host::bn_cmp(e1, n2.N) >= 0 || host::bn_cmp(e2, n2.N) >= 0) fail(CHX_EINVAL, "CRT idempotents must be below n^2");
+  BN one(H, 0u), zero(H, 0u);
+  one[0] = 1;
+  if (host::bn_cmp(host::bn_mod(e1, p2), one) != 0 || host::bn_cmp(host::bn_mod(e1, q2), zero) != 0 ||
+      host::bn_cmp(host::bn_mod(e2, q2), one) != 0 || host::bn_cmp(host::bn_mod(e2, p2), zero) != 0)
+    fail(CHX_EINVAL, "e_p must be 1 mod p^2 and 0 mod q^2, e_q the converse");
+  std::vector<uint32_t> er(2 * (size_t)W);
+  BN t1 = host::to_mont_host(e1, n2), t2 = host::to_mont_host(e2, n2);
+  std::copy(t1.begin(), t1.end(), er.begin());
+  std::copy(t2.begin(), t2.end(), er.begin() + W);
+  return dev_upload(er.data(), 2 * W, ctx->stream);
+}
+
+int chx_paillier_key_set_idempotents(chx_ctx* ctx, chx_pkey* k, const uint32_t* ep, const uint32_t* eq) {
+  return with_ctx(ctx, [&] {
+    if (!k || !ep || !eq) fail(CHX_EINVAL, "null argument");
+    if (k->ctx != ctx) fail(CHX_EINVAL, "key belongs to another context");
+    if (!k->crt) fail(CHX_EINVAL, "key was created without CRT factors");
+    if (k->d_crt) fail(CHX_EINVAL, "key already has CRT idempotents");
+    k->d_crt = upload_idempotents(ctx, k->n2, k->p2h, k->q2h, ep, eq);
+  });
+}
+
+int chx_paillier_key_has_idempotents(const chx_pkey* k) { return k && k->d_crt ? 1 : 0; }
 
 int chx_paillier_encrypt(chx_ctx* ctx, const chx_pkey* kc, const uint32_t* m, const uint32_t* r, uint32_t* c,
                          size_t count) {
@@ -1656,8 +1719,14 @@ static void paillier_decrypt_impl(chx_pkey* k, const uint32_t* dc, uint32_t* dm,
                                   const uint32_t* mask, uint32_t* pop) {
   chx_ctx* ctx = k->ctx;
   const int L = k->W;
-  const ExpSched& s = get_sched_p(k, 1, pick_window(ctx, CHX_OP_MODEXP, L, exp_bits_p(k, 1)));
   PaillierDev* d = k->d;
+  if (k->d_crt) {  // c^phi at half width, recombined exactly
+    const ExpSched& s = get_sched_p(k, 1, pick_window(ctx, CHX_OP_MODEXP, L / 2, exp_bits_p(k, 1)));
+    launch_exact_crt(ctx, k->lz_p2, k->lz_q2, s, &d->n2, k->d_crt, &d->n, d->muR.v, L, dc, dm, count, stride, mask,
+                     pop);
+    return;
+  }
+  const ExpSched& s = get_sched_p(k, 1, pick_window(ctx, CHX_OP_MODEXP, L, exp_bits_p(k, 1)));
   launch_decrypt(ctx, k->lz_n2, &d->n, d->muR.v, L, dc, dm, count, stride, s, mask, pop);
 }
 
@@ -1828,22 +1897,11 @@ int chx_lookup_key_set_crt(chx_ctx* ctx, chx_lkey* k, const uint32_t* p, const u
     }
     if (host::bn_cmp(pq, k->n.N) != 0) fail(CHX_EINVAL, "p * q != n");
     BN p2 = host::bn_sqr_lo(pb), q2 = host::bn_sqr_lo(qb);
-    BN e1 = host::bn_from(ep, W, W), e2 = host::bn_from(eq, W, W);
-    if (host::bn_cmp(e1, k->n2.N) >= 0 || host::bn_cmp(e2, k->n2.N) >= 0)
-      fail(CHX_EINVAL, "CRT idempotents must be below n^2");
-    BN one(H, 0u), zero(H, 0u);
-    one[0] = 1;
-    if (host::bn_cmp(host::bn_mod(e1, p2), one) != 0 || host::bn_cmp(host::bn_mod(e1, q2), zero) != 0 ||
-        host::bn_cmp(host::bn_mod(e2, q2), one) != 0 || host::bn_cmp(host::bn_mod(e2, p2), zero) != 0)
-      fail(CHX_EINVAL, "e_p must be 1 mod p^2 and 0 mod q^2, e_q the converse");
-    std::vector<uint32_t> er(2 * (size_t)W);
-    BN t1 = host::to_mont_host(e1, k->n2), t2 = host::to_mont_host(e2, k->n2);
-    std::copy(t1.begin(), t1.end(), er.begin());
-    std::copy(t2.begin(), t2.end(), er.begin() + W);
+    uint32_t* er = upload_idempotents(ctx, k->n2, p2, q2, ep, eq);
+    k->d_crt = er;
     try {  // a failure below leaves the key without CRT factors
       k->lz_p2 = make_lazy_set(host::mont_consts(p2), ctx->stream);
       k->lz_q2 = make_lazy_set(host::mont_consts(q2), ctx->stream);
-      k->d_crt = dev_upload(er.data(), 2 * W, ctx->stream);
     } catch (...) {
       free_lazy_set(k->lz_p2);
       free_lazy_set(k->lz_q2);
@@ -1904,39 +1962,14 @@ int chx_lookup_encrypt_dev(chx_ctx* ctx, const chx_lkey* kc, const uint8_t* digi
   });
 }
 
-// Half-width form: c^a mod p^2 and mod q^2 in one launch (both halves read the
-// whole ciphertext and reduce it on entry), then k_lookup_crt_finish
-// recombines c^a mod n^2 and applies L and g_a_inv as the full-width finish.
-static void launch_lookup_crt(chx_lkey* k, const uint32_t* dc, uint32_t* dm, size_t count, size_t stride,
-                              const uint32_t* mask, uint32_t* pop) {
-  chx_ctx* ctx = k->ctx;
-  const int L = k->W, H = L / 2;
-  const ExpSched& s = get_sched_l(k, pick_window(ctx, CHX_OP_MODEXP, H, host::bn_bits(k->e_a)));
-  const size_t half_words = (size_t)H * stride;
-  ctx->slot[S_MID0].ensure(2 * half_words * sizeof(uint32_t));
-  uint32_t* u = ctx->slot[S_MID0].as<uint32_t>();
-  const LazySet sets[2] = {k->lz_p2, k->lz_q2};
-  const ExpSched* scheds[2] = {&s, &s};
-  const uint32_t* ins[2] = {dc, dc};
-  const uint32_t* his[2] = {dc + half_words, dc + half_words};
-  uint32_t* outs[2] = {u, u + half_words};
-  run_lazy(ctx, H, sets, scheds, ins, outs, 2, count, stride, his);
-  const int tpi = pick_tpi(ctx, CHX_OP_MULMOD, L, count);
-  record_start(ctx, "decrypt_finish");
-  CHX_DISPATCH(L, tpi, {
-    hipLaunchKernelGGL((k_lookup_crt_finish<L, TPI>), grid_for(count, TPI), dim3(BLOCK_THREADS), lds_bytes(L, TPI),
-                       ctx->stream, k->d_n2, k->d_crt, k->d_n, k->d_gar, u, dm, (int)count, (int)stride, mask, pop);
-  });
-  record_stop(ctx);
-  check_launch("k_lookup_crt_finish");
-}
-
 static void lookup_decrypt_impl(chx_lkey* k, const uint32_t* dc, uint32_t* dm, size_t count, size_t stride,
                                 const uint32_t* mask, uint32_t* pop) {
   chx_ctx* ctx = k->ctx;
   const int L = k->W;
   if (k->crt) {
-    launch_lookup_crt(k, dc, dm, count, stride, mask, pop);
+    const ExpSched& s = get_sched_l(k, pick_window(ctx, CHX_OP_MODEXP, L / 2, host::bn_bits(k->e_a)));
+    launch_exact_crt(ctx, k->lz_p2, k->lz_q2, s, k->d_n2, k->d_crt, k->d_n, k->d_gar, L, dc, dm, count, stride, mask,
+                     pop);
     return;
   }
   const ExpSched& s = get_sched_l(k, pick_window(ctx, CHX_OP_MODEXP, L, host::bn_bits(k->e_a)));

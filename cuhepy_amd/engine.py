@@ -268,6 +268,15 @@ def crt_constants(p: int, q: int) -> dict:
     return {"hp": pow(-q % p, -1, p), "hq": pow(-p % q, -1, q), "qinv": pow(q, -1, p)}
 
 
+def crt_idempotents(p: int, q: int) -> tuple[int, int]:
+    """e_p = q^2 (q^-2 mod p^2) and e_q = p^2 (p^-2 mod q^2): e_p is 1 mod p^2 and
+    0 mod q^2, e_q the converse, so x_p e_p + x_q e_q mod n^2 is the x with
+    those residues."""
+    p2, q2 = p * p, q * q
+    n2 = p2 * q2
+    return q2 * pow(q2, -1, p2) % n2, p2 * pow(p2, -1, q2) % n2
+
+
 class PaillierKey:
     """Device-resident Paillier key (g = n + 1): encrypt, decrypt, CRT decrypt."""
 
@@ -301,6 +310,14 @@ class PaillierKey:
                                                   None, None, None, ctypes.byref(h)))
         self.handle = h
         assert N.lib.chx_paillier_width(h) == self.width
+        if self.crt:  # exact half-width decrypt / decode (same results as the full-width form)
+            E = ints_to_limbs(list(crt_idempotents(p, q)), self.width)
+            N.check(N.lib.chx_paillier_key_set_idempotents(ctx.handle, h, _p32(E[0:1]), _p32(E[1:2])))
+
+    @property
+    def exact_crt(self) -> bool:
+        """decrypt / decode_hamming run c^phi at half width (recombined exactly)."""
+        return bool(N.lib.chx_paillier_key_has_idempotents(self.handle))
 
     def __del__(self):  # pragma: no cover
         try:
@@ -409,9 +426,7 @@ class LookupKey:
         Raises (key unchanged) when p q != n or the width has no CRT form."""
         if p * q != self.n:
             raise ValueError("p * q != n")
-        p2, q2 = p * p, q * q
-        ep = q2 * pow(q2, -1, p2) % self.n2
-        eq = p2 * pow(p2, -1, q2) % self.n2
+        ep, eq = crt_idempotents(p, q)
         hl = max(limb_count(p), limb_count(q))
         P, Q = ints_to_limbs([p], hl), ints_to_limbs([q], hl)
         E = ints_to_limbs([ep, eq], self.width)

@@ -174,6 +174,20 @@ int chx_paillier_key_create(chx_ctx* ctx, const uint32_t* n, int n_limbs, const 
                             const uint32_t* hp, const uint32_t* hq, const uint32_t* qinv, chx_pkey** out);
 int chx_paillier_key_destroy(chx_pkey* k);
 int chx_paillier_width(const chx_pkey* k); /* limbs of n^2 */
+/* Exact half-width decryption for a key created with p and q: give it the CRT
+ * idempotents ep = q^2 (q^-2 mod p^2), eq = p^2 (p^-2 mod q^2) (W limbs each,
+ * W = chx_paillier_width). From then on chx_paillier_decrypt(_dev) and
+ * chx_paillier_decode_hamming compute c^phi mod p^2 and mod q^2 at half width
+ * and recombine c^phi mod n^2 before L, so every c in [0, n^2) gives the result
+ * of Paillier.decrypt (scheme.py:86-104) at half the multiply-adds of the
+ * full-width form (the reference's decrypt_kernel, cu:549-587, works at full
+ * width). Unlike chx_paillier_decrypt_crt, which follows the textbook CRT
+ * formula and agrees with it on valid ciphertexts only, this form agrees on
+ * non-units too. Refused (CHX_EINVAL) without CRT factors or with wrong
+ * idempotents; the key then keeps the full-width form. */
+int chx_paillier_key_set_idempotents(chx_ctx* ctx, chx_pkey* k, const uint32_t* ep, const uint32_t* eq);
+/* 1 when chx_paillier_key_set_idempotents has succeeded on k, else 0 */
+int chx_paillier_key_has_idempotents(const chx_pkey* k);
 
 /* c[i] = (1 + m[i]*n) * r[i]^n mod n^2; m < n, 0 < r < n, gcd(r, n) = 1 is the
  * caller's draw -- encrypt_kernel (cu:458-546) / Paillier.encrypt (scheme.py:63-83).

@@ -456,3 +456,38 @@ def test_lookup_crt_refuses_bad_factors(ctx):
     with pytest.raises(ValueError):
         key.set_crt(p, q)
     assert limbs_to_ints(key.decrypt([c])) == [O.lookup_decrypt(c, keys)]
+
+
+@pytest.mark.parametrize("key_len", [512, 1024])
+def test_paillier_exact_crt_decrypt_matches_full_width(ctx, key_len):
+    """A key with p and q decrypts and decodes at half width (c^phi mod p^2
+    and mod q^2, recombined to c^phi mod n^2 before L): equal to the
+    oracle's scheme.py:86-104 formula and to a key without factors for
+    valid ciphertexts and for non-units (multiples of p, p^2, q^2, zero),
+    where the textbook CRT formula (decrypt_crt) is not bound to agree; at
+    every lane count of the half-width exponentiation and of the finish."""
+    from cuhepy_amd.engine import Context, PaillierKey
+    from cuhepy_amd.limbs import limbs_to_ints
+    keys = O.key_gen(key_len, 700 + key_len)
+    n, p, q = keys["pk"]["n"], keys["p"], keys["q"]
+    phi, inv = keys["sk"]["phi"], keys["sk"]["inv"]
+    crt = PaillierKey(ctx, n, phi, inv, p, q)
+    full = PaillierKey(ctx, n, phi, inv)
+    assert crt.exact_crt and not full.exact_crt
+    n2, W = n * n, crt.width
+    rng = random.Random(key_len + 3)
+    ms = [0, 1, n - 1] + [rng.randrange(n) for _ in range(13)]
+    cs = [O.encrypt(m, keys["pk"], O.random_r(n, rng)) for m in ms]
+    lo = (1 << (16 * W)) - 1
+    cs += [0, 1, p, 3 * q, p * p, q * q, 11 * p * p, lo, n2 - 1, n] + [rng.randrange(n2) for _ in range(6)]
+    want = [O.decrypt(c, keys) for c in cs]
+    assert want[:len(ms)] == ms
+    assert limbs_to_ints(full.decrypt(cs)) == want
+    exp_tpis = Context.supported_tpi(W // 2, op=OP_MODEXP)
+    fin_tpis = Context.supported_tpi(W, op=OP_MULMOD)
+    for i in range(max(len(exp_tpis), len(fin_tpis))):
+        ctx.set_tuning(OP_MODEXP, W // 2, exp_tpis[i % len(exp_tpis)], 0)
+        ctx.set_tuning(OP_MULMOD, W, fin_tpis[i % len(fin_tpis)], 0)
+        assert limbs_to_ints(crt.decrypt(cs)) == want, f"variant {i}"
+    mask = sum(1 << b for b in range(1, 32 * W, 2))
+    assert [int(x) for x in crt.decode_hamming(cs, mask)] == [bin(m & mask).count("1") for m in want]
