@@ -706,6 +706,28 @@ void launch_encrypt(chx_pkey* k, const uint32_t* m, const uint32_t* r, uint32_t*
                     size_t stride) {
   chx_ctx* ctx = k->ctx;
   const int L = k->W;
+  if (k->d_crt) {  // r^n mod p^2 and mod q^2 at half width, recombined in the finish
+    const int H = L / 2;
+    const ExpSched& s = get_sched_p(k, 0, pick_window(ctx, CHX_OP_MODEXP, H, exp_bits_p(k, 0)));
+    const size_t half_words = (size_t)H * stride;
+    ctx->slot[S_MID0].ensure(2 * half_words * sizeof(uint32_t));
+    uint32_t* u = ctx->slot[S_MID0].as<uint32_t>();
+    const LazySet sets[2] = {k->lz_p2, k->lz_q2};
+    const ExpSched* scheds[2] = {&s, &s};
+    const uint32_t* ins[2] = {r, r};
+    const uint32_t* his[2] = {r + half_words, r + half_words};
+    uint32_t* outs[2] = {u, u + half_words};
+    run_lazy(ctx, H, sets, scheds, ins, outs, 2, count, stride, his);
+    const int tpi = pick_tpi(ctx, CHX_OP_MULMOD, L, count);
+    record_start(ctx, "encrypt_finish");
+    CHX_DISPATCH(L, tpi, {
+      hipLaunchKernelGGL((k_encrypt_crt_finish<L, TPI>), grid_for(count, TPI), dim3(BLOCK_THREADS),
+                         lds_bytes(L, TPI), ctx->stream, k->d, k->d_crt, m, u, c, (int)count, (int)stride);
+    });
+    record_stop(ctx);
+    check_launch("k_encrypt_crt_finish");
+    return;
+  }
   const ExpSched& s = get_sched_p(k, 0, pick_window(ctx, CHX_OP_MODEXP, L, exp_bits_p(k, 0)));
   ctx->slot[S_MID0].ensure(stride * L * sizeof(uint32_t));
   uint32_t* y = ctx->slot[S_MID0].as<uint32_t>();

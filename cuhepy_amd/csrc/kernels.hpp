@@ -287,6 +287,30 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_decrypt_finish(const ModDev* 
   }
 }
 
+// x = x_p e_p + x_q e_q mod n^2 (canonical) from the half-width residues in
+// U (x_p in rows [0, H), x_q in rows [H, 2H), H = L/2) and eR = [e_p R |
+// e_q R] mod n^2, e_p = 1 mod p^2 and 0 mod q^2, e_q the converse: the
+// unique x below n^2 with those residues (CRT).
+template <int L, int TPI>
+__device__ __forceinline__ void crt_recombine(Mont<L, TPI>& M2, const u32* __restrict__ eR,
+                                              const u32* __restrict__ U, int it, int stride, u32 (&x)[L / TPI]) {
+  constexpr int K = L / TPI, H = L / 2;
+  const int r0 = Grp<TPI>::rank() * K;
+  u32 xp[K], xq[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {  // half-width residues zero-extended to L limbs
+    const int l = r0 + k;
+    xp[k] = l < H ? U[limb_index<L>(l, it, stride)] : 0u;
+    xq[k] = l < H ? U[limb_index<L>(H + l, it, stride)] : 0u;
+  }
+  u32 e[K], y[K];
+  load_const<L, TPI>(e, eR);
+  M2.mul(x, xp, e);  // x_p e_p mod n^2 (x_p < R: the product is canonical)
+  load_const<L, TPI>(e, eR + L);
+  M2.mul(y, xq, e);  // x_q e_q mod n^2
+  add_mod<K, TPI>(x, y, M2.N);
+}
+
 // Lookup decryption from half-width residues (the CRT form of
 // paillier/lookup.py:257-278). U holds x_p = c^a mod p^2 in rows [0, H) and
 // x_q = c^a mod q^2 in rows [H, 2H), H = L/2. With the idempotents
@@ -304,25 +328,13 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lookup_crt_finish(const ModDe
                                                                      const u32* __restrict__ mask,
                                                                      u32* __restrict__ popcnt) {
   extern __shared__ uint4 smem[];
-  constexpr int K = L / TPI, H = L / 2;
+  constexpr int K = L / TPI;
   const int item = group_item<TPI>();
   const int it = item < count ? item : count - 1;
-  const int r0 = Grp<TPI>::rank() * K;
-  u32 xp[K], xq[K];
-#pragma unroll
-  for (int k = 0; k < K; ++k) {  // half-width residues zero-extended to L limbs
-    const int l = r0 + k;
-    xp[k] = l < H ? U[limb_index<L>(l, it, stride)] : 0u;
-    xq[k] = l < H ? U[limb_index<L>(H + l, it, stride)] : 0u;
-  }
   Mont<L, TPI> M2;
   mont_init<L, TPI>(M2, n2m, smem);
-  u32 e[K], x[K], y[K];
-  load_const<L, TPI>(e, eR);
-  M2.mul(x, xp, e);  // x_p e_p mod n^2 (x_p < R: the product is canonical)
-  load_const<L, TPI>(e, eR + L);
-  M2.mul(y, xq, e);  // x_q e_q mod n^2
-  add_mod<K, TPI>(x, y, M2.N);
+  u32 x[K];
+  crt_recombine<L, TPI>(M2, eR, U, it, stride, x);
   Mont<L, TPI> Mn;
   mont_init<L, TPI>(Mn, nm, smem);
   paillier_finish<L, TPI>(Mn, nm, x, cR);
@@ -348,6 +360,32 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_encrypt_finish(const Paillier
   u32 a[K], y[K], c[K];
   load_dist<L, TPI>(a, Mv, it, stride);
   load_dist<L, TPI>(y, Y, it, stride);
+  load_const<L, TPI>(c, key->nR.v);
+  M.mul_into(a, c);                  // m*n (< n^2, exact)
+  (void)add_small<K, TPI>(a, 1u);    // 1 + m*n
+  load_const<L, TPI>(c, key->n2.R2);
+  M.mul_into(y, c);                  // y*R
+  M.mul_into(a, y);                  // (1 + m n) * y
+  if (item < count) store_dist<L, TPI>(out, a, item, stride);
+}
+
+// k_encrypt_finish from half-width residues of y = r^n (U as crt_recombine):
+// c = (1 + m*n) * y mod n^2 with y recombined modulo n^2 first.
+template <int L, int TPI>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_encrypt_crt_finish(const PaillierDev* __restrict__ key,
+                                                                      const u32* __restrict__ eR,
+                                                                      const u32* __restrict__ Mv,
+                                                                      const u32* __restrict__ U, u32* __restrict__ out,
+                                                                      int count, int stride) {
+  extern __shared__ uint4 smem[];
+  constexpr int K = L / TPI;
+  Mont<L, TPI> M;
+  mont_init<L, TPI>(M, &key->n2, smem);
+  const int item = group_item<TPI>();
+  const int it = item < count ? item : count - 1;
+  u32 a[K], y[K], c[K];
+  crt_recombine<L, TPI>(M, eR, U, it, stride, y);
+  load_dist<L, TPI>(a, Mv, it, stride);
   load_const<L, TPI>(c, key->nR.v);
   M.mul_into(a, c);                  // m*n (< n^2, exact)
   (void)add_small<K, TPI>(a, 1u);    // 1 + m*n

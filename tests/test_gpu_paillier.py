@@ -1,6 +1,7 @@
 """GPU parity of Paillier encrypt / decrypt / CRT decrypt / decode and the
 lookup scheme against the oracle, on the golden vectors and seeded batches."""
 
+import math
 import random
 
 import numpy as np
@@ -460,7 +461,7 @@ def test_lookup_crt_refuses_bad_factors(ctx):
 
 @pytest.mark.parametrize("key_len", [512, 1024])
 def test_paillier_exact_crt_decrypt_matches_full_width(ctx, key_len):
-    """A key with p and q decrypts and decodes at half width (c^phi mod p^2
+    """A key with p and q encrypts, decrypts and decodes at half width (c^phi mod p^2
     and mod q^2, recombined to c^phi mod n^2 before L): equal to the
     oracle's scheme.py:86-104 formula and to a key without factors for
     valid ciphertexts and for non-units (multiples of p, p^2, q^2, zero),
@@ -491,3 +492,15 @@ def test_paillier_exact_crt_decrypt_matches_full_width(ctx, key_len):
         assert limbs_to_ints(crt.decrypt(cs)) == want, f"variant {i}"
     mask = sum(1 << b for b in range(1, 32 * W, 2))
     assert [int(x) for x in crt.decode_hamming(cs, mask)] == [bin(m & mask).count("1") for m in want]
+    # encryption: r^n mod p^2 and mod q^2 at half width, recombined; r = 1,
+    # n - 1, non-units (multiples of p and q), random
+    rs = [1, n - 1, p, 2 * q, 3 * p] + [O.random_r(n, rng) for _ in range(len(ms) - 5)]
+    g = keys["pk"]["g"]
+    # the oracle refuses non-unit r (scheme.py draws units); for those the closed form g^m r^n mod n^2
+    enc_want = [O.encrypt(m, keys["pk"], r) if math.gcd(r, n) == 1 else pow(g, m, n2) * pow(r, n, n2) % n2
+                for m, r in zip(ms, rs)]
+    assert limbs_to_ints(full.encrypt(ms, rs)) == enc_want
+    for i in range(max(len(exp_tpis), len(fin_tpis))):
+        ctx.set_tuning(OP_MODEXP, W // 2, exp_tpis[i % len(exp_tpis)], 0)
+        ctx.set_tuning(OP_MULMOD, W, fin_tpis[i % len(fin_tpis)], 0)
+        assert limbs_to_ints(crt.encrypt(ms, rs)) == enc_want, f"encrypt variant {i}"
