@@ -181,7 +181,8 @@ int chx_paillier_width(const chx_pkey* k); /* limbs of n^2 */
  * and recombine c^phi mod n^2 before L, so every c in [0, n^2) gives the result
  * of Paillier.decrypt (scheme.py:86-104) at half the multiply-adds of the
  * full-width form (the reference's decrypt_kernel, cu:549-587, works at full
- * width). Unlike chx_paillier_decrypt_crt, which follows the textbook CRT
+ * width); chx_paillier_encrypt(_dev) likewise computes r^n mod p^2 and mod q^2
+ * and recombines r^n mod n^2 (scheme.py:63-83, encrypt_kernel cu:458-546). Unlike chx_paillier_decrypt_crt, which follows the textbook CRT
  * formula and agrees with it on valid ciphertexts only, this form agrees on
  * non-units too. Refused (CHX_EINVAL) without CRT factors or with wrong
  * idempotents; the key then keeps the full-width form. */
