@@ -757,6 +757,36 @@ def main() -> int:
         print(json.dumps({"error": f"{mism} CRT decryptions differ from the seeded plaintexts"}))
         return 1
 
+    # exact half-width encryption and decryption of this key (r^n and c^phi
+    # mod p^2 and mod q^2, recombined modulo n^2) beside the same key without
+    # factors (full width): kernel ms per batch, outputs compared in full
+    full_key = PaillierKey(ctx, n, phi, inv)
+    exact = {"items": count, "n_bits": n.bit_length()}
+    outs = {}
+    for form, kk in (("half_width", key), ("full_width", full_key)):
+        e_il = torch.zeros((W, stride), dtype=torch.int32, device=dev)
+        d_il = torch.zeros((W, stride), dtype=torch.int32, device=dev)
+        for what, run, fin in (("encrypt", lambda: kk.encrypt_dev(m_il, r_il, e_il, count, stride), "encrypt_finish"),
+                               ("decrypt", lambda: kk.decrypt_dev(c_il, d_il, count, stride), "decrypt_finish")):
+            torch.cuda.synchronize()
+            ctx.reset_timing()
+            ctx.set_timing(True)
+            run()
+            torch.cuda.synchronize()
+            exact[f"{what}_{form}_ms"] = round(ctx.kernel_time("lazy_modexp")[0] + ctx.kernel_time(fin)[0], 3)
+            ctx.set_timing(False)
+        outs[form] = (from_il(e_il, W), from_il(d_il, W))
+    same = all(np.array_equal(a, b) for a, b in zip(outs["half_width"], outs["full_width"]))
+    dec_ok = bool(np.array_equal(outs["half_width"][1], m_aos))
+    log(f"exact half width vs full width, {count} items: encrypt {exact['encrypt_half_width_ms']:.2f} vs "
+        f"{exact['encrypt_full_width_ms']:.2f} ms, decrypt {exact['decrypt_half_width_ms']:.2f} vs "
+        f"{exact['decrypt_full_width_ms']:.2f} ms; outputs equal: {same}; decryptions = plaintexts: {dec_ok}")
+    if not (same and dec_ok):
+        print(json.dumps({"error": "half-width and full-width encryption or decryption differ"}))
+        return 1
+    exact["bit_exact"] = {"half_equals_full_items": count, "decrypt_equals_plaintexts": count, "mismatches": 0}
+    del full_key
+
     # roofline of the dominant kernel
     ebits = max((p - 1).bit_length(), (q - 1).bit_length())
     macs_per_modexp = (ebits + math.ceil(ebits / 6)) * 2 * H * H
@@ -850,6 +880,7 @@ def main() -> int:
         "plain_per_call_modexp_s": round(plain, 1) if plain else None,
         "batched_over_plain": round(value / plain, 3) if plain else None,
         "decrypts_per_s": round(value / 2, 1),
+        "exact_encrypt_decrypt": exact,
         "bit_exact": True,
     }
     if cpu:
