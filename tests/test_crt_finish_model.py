@@ -95,3 +95,31 @@ def test_lookup_crt_model_matches_oracle(key_len):
     assert want[:4] == ms
     assert [lookup_crt_model(c, keys) for c in cs] == want
     assert [lookup_crt_model(c, keys, swap_idempotents=True) for c in cs] != want
+
+
+@pytest.mark.parametrize("key_len", [512, 1024])
+def test_paillier_exact_crt_model_matches_oracle(key_len):
+    """k_encrypt_crt_finish / k_lookup_crt_finish for plain keys: r^n and
+    c^phi recombined from their residues mod p^2 and q^2 equal the full-width
+    powers for units and non-units, so encryption equals g^m r^n mod n^2 and
+    decryption equals scheme.py:86-104's formula (the oracle) for every c; a
+    planted mistake (x_q taken as x_p) fails."""
+    from cuhepy_amd.engine import crt_idempotents
+    keys = O.key_gen(key_len, 700 + key_len)
+    n, p, q = keys["pk"]["n"], keys["p"], keys["q"]
+    n2, phi, inv = n * n, keys["sk"]["phi"], keys["sk"]["inv"]
+    p2, q2 = p * p, q * q
+    ep, eq = crt_idempotents(p, q)
+
+    def power(x, e, planted=False):
+        xp, xq = pow(x % p2, e, p2), pow(x % q2, e, q2)
+        return (xp * ep + (xp if planted else xq) * eq) % n2
+
+    rng = random.Random(key_len)
+    xs = [0, 1, p, 3 * q, p2, q2, n, n2 - 1] + [rng.randrange(n2) for _ in range(6)]
+    for x in xs:
+        assert power(x, n) == pow(x, n, n2)
+        assert (power(x, phi) - 1) // n * inv % n == O.decrypt(x, keys)
+    assert any(power(x, phi, planted=True) != pow(x, phi, n2) for x in xs)
+    m, r = rng.randrange(n), O.random_r(n, rng)
+    assert (1 + m * n) * power(r, n) % n2 == O.encrypt(m, keys["pk"], r)
