@@ -129,7 +129,7 @@ struct chx_ctx {
   bool timing = false;
   std::vector<TimedLaunch> pending;
   std::map<std::string, std::pair<double, int>> stats;
-  uint32_t packed_token = 0;  // per-call value of k_il_digits's flag (never 0)
+  uint32_t packed_token = 0;  // per-call value of k_il_digits's and k_crt_finish's flags (never 0)
 };
 
 // Lazy-radix constants of one modulus, one block per built lazy configuration
@@ -743,7 +743,9 @@ void launch_encrypt(chx_pkey* k, const uint32_t* m, const uint32_t* r, uint32_t*
   check_launch("k_encrypt_finish");
 }
 
-void launch_crt(chx_pkey* k, const uint32_t* c, uint32_t* m, size_t count, size_t stride) {
+void launch_crt(chx_pkey* k, const uint32_t* c, uint32_t* m, size_t count, size_t stride,
+                uint32_t* flag = nullptr, uint32_t token = 0, const uint32_t* mask = nullptr,
+                uint32_t* pop = nullptr) {
   chx_ctx* ctx = k->ctx;
   if (!k->crt) fail(CHX_EINVAL, "key was created without CRT factors");
   const int H = k->H;
@@ -766,7 +768,7 @@ void launch_crt(chx_pkey* k, const uint32_t* c, uint32_t* m, size_t count, size_
   record_start(ctx, "crt_finish");
   CHX_DISPATCH_H(H, tpi, {
     hipLaunchKernelGGL((k_crt_finish<H, TPI>), grid_for(count, TPI), dim3(BLOCK_THREADS), lds_bytes(H, TPI),
-                       ctx->stream, k->d, u, m, (int)count, (int)stride);
+                       ctx->stream, k->d, u, m, (int)count, (int)stride, flag, token, mask, pop);
   });
   record_stop(ctx);
   check_launch("k_crt_finish");
@@ -1752,6 +1754,35 @@ static void paillier_decrypt_impl(chx_pkey* k, const uint32_t* dc, uint32_t* dm,
   launch_decrypt(ctx, k->lz_n2, &d->n, d->muR.v, L, dc, dm, count, stride, s, mask, pop);
 }
 
+// Host entry points of a key with CRT factors: the textbook CRT decryption
+// (launch_crt, two half-width exponents of |p| and |q| bits) equals
+// scheme.py:86-104 for every unit c, since every unit of Z_(n^2)^* is an
+// encryption (1 + n)^m r^n. Its finish raises a per-call flag for any
+// non-unit (p | c or q | c); only then is the batch decrypted again by
+// paillier_decrypt_impl, whose results equal the formula for every c. A stale
+// flag word can only cause that second pass, never a different result.
+// Returns false (nothing launched) for a key without CRT factors.
+static bool decrypt_units_first(chx_pkey* k, const uint32_t* dc, uint32_t* dm, size_t count, size_t stride,
+                                const uint32_t* mask, uint32_t* pop) {
+  if (!k->crt) return false;
+  chx_ctx* ctx = k->ctx;
+  ctx->slot[S_AUX].ensure(16);
+  uint32_t* flag = ctx->slot[S_AUX].as<uint32_t>();
+  uint32_t token = ++ctx->packed_token;
+  if (token == 0) token = ctx->packed_token = 1;
+  launch_crt(k, dc, dm, count, stride, flag, token, mask, pop);  // dm: rows [0, H)
+  uint32_t h = 0;
+  hip_check(hipMemcpyAsync(&h, flag, 4, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+  hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  if (h == token) {
+    paillier_decrypt_impl(k, dc, dm, count, stride, mask, pop);
+  } else if (dm) {  // m < n: the upper half of the W-limb rows is zero
+    hip_check(hipMemsetAsync(dm + (size_t)k->H * stride, 0, (size_t)k->H * stride * sizeof(uint32_t), ctx->stream),
+              "hipMemsetAsync");
+  }
+  return true;
+}
+
 int chx_paillier_decrypt(chx_ctx* ctx, const chx_pkey* kc, const uint32_t* c, uint32_t* m, size_t count) {
   chx_pkey* k = const_cast<chx_pkey*>(kc);
   return with_ctx(ctx, [&] {
@@ -1762,7 +1793,8 @@ int chx_paillier_decrypt(chx_ctx* ctx, const chx_pkey* kc, const uint32_t* c, ui
     const size_t stride = batch_stride(count);
     uint32_t* dc = upload_il(ctx, S_IN0, c, count, L, stride);
     ctx->slot[S_OUT].ensure(stride * L * 4);
-    paillier_decrypt_impl(k, dc, ctx->slot[S_OUT].as<uint32_t>(), count, stride, nullptr, nullptr);
+    if (!decrypt_units_first(k, dc, ctx->slot[S_OUT].as<uint32_t>(), count, stride, nullptr, nullptr))
+      paillier_decrypt_impl(k, dc, ctx->slot[S_OUT].as<uint32_t>(), count, stride, nullptr, nullptr);
     download_il(ctx, ctx->slot[S_OUT].as<uint32_t>(), m, count, L, stride);
   });
 }
@@ -1817,8 +1849,10 @@ int chx_paillier_decode_hamming(chx_ctx* ctx, const chx_pkey* kc, const uint32_t
     ctx->slot[S_IN1].ensure(L * 4);
     hip_check(hipMemcpyAsync(ctx->slot[S_IN1].p, mask, L * 4, hipMemcpyHostToDevice, ctx->stream), "H2D");
     ctx->slot[S_OUT].ensure(count * 4);
-    paillier_decrypt_impl(k, dc, nullptr, count, stride, ctx->slot[S_IN1].as<uint32_t>(),
-                          ctx->slot[S_OUT].as<uint32_t>());
+    if (!decrypt_units_first(k, dc, nullptr, count, stride, ctx->slot[S_IN1].as<uint32_t>(),
+                             ctx->slot[S_OUT].as<uint32_t>()))
+      paillier_decrypt_impl(k, dc, nullptr, count, stride, ctx->slot[S_IN1].as<uint32_t>(),
+                            ctx->slot[S_OUT].as<uint32_t>());
     hip_check(hipMemcpyAsync(counts, ctx->slot[S_OUT].p, count * 4, hipMemcpyDeviceToHost, ctx->stream), "D2H");
     hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
   });

@@ -405,10 +405,16 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_encrypt_crt_finish(const Pail
 // gives floor(-1 / p) = -1, i.e. p - 1 (the floor semantics of scheme.py:86-104).
 // m_p = L_p(u_p) h_p mod p, m_q likewise; then Garner
 // m = m_q + q ((m_p - m_q) q^-1 mod p), q h < n exact at H limbs.
+// flag != nullptr: an item with p | c or q | c (a non-unit: u_p or u_q is 0)
+// writes `token` to flag[0]; such items are where this formula and the
+// full-width one may differ. out may be nullptr; popcnt != nullptr: the
+// odd-bit count of m under the low H limbs of mask.
 template <int H, int TPI>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_crt_finish(const PaillierDev* __restrict__ key,
                                                               const u32* __restrict__ U, u32* __restrict__ out,
-                                                              int count, int stride) {
+                                                              int count, int stride, u32* __restrict__ flag,
+                                                              u32 token, const u32* __restrict__ mask,
+                                                              u32* __restrict__ popcnt) {
   extern __shared__ uint4 smem[];
   constexpr int HH = H / 2, TH = TPI / 2, K = H / TPI;
   static_assert(TPI >= 2 && HH / TH == K, "two half-width halves per group");
@@ -425,6 +431,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_crt_finish(const PaillierDev*
 #pragma unroll
   for (int k = 0; k < K; ++k) nz |= u[k] | hi[k];
   const bool is_zero = Grp<TH>::group_bits(__ballot(nz != 0)) == 0ull;
+  if (flag != nullptr && is_zero && item < count && Grp<TH>::rank() == 0) flag[0] = token;
   (void)sub_small<K, TH>(u, 1u);
   u32 lq[K];
   M.exact_div(lq, u);
@@ -464,7 +471,11 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_crt_finish(const PaillierDev*
   load_const<H, TPI>(qr, key->qRn.v);
   Mn.mul(res, h1, qr);  // q*h  (< n, exact)
   (void)add_dist<K, TPI>(res, mq);
-  if (item < count) store_dist<H, TPI>(out, res, item, stride);
+  if (out != nullptr && item < count) store_dist<H, TPI>(out, res, item, stride);
+  if (popcnt != nullptr) {
+    const u32 c = group_popcount<H, TPI>(res, mask);
+    if (item < count && Grp<TPI>::rank() == 0) popcnt[item] = c;
+  }
 }
 
 // ---------------------------------------------------------------------------

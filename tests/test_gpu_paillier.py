@@ -492,6 +492,17 @@ def test_paillier_exact_crt_decrypt_matches_full_width(ctx, key_len):
         assert limbs_to_ints(crt.decrypt(cs)) == want, f"variant {i}"
     mask = sum(1 << b for b in range(1, 32 * W, 2))
     assert [int(x) for x in crt.decode_hamming(cs, mask)] == [bin(m & mask).count("1") for m in want]
+    # host entry points: units alone take the textbook CRT pass only; a batch
+    # with a non-unit raises its flag and is decrypted again in the exact form
+    nv = len(ms)
+    for batch, passes in ((cs[:nv], 0), (cs, 1)):
+        ctx.reset_timing()
+        ctx.set_timing(True)
+        got = limbs_to_ints(crt.decrypt(batch))
+        ctx.set_timing(False)
+        assert got == want[:len(batch)]
+        assert ctx.kernel_time("crt_finish")[1] == 1 and ctx.kernel_time("decrypt_finish")[1] == passes
+    assert [int(x) for x in crt.decode_hamming(cs[:nv], mask)] == [bin(m & mask).count("1") for m in ms]
     # encryption: r^n mod p^2 and mod q^2 at half width, recombined; r = 1,
     # n - 1, non-units (multiples of p and q), random
     rs = [1, n - 1, p, 2 * q, 3 * p] + [O.random_r(n, rng) for _ in range(len(ms) - 5)]
