@@ -198,7 +198,11 @@ int chx_paillier_encrypt(chx_ctx* ctx, const chx_pkey* k, const uint32_t* m, con
 int chx_paillier_encrypt_dev(chx_ctx* ctx, const chx_pkey* k, const uint32_t* m, const uint32_t* r,
                              uint32_t* c, size_t count, size_t stride);
 /* m[i] = L(c[i]^phi mod n^2) * inv mod n -- decrypt_only_kernel (cu:549-587) /
- * Paillier.decrypt (scheme.py:86-104). c < n^2 required. */
+ * Paillier.decrypt (scheme.py:86-104). c < n^2 required. For a key created
+ * with p and q, this and chx_paillier_decode_hamming first run the CRT
+ * decryption (equal to the formula for every unit c) and decrypt the batch
+ * again in the exact form only when it holds a non-unit (p | c or q | c);
+ * the results are the formula's for every c either way. */
 int chx_paillier_decrypt(chx_ctx* ctx, const chx_pkey* k, const uint32_t* c, uint32_t* m, size_t count);
 int chx_paillier_decrypt_dev(chx_ctx* ctx, const chx_pkey* k, const uint32_t* c, uint32_t* m, size_t count,
                              size_t stride);
