@@ -228,3 +228,85 @@ def test_planted_mistakes_are_caught(golden, monkeypatch):
     mb = O.embed_to_plaintexts(b, 16, 1024)[0]
     even_mask = O.odd_mask(2048) << 1
     assert bin((ma + mb) & even_mask).count("1") != O.hamming(a, b)
+
+
+# --------------------------------------------------------------------------
+# bit packing and decode pinned to their closed forms
+# --------------------------------------------------------------------------
+# hamming/paillier.py:86-94 writes the pair "0 b_j" for every bit, cuts the
+# string into chunks of chunk_len = 2 key_len characters and parses each chunk
+# as a binary numeral, most significant character first. Chunk c therefore
+# holds the bits j in [c key_len, min((c+1) key_len, embed_len)); with E_c of
+# them, bit u of the chunk (u = j - c key_len) sits at string index 2u + 1 of
+# a numeral of 2 E_c digits, i.e. at weight 2^(2 E_c - 2 - 2u) = 4^(E_c-1-u).
+# The last chunk may be shorter: its numeral is just its own 2 E_c digits.
+
+def packing_closed_form(bits, key_len):
+    out = []
+    for c0 in range(0, len(bits), key_len):
+        chunk = bits[c0:c0 + key_len]
+        e = len(chunk)
+        out.append(sum(b * 4 ** (e - 1 - u) for u, b in enumerate(chunk)))
+    return out
+
+
+def packing_reversed_pairs(bits, key_len):
+    """A plausible mistake: the pairs of a chunk emitted in reversed bit order."""
+    return [sum(b * 4 ** u for u, b in enumerate(bits[c0:c0 + key_len])) for c0 in range(0, len(bits), key_len)]
+
+
+def test_bit_packing_known_answers():
+    # literal numerals of the "0 b" string: [0,1,0,1,1,0,1,0] -> "0001000101000100"
+    assert O.embed_to_plaintexts([0, 1, 0, 1, 1, 0, 1, 0], 8, 1024) == [0b0001000101000100]
+    assert O.embed_to_plaintexts([1, 1, 0, 0, 1, 0, 0, 1], 8, 1024) == [0b0101000001000001]
+    assert O.embed_to_plaintexts([1] + [0] * 7, 8, 1024) == [4 ** 7]
+    assert O.embed_to_plaintexts([0] * 7 + [1], 8, 1024) == [1]
+    # two chunks at key_len 4 (chunk_len 8): bits 0..3 and 4..5 (short last chunk)
+    assert O.embed_to_plaintexts([1, 0, 0, 1, 1, 1], 6, 4) == [0b01000001, 0b0101]
+
+
+@pytest.mark.parametrize("embed_len,key_len", [(8, 1024), (512, 1024), (1024, 1024), (1500, 1024),
+                                               (2048, 1024), (2049, 1024), (600, 256), (64, 512)])
+def test_bit_packing_matches_closed_form(embed_len, key_len):
+    rng = random.Random(embed_len * 31 + key_len)
+    chunk_len, chunk_num = O.chunk_geometry(embed_len, key_len)
+    for t in range(8):
+        bits = [rng.getrandbits(1) for _ in range(embed_len)]
+        if t == 0:
+            bits = [1] * embed_len  # every pair set: the widest numeral of each chunk
+        want = packing_closed_form(bits, key_len)
+        assert len(want) == chunk_num
+        assert O.embed_to_plaintexts(bits, embed_len, key_len) == want
+        # every chunk's numeral fits a plaintext of n (2 key_len bits, top two set)
+        assert all(m < 2 ** (chunk_len - 1) for m in want)
+        # decode of a single encoding is the weight of the embedding
+        assert O.decode_hamming(want, chunk_len) == sum(bits)
+
+
+def test_decode_matches_closed_form():
+    """decode = popcount at even LSB indices of each plaintext (the odd string
+    indices after left-padding to chunk_len), for any plaintext below 2^chunk_len."""
+    rng = random.Random(77)
+    for chunk_len in (8, 512, 2048):
+        even = (4 ** (chunk_len // 2) - 1) // 3
+        for _ in range(20):
+            vals = [rng.getrandbits(rng.randrange(1, chunk_len + 1)) for _ in range(rng.randrange(1, 4))]
+            want = sum(bin(v & even).count("1") for v in vals)
+            assert O.decode_hamming(vals, chunk_len) == want
+        assert O.decode_hamming([0], chunk_len) == 0
+        assert O.decode_hamming([1], chunk_len) == 1  # "1" padded: index chunk_len - 1 is odd
+        assert O.decode_hamming([2], chunk_len) == 0
+        assert O.odd_mask(chunk_len) == even
+
+
+def test_planted_reversed_pairs_are_caught():
+    """Reversed pair order keeps every order-free identity but fails the closed form."""
+    rng = random.Random(5)
+    a = [rng.getrandbits(1) for _ in range(512)]
+    b = [rng.getrandbits(1) for _ in range(512)]
+    ra, rb = packing_reversed_pairs(a, 1024), packing_reversed_pairs(b, 1024)
+    # the identity decode(enc(a) + enc(b)) = Hamming(a, b) still holds ...
+    assert O.decode_hamming([x + y for x, y in zip(ra, rb)], 2048) == O.hamming(a, b)
+    # ... but the closed form (and so the oracle) disagrees
+    assert ra != packing_closed_form(a, 1024)
+    assert ra != O.embed_to_plaintexts(a, 512, 1024)
