@@ -112,6 +112,7 @@ SIGNATURES = {
     "chx_il_to_aos_dev": (c_int, [vp, vp, vp, c_size_t, c_int, c_size_t]),
     "chx_host_schedule": (c_int, [u32p, c_int, c_int, u32p, c_int]),
     "chx_host_mont_consts": (c_int, [u32p, c_int, u32p, u32p]),
+    "chx_host_paillier_check": (c_int, [u32p, c_int, u32p, u32p, u32p, u32p, c_int, u32p, u32p, u32p]),
     "chx_probe_mad": (c_int, [vp, c_int, c_int, POINTER(c_double), POINTER(c_double)]),
     "chx_probe_rate": (c_int, [vp, c_int, c_int, c_int, POINTER(c_double), POINTER(c_double)]),
     "chx_probe_copy": (c_int, [vp, vp, vp, c_size_t, c_size_t, c_int, c_int]),

@@ -203,5 +203,156 @@ inline int default_window(int ebits) {
   return 2;
 }
 
+// Value comparison of two vectors of different lengths (high zero limbs ignored).
+inline int bn_cmp_val(const BN& a, const BN& b) {
+  const size_t n = a.size() > b.size() ? a.size() : b.size();
+  for (size_t k = n; k-- > 0;) {
+    const uint32_t x = k < a.size() ? a[k] : 0u, y = k < b.size() ? b[k] : 0u;
+    if (x != y) return x < y ? -1 : 1;
+  }
+  return 0;
+}
+
+// Full product a * b (a.size() + b.size() limbs), schoolbook.
+inline BN bn_mul(const BN& a, const BN& b) {
+  BN r(a.size() + b.size(), 0u);
+  for (size_t i = 0; i < a.size(); ++i) {
+    uint64_t c = 0;
+    for (size_t j = 0; j < b.size(); ++j) {
+      const uint64_t t = (uint64_t)a[i] * b[j] + r[i + j] + c;
+      r[i + j] = (uint32_t)t;
+      c = t >> 32;
+    }
+    r[i + b.size()] = (uint32_t)c;
+  }
+  return r;
+}
+
+// a * b mod m (any widths; result at m's width)
+inline BN bn_mulmod(const BN& a, const BN& b, const BN& m) { return bn_mod(bn_mul(a, b), m); }
+
+// Montgomery product a b R^-1 mod N for a, b < N (CIOS, Koc-Acar-Kaliski 1996),
+// canonical result.
+inline BN mont_mul(const BN& a, const BN& b, const MontConsts& c) {
+  const int L = c.L;
+  std::vector<uint32_t> t(L + 2, 0u);
+  for (int i = 0; i < L; ++i) {
+    uint64_t C = 0;
+    for (int j = 0; j < L; ++j) {
+      const uint64_t s = (uint64_t)a[j] * b[i] + t[j] + C;
+      t[j] = (uint32_t)s;
+      C = s >> 32;
+    }
+    uint64_t s = (uint64_t)t[L] + C;
+    t[L] = (uint32_t)s;
+    t[L + 1] = (uint32_t)(s >> 32);
+    const uint32_t m = t[0] * c.n0inv;
+    s = (uint64_t)m * c.N[0] + t[0];
+    C = s >> 32;
+    for (int j = 1; j < L; ++j) {
+      s = (uint64_t)m * c.N[j] + t[j] + C;
+      t[j - 1] = (uint32_t)s;
+      C = s >> 32;
+    }
+    s = (uint64_t)t[L] + C;
+    t[L - 1] = (uint32_t)s;
+    t[L] = t[L + 1] + (uint32_t)(s >> 32);
+    t[L + 1] = 0;
+  }
+  BN r(t.begin(), t.begin() + L);
+  if (t[L] || bn_cmp(r, c.N) >= 0) bn_sub(r, c.N);
+  return r;
+}
+
+// b^e mod N (N odd; b of any width), left-to-right binary.
+inline BN bn_powmod(const BN& b, const BN& e, const MontConsts& c) {
+  const BN x = mont_mul(bn_mod(b, c.N), c.R2, c);
+  BN acc = c.R1;
+  for (int i = bn_bits(e) - 1; i >= 0; --i) {
+    acc = mont_mul(acc, acc, c);
+    if ((e[i / 32] >> (i % 32)) & 1u) acc = mont_mul(acc, x, c);
+  }
+  BN one(c.L, 0u);
+  one[0] = 1;
+  return mont_mul(acc, one, c);
+}
+
+// Miller-Rabin with the first `rounds` primes as bases after trial division
+// (HAC 4.24). An integrity check of caller-supplied factors, not a prime
+// search: fixed bases catch every accidental composite.
+inline bool bn_probable_prime(const BN& n, int rounds) {
+  static const uint32_t small[] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53};
+  const int nbits = bn_bits(n);
+  if (nbits <= 32) {  // exact trial division for one-limb values
+    const uint32_t v = n[0];
+    if (v < 2) return false;
+    for (uint32_t d = 2; (uint64_t)d * d <= v; ++d)
+      if (v % d == 0) return false;
+    return true;
+  }
+  auto mod_small = [&](uint32_t d) {
+    uint64_t r = 0;
+    for (size_t k = n.size(); k-- > 0;) r = ((r << 32) | n[k]) % d;
+    return (uint32_t)r;
+  };
+  for (uint32_t d : small)
+    if (mod_small(d) == 0) return false;
+  BN w(n.begin(), n.begin() + (nbits + 31) / 32);
+  const MontConsts c = mont_consts(w);
+  BN one(w.size(), 0u), nm1 = w;
+  one[0] = 1;
+  bn_sub(nm1, one);
+  int s = 0;
+  while (!((nm1[s / 32] >> (s % 32)) & 1u)) ++s;
+  BN d(w.size(), 0u);  // (n - 1) >> s
+  for (int bit = s; bit < nbits; ++bit)
+    if ((nm1[bit / 32] >> (bit % 32)) & 1u) d[(bit - s) / 32] |= 1u << ((bit - s) % 32);
+  for (int k = 0; k < rounds && k < 16; ++k) {
+    BN a(w.size(), 0u);
+    a[0] = small[k];
+    BN x = bn_powmod(a, d, c);
+    if (bn_cmp(x, one) == 0 || bn_cmp(x, nm1) == 0) continue;
+    bool witness = true;
+    for (int i = 1; i < s && witness; ++i) {
+      x = bn_mulmod(x, x, w);
+      if (bn_cmp(x, nm1) == 0) witness = false;
+    }
+    if (witness) return false;
+  }
+  return true;
+}
+
+// Consistency of a Paillier secret key given with its factors (g = n + 1):
+// p, q distinct odd primes with p q = n; phi inv = 1 (mod n); lcm(p-1, q-1)
+// divides phi (so c^phi of every unit c = (1+n)^m r^n is (1+n)^(m phi));
+// h_p (-q) = 1 (mod p), h_q (-p) = 1 (mod q) (h_p = L_p(g^(p-1) mod p^2)^-1,
+// as g^(p-1) = 1 + (p-1) n mod p^2); q_inv q = 1 (mod p). Under these the
+// textbook CRT decryption equals L(c^phi mod n^2) inv mod n on every unit.
+// Throws std::invalid_argument naming the first failed condition.
+inline void check_paillier_factors(const BN& n, const BN& phi, const BN& inv, const BN& p, const BN& q,
+                                   const BN& hp, const BN& hq, const BN& qinv) {
+  auto bad = [](const char* what) { throw std::invalid_argument(what); };
+  BN one(1, 1u);
+  if (bn_bits(p) < 2 || bn_bits(q) < 2 || !(p[0] & 1u) || !(q[0] & 1u)) bad("p and q must be odd and above 2");
+  if (bn_cmp_val(p, q) == 0) bad("p and q must differ");
+  if (bn_cmp_val(bn_mul(p, q), n) != 0) bad("p * q != n");
+  if (!bn_probable_prime(p, 12) || !bn_probable_prime(q, 12)) bad("p and q must be prime");
+  if (bn_cmp_val(bn_mulmod(phi, inv, n), one) != 0) bad("phi * inv != 1 (mod n)");
+  BN pm1 = p, qm1 = q;  // p and q are odd: clearing bit 0 subtracts 1
+  pm1[0] &= ~1u;
+  qm1[0] &= ~1u;
+  if (!bn_is_zero(bn_mod(phi, pm1)) || !bn_is_zero(bn_mod(phi, qm1))) bad("lcm(p-1, q-1) must divide phi");
+  auto neg_mod = [](const BN& x, const BN& m) {  // (-x) mod m
+    BN r = bn_mod(x, m);
+    if (bn_is_zero(r)) return r;
+    BN t = m;
+    bn_sub(t, r);
+    return t;
+  };
+  if (bn_cmp_val(bn_mulmod(hp, neg_mod(q, p), p), one) != 0) bad("h_p * (-q) != 1 (mod p)");
+  if (bn_cmp_val(bn_mulmod(hq, neg_mod(p, q), q), one) != 0) bad("h_q * (-p) != 1 (mod q)");
+  if (bn_cmp_val(bn_mulmod(qinv, q, p), one) != 0) bad("q_inv * q != 1 (mod p)");
+}
+
 }  // namespace host
 }  // namespace chx

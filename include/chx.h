@@ -167,7 +167,8 @@ int chx_ragged_destroy(chx_ragged* rg);
  * n_limbs: limbs of n (<= 64); the working width is the smallest of 32/64/128
  * limbs that holds n^2. inv = phi^-1 mod n. When p != NULL, CRT decryption is
  * enabled: p, q (half_limbs each, p*q = n), hp = L_p(g^(p-1) mod p^2)^-1 mod p,
- * hq likewise, qinv = q^-1 mod p. Replaces PaillierCPU::key_gen output handling
+ * hq likewise, qinv = q^-1 mod p; such a key must pass chx_host_paillier_check
+ * (CHX_EINVAL otherwise). Replaces PaillierCPU::key_gen output handling
  * (cu:183-216) and load_stringified_keys (cu:826-851). */
 int chx_paillier_key_create(chx_ctx* ctx, const uint32_t* n, int n_limbs, const uint32_t* phi,
                             const uint32_t* inv, const uint32_t* p, const uint32_t* q, int half_limbs,
@@ -286,6 +287,16 @@ int chx_il_to_aos_dev(chx_ctx* ctx, const uint32_t* il, uint32_t* aos, size_t co
 int chx_host_schedule(const uint32_t* e, int e_limbs, int window, uint32_t* out, int cap);
 /* Montgomery constants of odd n (limbs): n0inv = -n^-1 mod 2^32, r2 = R^2 mod n */
 int chx_host_mont_consts(const uint32_t* n, int limbs, uint32_t* n0inv, uint32_t* r2);
+/* Consistency of a Paillier secret key given with its factors, the check
+ * chx_paillier_key_create applies to such a key: p, q distinct primes (odd,
+ * Miller-Rabin with 12 fixed bases) with p q = n, phi inv = 1 mod n,
+ * lcm(p-1, q-1) | phi, h_p (-q) = 1 mod p, h_q (-p) = 1 mod q, q_inv q = 1
+ * mod p. n, phi, inv: n_limbs limbs; the rest half_limbs. CHX_EINVAL names
+ * the first failed condition (chx_last_error). No counterpart in the
+ * reference, whose keys carry no factors (scheme.py:24-44). */
+int chx_host_paillier_check(const uint32_t* n, int n_limbs, const uint32_t* phi, const uint32_t* inv,
+                            const uint32_t* p, const uint32_t* q, int half_limbs, const uint32_t* hp,
+                            const uint32_t* hq, const uint32_t* qinv);
 
 /* ---- measurement ---- */
 /* time `iters` rounds of 16 dependent-free v_mad_u64_u32 per lane on `blocks`

@@ -194,3 +194,47 @@ def test_product_keygen_is_valid():
     n, n2, g, a = lk["pk"]["n"], lk["pk"]["n_squared"], lk["pk"]["g"], lk["sk"]["a"]
     assert pow(g, n * a, n2) == 1 and pow(g, n, n2) != 1
     assert lk["sk"]["g_a_inv"] == pow(O.L(pow(g, a, n2), n), -1, n)
+
+
+@pytest.mark.parametrize("key_len", [16, 256, 512])
+def test_paillier_factor_check_accepts_valid_and_refuses_each_inconsistency(key_len):
+    """chx_host_paillier_check (run by chx_paillier_key_create on keys given
+    with factors): a valid key passes; each single inconsistency is refused
+    with a message that names it."""
+    import math
+    from cuhepy_amd import _native as N
+    from cuhepy_amd.engine import crt_constants
+    k = O.key_gen(key_len, 31 + key_len)
+    n, phi, inv, p, q = k["pk"]["n"], k["sk"]["phi"], k["sk"]["inv"], k["p"], k["q"]
+    c = crt_constants(p, q)
+
+    def run(**kw):
+        v = dict(n=n, phi=phi, inv=inv, p=p, q=q, hp=c["hp"], hq=c["hq"], qinv=c["qinv"])
+        v.update(kw)
+        big = ("n", "phi", "inv")
+        nl = max((v[x].bit_length() + 31) // 32 for x in big)
+        hl = max((v[x].bit_length() + 31) // 32 for x in v if x not in big)
+        a = {key: arr(val, nl if key in big else hl) for key, val in v.items()}
+        rc = N.lib.chx_host_paillier_check(p32(a["n"]), nl, p32(a["phi"]), p32(a["inv"]), p32(a["p"]), p32(a["q"]),
+                                           hl, p32(a["hp"]), p32(a["hq"]), p32(a["qinv"]))
+        return rc, N.last_error()
+
+    assert run()[0] == N.CHX_OK, run()[1]
+    # phi = lambda = lcm(p-1, q-1) with its own inverse is a consistent key too
+    lam = (p - 1) * (q - 1) // math.gcd(p - 1, q - 1)
+    assert run(phi=lam, inv=pow(lam, -1, n))[0] == N.CHX_OK
+    bad_phi = phi + 1 if math.gcd(phi + 1, n) == 1 else phi + 3  # odd: p - 1 does not divide it
+    cases = {
+        "h_p": dict(hp=(c["hp"] + 1) % p),
+        "h_q": dict(hq=(c["hq"] * 2) % q),
+        "q_inv": dict(qinv=(c["qinv"] + 1) % p),
+        "phi * inv": dict(inv=(inv + 1) % n),
+        "lcm": dict(phi=bad_phi, inv=pow(bad_phi, -1, n)),
+        "p * q": dict(q=q + 2),
+        "prime": dict(p=p * 3, n=p * 3 * q),
+        "differ": dict(q=p, n=p * p),
+    }
+    for what, kw in cases.items():
+        rc, msg = run(**kw)
+        assert rc == N.CHX_EINVAL, (what, msg)
+        assert what in msg, (what, msg)
