@@ -129,7 +129,7 @@ struct chx_ctx {
   bool timing = false;
   std::vector<TimedLaunch> pending;
   std::map<std::string, std::pair<double, int>> stats;
-  uint32_t packed_token = 0;  // per-call value of k_il_digits's and k_crt_finish's flags (never 0)
+  uint32_t packed_token = 0;  // per-call value of k_il_digits's flag (never 0)
 };
 
 // Lazy-radix constants of one modulus, one block per built lazy configuration
@@ -145,6 +145,7 @@ struct chx_mod {
   ModDev* d = nullptr;
   LazySet lz;
   uint32_t* d_fixb = nullptr;  // served-multiply constants 2^(RB(i+2)) R' mod N (built on first use)
+  std::mutex mu;               // guards d_fixb: a modulus may serve several contexts of its device
 };
 
 struct ExpSched {
@@ -744,8 +745,7 @@ void launch_encrypt(chx_pkey* k, const uint32_t* m, const uint32_t* r, uint32_t*
 }
 
 void launch_crt(chx_pkey* k, const uint32_t* c, uint32_t* m, size_t count, size_t stride,
-                uint32_t* flag = nullptr, uint32_t token = 0, const uint32_t* mask = nullptr,
-                uint32_t* pop = nullptr) {
+                uint32_t* bad = nullptr, const uint32_t* mask = nullptr, uint32_t* pop = nullptr) {
   chx_ctx* ctx = k->ctx;
   if (!k->crt) fail(CHX_EINVAL, "key was created without CRT factors");
   const int H = k->H;
@@ -768,7 +768,7 @@ void launch_crt(chx_pkey* k, const uint32_t* c, uint32_t* m, size_t count, size_
   record_start(ctx, "crt_finish");
   CHX_DISPATCH_H(H, tpi, {
     hipLaunchKernelGGL((k_crt_finish<H, TPI>), grid_for(count, TPI), dim3(BLOCK_THREADS), lds_bytes(H, TPI),
-                       ctx->stream, k->d, u, m, (int)count, (int)stride, flag, token, mask, pop);
+                       ctx->stream, k->d, u, m, (int)count, (int)stride, bad, mask, pop);
   });
   record_stop(ctx);
   check_launch("k_crt_finish");
@@ -1101,6 +1101,7 @@ int chx_mulmod(chx_ctx* ctx, const chx_mod* m, const uint32_t* a, const uint32_t
                size_t count) {
   return with_ctx(ctx, [&] {
     if (!m || !a || !b || !out) fail(CHX_EINVAL, "null argument");
+    if (m->ctx->device != ctx->device) fail(CHX_EINVAL, "modulus lives on another device");
     if (count == 0) return;
     const int L = m->hc.L;
     const size_t stride = batch_stride(count);
@@ -1116,6 +1117,7 @@ int chx_mulmod_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* a, const uint
                    size_t count, size_t stride) {
   return with_ctx(ctx, [&] {
     if (!m || !a || !b || !out) fail(CHX_EINVAL, "null argument");
+    if (m->ctx->device != ctx->device) fail(CHX_EINVAL, "modulus lives on another device");
     if (count == 0) return;
     check_stride(count, stride);
     launch_mulmod_mod(ctx, m, a, b, out, count, stride);
@@ -1139,6 +1141,7 @@ int chx_mulmod_operand_words(const chx_mod* m) {
 
 // B_i = 2^(RB (i + 2)) R' mod N as positions (LP rows), uploaded once per modulus
 static const uint32_t* fixed_constants(chx_ctx* ctx, chx_mod* m) {
+  std::lock_guard<std::mutex> lock(m->mu);
   if (m->d_fixb) return m->d_fixb;
   const LazyCfg c = lazy_cfgs(m->hc.L)[kServeCfg];
   std::vector<uint32_t> h((size_t)c.lp * c.lp);
@@ -1168,6 +1171,7 @@ int chx_mulmod_operands_dev(chx_ctx* ctx, const chx_mod* mc, const uint32_t* y, 
   chx_mod* m = const_cast<chx_mod*>(mc);
   return with_ctx(ctx, [&] {
     if (!m || !y || !ops) fail(CHX_EINVAL, "null argument");
+    if (m->ctx->device != ctx->device) fail(CHX_EINVAL, "modulus lives on another device");
     if (count == 0) return;
     launch_operands(ctx, m, y, ops, count);
   });
@@ -1177,6 +1181,7 @@ int chx_mulmod_operands(chx_ctx* ctx, const chx_mod* mc, const uint32_t* y, uint
   chx_mod* m = const_cast<chx_mod*>(mc);
   return with_ctx(ctx, [&] {
     if (!m || !y || !ops) fail(CHX_EINVAL, "null argument");
+    if (m->ctx->device != ctx->device) fail(CHX_EINVAL, "modulus lives on another device");
     if (count == 0) return;
     const size_t bytes = count * (size_t)m->hc.L * sizeof(uint32_t);
     ctx->slot[S_AUX].ensure(bytes);
@@ -1205,6 +1210,7 @@ int chx_mulmod_multi_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* x, size
                          size_t out_stride) {
   return with_ctx(ctx, [&] {
     if (!m || !x || !ops || !outs) fail(CHX_EINVAL, "null argument");
+    if (m->ctx->device != ctx->device) fail(CHX_EINVAL, "modulus lives on another device");
     if (nq < 1 || nq > MQ_MAX) fail(CHX_EINVAL, "nq=%d out of range [1, %d]", nq, MQ_MAX);
     if (count == 0) return;
     if (begin + count > stride) fail(CHX_EINVAL, "range [%zu, %zu) exceeds the index stride %zu", begin,
@@ -1243,6 +1249,7 @@ int chx_mulmod_bcast_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* x, cons
                          uint32_t* out, size_t count, size_t stride) {
   return with_ctx(ctx, [&] {
     if (!m || !x || !y_host || !out) fail(CHX_EINVAL, "null argument");
+    if (m->ctx->device != ctx->device) fail(CHX_EINVAL, "modulus lives on another device");
     if (count == 0) return;
     const int L = m->hc.L;
     check_stride(count, stride);
@@ -1273,6 +1280,7 @@ int chx_modexp(chx_ctx* ctx, const chx_mod* m, const uint32_t* base, size_t coun
                int e_limbs, uint32_t* out) {
   return with_ctx(ctx, [&] {
     if (!m || !base || !out || (!e && e_limbs)) fail(CHX_EINVAL, "null argument");
+    if (m->ctx->device != ctx->device) fail(CHX_EINVAL, "modulus lives on another device");
     if (count == 0) return;
     const int L = m->hc.L;
     check_below(base, count, L, m->hc.N, "base");
@@ -1300,6 +1308,7 @@ int chx_modexp_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* base, uint32_
                    size_t stride, const uint32_t* e, int e_limbs) {
   return with_ctx(ctx, [&] {
     if (!m || !base || !out || !e) fail(CHX_EINVAL, "null argument");
+    if (m->ctx->device != ctx->device) fail(CHX_EINVAL, "modulus lives on another device");
     if (count == 0) return;
     check_stride(count, stride);
     const int L = m->hc.L;
@@ -1318,6 +1327,7 @@ int chx_modexp_each(chx_ctx* ctx, const chx_mod* m, const uint32_t* base, size_t
                     int e_limbs, uint32_t* out) {
   return with_ctx(ctx, [&] {
     if (!m || !base || !out || !exps || e_limbs <= 0) fail(CHX_EINVAL, "null argument");
+    if (m->ctx->device != ctx->device) fail(CHX_EINVAL, "modulus lives on another device");
     if (count == 0) return;
     const int L = m->hc.L;
     check_below(base, count, L, m->hc.N, "base");
@@ -1339,6 +1349,7 @@ int chx_modexp_each_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* base, co
   return with_ctx(ctx, [&] {
     if (!m || !base || !out || !exps || e_limbs <= 0 || e_bits <= 0 || e_bits > 32 * e_limbs)
       fail(CHX_EINVAL, "bad argument");
+    if (m->ctx->device != ctx->device) fail(CHX_EINVAL, "modulus lives on another device");
     if (count == 0) return;
     check_stride(count, stride);
     launch_modexp_each(ctx, m->lz, base, exps, e_limbs, e_bits, out, count, stride);
@@ -1574,6 +1585,19 @@ int chx_ragged_destroy(chx_ragged* rg) {
 // ---------------------------------------------------------------------------
 // Paillier
 // ---------------------------------------------------------------------------
+namespace {
+// Host check of a Paillier key given with its factors (host::check_paillier_factors).
+void check_factors(const uint32_t* n, int n_limbs, const uint32_t* phi, const uint32_t* inv, const uint32_t* p,
+                   const uint32_t* q, int half_limbs, const uint32_t* hp, const uint32_t* hq, const uint32_t* qinv) {
+  if (!n || !phi || !inv || !p || !q || !hp || !hq || !qinv || n_limbs <= 0 || n_limbs > 64 || half_limbs <= 0 ||
+      half_limbs > 64)
+    fail(CHX_EINVAL, "bad key arguments");
+  auto v = [](const uint32_t* x, int l) { return host::bn_from(x, l, l); };
+  host::check_paillier_factors(v(n, n_limbs), v(phi, n_limbs), v(inv, n_limbs), v(p, half_limbs), v(q, half_limbs),
+                               v(hp, half_limbs), v(hq, half_limbs), v(qinv, half_limbs));
+}
+}  // namespace
+
 int chx_paillier_key_create(chx_ctx* ctx, const uint32_t* n, int n_limbs, const uint32_t* phi,
                             const uint32_t* inv, const uint32_t* p, const uint32_t* q, int half_limbs,
                             const uint32_t* hp, const uint32_t* hq, const uint32_t* qinv, chx_pkey** out) {
@@ -1608,6 +1632,9 @@ int chx_paillier_key_create(chx_ctx* ctx, const uint32_t* n, int n_limbs, const 
       if (p && q && hp && hq && qinv && half_limbs > 0) {
         const int H = W / 2;
         if (half_limbs > H) fail(CHX_EINVAL, "factors wider than half the width");
+        // the textbook CRT pass equals the reference formula only for a
+        // consistent key: refuse any other
+        check_factors(n, n_limbs, phi, inv, p, q, half_limbs, hp, hq, qinv);
         if (H != 32 && H != 64) fail(CHX_EUNSUPPORTED, "CRT needs 32- or 64-limb halves (width %d)", W);
         BN pb = host::bn_from(p, half_limbs, H), qb = host::bn_from(q, half_limbs, H);
         // p^2 and q^2 must fit in H limbs, so p and q in H/2 (the CRT finish
@@ -1715,6 +1742,7 @@ int chx_paillier_encrypt(chx_ctx* ctx, const chx_pkey* kc, const uint32_t* m, co
   return with_ctx(ctx, [&] {
     if (!k || !m || !r || !c) fail(CHX_EINVAL, "null argument");
     if (k->ctx != ctx) fail(CHX_EINVAL, "key belongs to another context");
+    if (k->ctx != ctx) fail(CHX_EINVAL, "key belongs to another context");
     if (count == 0) return;
     const int L = k->W;
     check_below(m, count, L, k->n.N, "m");
@@ -1733,6 +1761,7 @@ int chx_paillier_encrypt_dev(chx_ctx* ctx, const chx_pkey* kc, const uint32_t* m
   chx_pkey* k = const_cast<chx_pkey*>(kc);
   return with_ctx(ctx, [&] {
     if (!k || !m || !r || !c) fail(CHX_EINVAL, "null argument");
+    if (k->ctx != ctx) fail(CHX_EINVAL, "key belongs to another context");
     if (count == 0) return;
     check_stride(count, stride);
     launch_encrypt(k, m, r, c, count, stride);
@@ -1757,45 +1786,83 @@ static void paillier_decrypt_impl(chx_pkey* k, const uint32_t* dc, uint32_t* dm,
 // Host entry points of a key with CRT factors: the textbook CRT decryption
 // (launch_crt, two half-width exponents of |p| and |q| bits) equals
 // scheme.py:86-104 for every unit c, since every unit of Z_(n^2)^* is an
-// encryption (1 + n)^m r^n. Its finish raises a per-call flag for any
-// non-unit (p | c or q | c); only then is the batch decrypted again by
-// paillier_decrypt_impl, whose results equal the formula for every c. A stale
-// flag word can only cause that second pass, never a different result.
-// Returns false (nothing launched) for a key without CRT factors.
+// encryption (1 + n)^m r^n (the key passed chx_host_paillier_check). Its
+// finish lists every non-unit (p | c or q | c); only those items are
+// decrypted again by paillier_decrypt_impl, whose results equal the formula
+// for every c, and their outputs replaced. When more than a quarter of the
+// batch is listed the whole batch takes the exact form instead (one pass).
+// Returns false (nothing launched) for a key without CRT factors; else the
+// listed item indices in `redo` (empty when the batch was fully decrypted).
 static bool decrypt_units_first(chx_pkey* k, const uint32_t* dc, uint32_t* dm, size_t count, size_t stride,
-                                const uint32_t* mask, uint32_t* pop) {
+                                const uint32_t* mask, uint32_t* pop, std::vector<uint32_t>& redo) {
+  redo.clear();
   if (!k->crt) return false;
   chx_ctx* ctx = k->ctx;
-  ctx->slot[S_AUX].ensure(16);
-  uint32_t* flag = ctx->slot[S_AUX].as<uint32_t>();
-  uint32_t token = ++ctx->packed_token;
-  if (token == 0) token = ctx->packed_token = 1;
-  launch_crt(k, dc, dm, count, stride, flag, token, mask, pop);  // dm: rows [0, H)
-  uint32_t h = 0;
-  hip_check(hipMemcpyAsync(&h, flag, 4, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+  ctx->slot[S_AUX].ensure((count + 1) * sizeof(uint32_t));
+  uint32_t* bad = ctx->slot[S_AUX].as<uint32_t>();
+  hip_check(hipMemsetAsync(bad, 0, sizeof(uint32_t), ctx->stream), "hipMemsetAsync");
+  launch_crt(k, dc, dm, count, stride, bad, mask, pop);  // dm: rows [0, H)
+  uint32_t nbad = 0;
+  hip_check(hipMemcpyAsync(&nbad, bad, 4, hipMemcpyDeviceToHost, ctx->stream), "D2H");
   hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-  if (h == token) {
+  if (nbad > count) fail(CHX_EDEVICE, "non-unit list overflow");
+  if ((size_t)nbad * 4 > count) {  // many non-units: one exact pass over the batch
     paillier_decrypt_impl(k, dc, dm, count, stride, mask, pop);
-  } else if (dm) {  // m < n: the upper half of the W-limb rows is zero
+    return true;
+  }
+  if (dm)  // m < n: the upper half of the W-limb rows is zero
     hip_check(hipMemsetAsync(dm + (size_t)k->H * stride, 0, (size_t)k->H * stride * sizeof(uint32_t), ctx->stream),
               "hipMemsetAsync");
+  if (nbad) {
+    redo.resize(nbad);
+    hip_check(hipMemcpyAsync(redo.data(), bad + 1, (size_t)nbad * 4, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+    hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
   }
   return true;
+}
+
+// Exact decryption of the listed items of a host batch c (W limbs each), after
+// the main pass's results are on the host: m rows (W limbs) or odd-bit counts
+// (pop, with the device mask) of those items are replaced.
+static void redo_exact(chx_pkey* k, const uint32_t* c, const std::vector<uint32_t>& redo, uint32_t* m,
+                       const uint32_t* dmask, uint32_t* pop) {
+  if (redo.empty()) return;
+  chx_ctx* ctx = k->ctx;
+  const int L = k->W;
+  const size_t nb = redo.size(), st = batch_stride(nb);
+  std::vector<uint32_t> sub(nb * L);
+  for (size_t i = 0; i < nb; ++i) memcpy(&sub[i * L], c + (size_t)redo[i] * L, L * sizeof(uint32_t));
+  uint32_t* dc = upload_il(ctx, S_IN0, sub.data(), nb, L, st);
+  if (m) {
+    ctx->slot[S_OUT].ensure(st * L * 4);
+    paillier_decrypt_impl(k, dc, ctx->slot[S_OUT].as<uint32_t>(), nb, st, nullptr, nullptr);
+    download_il(ctx, ctx->slot[S_OUT].as<uint32_t>(), sub.data(), nb, L, st);
+    for (size_t i = 0; i < nb; ++i) memcpy(m + (size_t)redo[i] * L, &sub[i * L], L * sizeof(uint32_t));
+  } else {
+    ctx->slot[S_OUT].ensure(nb * 4);
+    paillier_decrypt_impl(k, dc, nullptr, nb, st, dmask, ctx->slot[S_OUT].as<uint32_t>());
+    hip_check(hipMemcpyAsync(sub.data(), ctx->slot[S_OUT].p, nb * 4, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+    hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    for (size_t i = 0; i < nb; ++i) pop[redo[i]] = sub[i];
+  }
 }
 
 int chx_paillier_decrypt(chx_ctx* ctx, const chx_pkey* kc, const uint32_t* c, uint32_t* m, size_t count) {
   chx_pkey* k = const_cast<chx_pkey*>(kc);
   return with_ctx(ctx, [&] {
     if (!k || !c || !m) fail(CHX_EINVAL, "null argument");
+    if (k->ctx != ctx) fail(CHX_EINVAL, "key belongs to another context");
     if (count == 0) return;
     const int L = k->W;
     check_below(c, count, L, k->n2.N, "c");
     const size_t stride = batch_stride(count);
     uint32_t* dc = upload_il(ctx, S_IN0, c, count, L, stride);
     ctx->slot[S_OUT].ensure(stride * L * 4);
-    if (!decrypt_units_first(k, dc, ctx->slot[S_OUT].as<uint32_t>(), count, stride, nullptr, nullptr))
+    std::vector<uint32_t> redo;
+    if (!decrypt_units_first(k, dc, ctx->slot[S_OUT].as<uint32_t>(), count, stride, nullptr, nullptr, redo))
       paillier_decrypt_impl(k, dc, ctx->slot[S_OUT].as<uint32_t>(), count, stride, nullptr, nullptr);
     download_il(ctx, ctx->slot[S_OUT].as<uint32_t>(), m, count, L, stride);
+    redo_exact(k, c, redo, m, nullptr, nullptr);
   });
 }
 
@@ -1804,6 +1871,7 @@ int chx_paillier_decrypt_dev(chx_ctx* ctx, const chx_pkey* kc, const uint32_t* c
   chx_pkey* k = const_cast<chx_pkey*>(kc);
   return with_ctx(ctx, [&] {
     if (!k || !c || !m) fail(CHX_EINVAL, "null argument");
+    if (k->ctx != ctx) fail(CHX_EINVAL, "key belongs to another context");
     if (count == 0) return;
     check_stride(count, stride);
     paillier_decrypt_impl(k, c, m, count, stride, nullptr, nullptr);
@@ -1814,6 +1882,7 @@ int chx_paillier_decrypt_crt(chx_ctx* ctx, const chx_pkey* kc, const uint32_t* c
   chx_pkey* k = const_cast<chx_pkey*>(kc);
   return with_ctx(ctx, [&] {
     if (!k || !c || !m) fail(CHX_EINVAL, "null argument");
+    if (k->ctx != ctx) fail(CHX_EINVAL, "key belongs to another context");
     if (count == 0) return;
     const int L = k->W;
     check_below(c, count, L, k->n2.N, "c");
@@ -1830,6 +1899,7 @@ int chx_paillier_decrypt_crt_dev(chx_ctx* ctx, const chx_pkey* kc, const uint32_
   chx_pkey* k = const_cast<chx_pkey*>(kc);
   return with_ctx(ctx, [&] {
     if (!k || !c || !m) fail(CHX_EINVAL, "null argument");
+    if (k->ctx != ctx) fail(CHX_EINVAL, "key belongs to another context");
     if (count == 0) return;
     check_stride(count, stride);
     launch_crt(k, c, m, count, stride);
@@ -1841,6 +1911,7 @@ int chx_paillier_decode_hamming(chx_ctx* ctx, const chx_pkey* kc, const uint32_t
   chx_pkey* k = const_cast<chx_pkey*>(kc);
   return with_ctx(ctx, [&] {
     if (!k || !c || !mask || !counts) fail(CHX_EINVAL, "null argument");
+    if (k->ctx != ctx) fail(CHX_EINVAL, "key belongs to another context");
     if (count == 0) return;
     const int L = k->W;
     check_below(c, count, L, k->n2.N, "c");
@@ -1849,12 +1920,14 @@ int chx_paillier_decode_hamming(chx_ctx* ctx, const chx_pkey* kc, const uint32_t
     ctx->slot[S_IN1].ensure(L * 4);
     hip_check(hipMemcpyAsync(ctx->slot[S_IN1].p, mask, L * 4, hipMemcpyHostToDevice, ctx->stream), "H2D");
     ctx->slot[S_OUT].ensure(count * 4);
+    std::vector<uint32_t> redo;
     if (!decrypt_units_first(k, dc, nullptr, count, stride, ctx->slot[S_IN1].as<uint32_t>(),
-                             ctx->slot[S_OUT].as<uint32_t>()))
+                             ctx->slot[S_OUT].as<uint32_t>(), redo))
       paillier_decrypt_impl(k, dc, nullptr, count, stride, ctx->slot[S_IN1].as<uint32_t>(),
                             ctx->slot[S_OUT].as<uint32_t>());
     hip_check(hipMemcpyAsync(counts, ctx->slot[S_OUT].p, count * 4, hipMemcpyDeviceToHost, ctx->stream), "D2H");
     hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    redo_exact(k, c, redo, nullptr, ctx->slot[S_IN1].as<uint32_t>(), counts);
   });
 }
 
@@ -1978,6 +2051,7 @@ int chx_lookup_encrypt(chx_ctx* ctx, const chx_lkey* kc, const uint8_t* digits, 
   chx_lkey* k = const_cast<chx_lkey*>(kc);
   return with_ctx(ctx, [&] {
     if (!k || !digits || !c || (nmult > 0 && !noise_idx)) fail(CHX_EINVAL, "null argument");
+    if (k->ctx != ctx) fail(CHX_EINVAL, "key belongs to another context");
     if (count == 0) return;
     if (nmult < 0 || nmult > 64) fail(CHX_EINVAL, "nmult out of range");
     for (size_t i = 0; i < count * (size_t)nmult; ++i)
@@ -2012,6 +2086,7 @@ int chx_lookup_encrypt_dev(chx_ctx* ctx, const chx_lkey* kc, const uint8_t* digi
   chx_lkey* k = const_cast<chx_lkey*>(kc);
   return with_ctx(ctx, [&] {
     if (!k || !digits || !c || (nmult > 0 && !noise_idx)) fail(CHX_EINVAL, "null argument");
+    if (k->ctx != ctx) fail(CHX_EINVAL, "key belongs to another context");
     if (count == 0) return;
     check_stride(count, stride);
     launch_lookup_encrypt(k, digits, noise_idx, nmult, c, count, stride);
@@ -2036,6 +2111,7 @@ int chx_lookup_decrypt(chx_ctx* ctx, const chx_lkey* kc, const uint32_t* c, uint
   chx_lkey* k = const_cast<chx_lkey*>(kc);
   return with_ctx(ctx, [&] {
     if (!k || !c || !m) fail(CHX_EINVAL, "null argument");
+    if (k->ctx != ctx) fail(CHX_EINVAL, "key belongs to another context");
     if (count == 0) return;
     const int L = k->W;
     check_below(c, count, L, k->n2.N, "c");
@@ -2052,6 +2128,7 @@ int chx_lookup_decode_hamming(chx_ctx* ctx, const chx_lkey* kc, const uint32_t* 
   chx_lkey* k = const_cast<chx_lkey*>(kc);
   return with_ctx(ctx, [&] {
     if (!k || !c || !mask || !counts) fail(CHX_EINVAL, "null argument");
+    if (k->ctx != ctx) fail(CHX_EINVAL, "key belongs to another context");
     if (count == 0) return;
     const int L = k->W;
     check_below(c, count, L, k->n2.N, "c");
@@ -2169,6 +2246,12 @@ int chx_host_schedule(const uint32_t* e, int e_limbs, int window, uint32_t* out,
     n = (int)s.size();
   });
   return rc == CHX_OK ? n : -rc;
+}
+
+int chx_host_paillier_check(const uint32_t* n, int n_limbs, const uint32_t* phi, const uint32_t* inv,
+                            const uint32_t* p, const uint32_t* q, int half_limbs, const uint32_t* hp,
+                            const uint32_t* hq, const uint32_t* qinv) {
+  return guarded([&] { check_factors(n, n_limbs, phi, inv, p, q, half_limbs, hp, hq, qinv); });
 }
 
 int chx_host_mont_consts(const uint32_t* n, int limbs, uint32_t* n0inv, uint32_t* r2) {

@@ -405,15 +405,16 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_encrypt_crt_finish(const Pail
 // gives floor(-1 / p) = -1, i.e. p - 1 (the floor semantics of scheme.py:86-104).
 // m_p = L_p(u_p) h_p mod p, m_q likewise; then Garner
 // m = m_q + q ((m_p - m_q) q^-1 mod p), q h < n exact at H limbs.
-// flag != nullptr: an item with p | c or q | c (a non-unit: u_p or u_q is 0)
-// writes `token` to flag[0]; such items are where this formula and the
-// full-width one may differ. out may be nullptr; popcnt != nullptr: the
+// bad != nullptr: each item with p | c or q | c (a non-unit: u_p or u_q is 0)
+// appends its index to bad[1..] and counts itself in bad[0] (zeroed by the
+// caller); such items are where this formula and the full-width one may
+// differ. out may be nullptr; popcnt != nullptr: the
 // odd-bit count of m under the low H limbs of mask.
 template <int H, int TPI>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_crt_finish(const PaillierDev* __restrict__ key,
                                                               const u32* __restrict__ U, u32* __restrict__ out,
-                                                              int count, int stride, u32* __restrict__ flag,
-                                                              u32 token, const u32* __restrict__ mask,
+                                                              int count, int stride, u32* __restrict__ bad,
+                                                              const u32* __restrict__ mask,
                                                               u32* __restrict__ popcnt) {
   extern __shared__ uint4 smem[];
   constexpr int HH = H / 2, TH = TPI / 2, K = H / TPI;
@@ -431,7 +432,10 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_crt_finish(const PaillierDev*
 #pragma unroll
   for (int k = 0; k < K; ++k) nz |= u[k] | hi[k];
   const bool is_zero = Grp<TH>::group_bits(__ballot(nz != 0)) == 0ull;
-  if (flag != nullptr && is_zero && item < count && Grp<TH>::rank() == 0) flag[0] = token;
+  if (bad != nullptr) {  // non-units (p | c or q | c): append the item once to the list bad[1 + i]
+    const bool zq = __shfl_xor(is_zero ? 1 : 0, TH) != 0;  // the other half's test
+    if ((is_zero || zq) && item < count && Grp<TPI>::rank() == 0) bad[1 + atomicAdd(bad, 1u)] = (u32)item;
+  }
   (void)sub_small<K, TH>(u, 1u);
   u32 lq[K];
   M.exact_div(lq, u);

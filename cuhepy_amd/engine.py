@@ -319,11 +319,15 @@ class PaillierKey:
         """decrypt / decode_hamming run c^phi at half width (recombined exactly)."""
         return bool(N.lib.chx_paillier_key_has_idempotents(self.handle))
 
+    def close(self) -> None:
+        """Free the key's device memory (before its context is closed)."""
+        if getattr(self, "handle", None):
+            N.lib.chx_paillier_key_destroy(self.handle)
+            self.handle = None
+
     def __del__(self):  # pragma: no cover
         try:
-            if getattr(self, "handle", None):
-                N.lib.chx_paillier_key_destroy(self.handle)
-                self.handle = None
+            self.close()
         except Exception:
             pass
 

@@ -44,25 +44,33 @@ def test_encrypt_decrypt_golden(ctx, golden, case):
 
 @pytest.mark.parametrize("key_len", [256, 512, 1024])
 def test_seeded_batches_every_variant(ctx, key_len):
-    """encrypt / decrypt / CRT on seeded batches for every built lane-group size."""
+    """encrypt / decrypt / CRT on seeded batches for every built lane-group size,
+    with the key given its factors (half-width forms) and without them
+    (full-width exponentiation and finishes)."""
     from cuhepy_amd.engine import Context, PaillierKey
     from cuhepy_amd.limbs import limbs_to_ints
     keys = O.key_gen(key_len, 300 + key_len)
     n = keys["pk"]["n"]
     key = PaillierKey(ctx, n, keys["sk"]["phi"], keys["sk"]["inv"], keys["p"], keys["q"])
+    plain = PaillierKey(ctx, n, keys["sk"]["phi"], keys["sk"]["inv"])
     rng = random.Random(key_len)
     count = 97
     ms = [rng.randrange(n) for _ in range(count)]
     rs = [O.random_r(n, rng) for _ in range(count)]
     want_c = [O.encrypt(m, keys["pk"], r) for m, r in zip(ms, rs)]
-    exp_tpis = Context.supported_tpi(key.width, op=OP_MODEXP)
-    fin_tpis = Context.supported_tpi(key.width, op=OP_MULMOD)
-    for i in range(max(len(exp_tpis), len(fin_tpis))):
-        ctx.set_tuning(OP_MODEXP, key.width, exp_tpis[i % len(exp_tpis)], 0)
-        ctx.set_tuning(OP_MULMOD, key.width, fin_tpis[i % len(fin_tpis)], 0)
-        tag = f"exp tpi={exp_tpis[i % len(exp_tpis)]} finish tpi={fin_tpis[i % len(fin_tpis)]}"
-        assert limbs_to_ints(key.encrypt(ms, rs)) == want_c, f"encrypt {tag}"
-        assert limbs_to_ints(key.decrypt(want_c)) == ms, f"decrypt {tag}"
+    for k, widths in ((plain, (key.width,)), (key, (key.width, key.half) if key.crt else (key.width,))):
+        for w in widths:  # the lane counts that steer this key's launches
+            exp_tpis = Context.supported_tpi(w, op=OP_MODEXP)
+            fin_tpis = [t for t in Context.supported_tpi(w, op=OP_MULMOD) if k is plain or w == key.width or t >= 4]
+            for i in range(max(len(exp_tpis), len(fin_tpis))):
+                ctx.set_tuning(OP_MODEXP, w, exp_tpis[i % len(exp_tpis)], 0)
+                ctx.set_tuning(OP_MULMOD, w, fin_tpis[i % len(fin_tpis)], 0)
+                tag = f"{'factors' if k.crt else 'no factors'} width {w}: exp tpi={exp_tpis[i % len(exp_tpis)]} " \
+                      f"finish tpi={fin_tpis[i % len(fin_tpis)]}"
+                assert limbs_to_ints(k.encrypt(ms, rs)) == want_c, f"encrypt {tag}"
+                assert limbs_to_ints(k.decrypt(want_c)) == ms, f"decrypt {tag}"
+            ctx.set_tuning(OP_MODEXP, w, 0, 0)
+            ctx.set_tuning(OP_MULMOD, w, 0, 0)
     ctx.set_tuning(OP_MODEXP, key.width, 0, 0)
     ctx.set_tuning(OP_MULMOD, key.width, 0, 0)
     if not key.crt:  # 16-limb halves are not built: CRT is refused, plain decrypt covers the key
@@ -492,17 +500,30 @@ def test_paillier_exact_crt_decrypt_matches_full_width(ctx, key_len):
         assert limbs_to_ints(crt.decrypt(cs)) == want, f"variant {i}"
     mask = sum(1 << b for b in range(1, 32 * W, 2))
     assert [int(x) for x in crt.decode_hamming(cs, mask)] == [bin(m & mask).count("1") for m in want]
-    # host entry points: units alone take the textbook CRT pass only; a batch
-    # with a non-unit raises its flag and is decrypted again in the exact form
+    # host entry points: units alone take the textbook CRT pass only; the
+    # non-units of a batch are listed by the finish and only they are
+    # decrypted again in the exact form (a batch with more than a quarter of
+    # non-units takes the exact form whole); at every lane count of the
+    # half-width finish that lists them
     nv = len(ms)
-    for batch, passes in ((cs[:nv], 0), (cs, 1)):
-        ctx.reset_timing()
-        ctx.set_timing(True)
-        got = limbs_to_ints(crt.decrypt(batch))
-        ctx.set_timing(False)
-        assert got == want[:len(batch)]
-        assert ctx.kernel_time("crt_finish")[1] == 1 and ctx.kernel_time("decrypt_finish")[1] == passes
-    assert [int(x) for x in crt.decode_hamming(cs[:nv], mask)] == [bin(m & mask).count("1") for m in ms]
+    nonunit = [math.gcd(c, n) != 1 for c in cs]
+    few = cs[:nv] + [c for c, b in zip(cs, nonunit) if b][:3]  # 3 of 19: listed and redone
+    many = cs[nv:]  # more than a quarter non-units: one exact pass
+    assert sum(nonunit[nv:]) * 4 > len(many)
+    few_want = [O.decrypt(c, keys) for c in few]
+    for t in Context.supported_tpi(W // 2, op=OP_MULMOD):
+        if t < 4:  # the finish splits a lane group into p- and q-halves
+            continue
+        ctx.set_tuning(OP_MULMOD, W // 2, t, 0)
+        for batch, bwant, passes in ((cs[:nv], ms, 0), (few, few_want, 1), (many, want[nv:], 1), (cs, want, 1)):
+            ctx.reset_timing()
+            ctx.set_timing(True)
+            got = limbs_to_ints(crt.decrypt(batch))
+            ctx.set_timing(False)
+            assert got == bwant, f"finish tpi {t}"
+            assert ctx.kernel_time("crt_finish")[1] == 1 and ctx.kernel_time("decrypt_finish")[1] == passes
+            assert [int(x) for x in crt.decode_hamming(batch, mask)] == [bin(m & mask).count("1") for m in bwant]
+    ctx.set_tuning(OP_MULMOD, W // 2, 0, 0)
     # encryption: r^n mod p^2 and mod q^2 at half width, recombined; r = 1,
     # n - 1, non-units (multiples of p and q), random
     rs = [1, n - 1, p, 2 * q, 3 * p] + [O.random_r(n, rng) for _ in range(len(ms) - 5)]
