@@ -93,6 +93,7 @@ SIGNATURES = {
     "chx_paillier_decrypt_dev": (c_int, [vp, vp, vp, vp, c_size_t, c_size_t]),
     "chx_paillier_decrypt_crt": (c_int, [vp, vp, u32p, u32p, c_size_t]),
     "chx_paillier_decrypt_crt_dev": (c_int, [vp, vp, vp, vp, c_size_t, c_size_t]),
+    "chx_paillier_decrypt_crt_noalloc": (c_int, [vp, vp, vp, vp, c_size_t, c_size_t]),
     "chx_paillier_decode_hamming": (c_int, [vp, vp, u32p, u32p, u32p, c_size_t]),
     "chx_lookup_key_create": (c_int, [vp, u32p, c_int, u32p, c_int, u32p, u32p, c_int, u32p, c_int,
                                       POINTER(vp)]),

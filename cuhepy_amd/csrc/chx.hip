@@ -111,6 +111,47 @@ struct DevBuf {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// scratch of the allocation-free entry point (chx_paillier_decrypt_crt_noalloc)
+// ---------------------------------------------------------------------------
+// One file-scope device array: every GPU the library runs on holds its own copy
+// until the process ends. In host code `scratch` names a host-side stand-in, not
+// a device address: kernels and copies use the per-device address taken once
+// with hipGetSymbolAddress (chx_ctx_create, with the device current). Calls of
+// the entry point on one GPU share the array, so calls that overlap on one GPU
+// race; nothing in it has to start at zero (each call writes every word it
+// reads before reading it).
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) uint32_t scratch[kNoAllocScratchBytes / sizeof(uint32_t)];
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices];             // device address of `scratch` on each GPU
+std::string g_scratch_err[kMaxDevices];       // why it could not be taken, if it could not
+
+// With `device` current: take the device address of `scratch` once.
+void init_scratch(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices || g_scratch_ptr[device]) return;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || ptr == nullptr) {
+    g_scratch_err[device] = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    return;
+  }
+  g_scratch_ptr[device] = ptr;
+}
+
+uint32_t* scratch_ptr(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices) fail(CHX_EUNSUPPORTED, "device %d has no scratch array", device);
+  if (!g_scratch_ptr[device])
+    fail(CHX_EDEVICE, "scratch array of device %d unavailable: %s", device,
+         g_scratch_err[device].empty() ? "not initialised" : g_scratch_err[device].c_str());
+  return static_cast<uint32_t*>(g_scratch_ptr[device]);
+}
+}  // namespace
+
 enum Slot { S_IN0, S_IN1, S_OUT, S_AOS, S_TAB, S_TMP, S_AUX, S_MID0, S_MID1, S_NSLOT };
 
 struct TimedLaunch {
@@ -408,6 +449,12 @@ void download_il(chx_ctx* ctx, const uint32_t* il, uint32_t* host, size_t count,
   hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
 }
 
+// A schedule of the key already on the device, or nullptr (never builds one).
+const ExpSched* find_sched_p(chx_pkey* k, int which, int window) {
+  auto it = k->scheds.find(std::make_pair(which, window));
+  return it == k->scheds.end() ? nullptr : &it->second;
+}
+
 const ExpSched& get_sched_p(chx_pkey* k, int which, int window) {
   auto key = std::make_pair(which, window);
   auto it = k->scheds.find(key);
@@ -553,9 +600,16 @@ size_t lazy_lds_bytes() {
 // jobs[j]: input/output interleaved 32-bit buffers, schedule; the modulus
 // block of the chosen configuration is taken from sets[j].
 // in_hi (optional, per job): upper L32 limbs of double-width inputs
+// Window-table words of one job of run_lazy (configuration ci, window w).
+size_t lazy_table_words(int L32, int ci, int window, size_t count) {
+  return ((size_t)1 << (window - 1)) * lazy_cfgs(L32)[ci].lp * round_up(count, BLOCK_THREADS);
+}
+
+// tab: the window tables' scratch (njobs * lazy_table_words words), or
+// nullptr for the context's slot.
 void run_lazy(chx_ctx* ctx, int L32, const LazySet* sets, const ExpSched* const* scheds, const uint32_t* const* in,
               uint32_t* const* out, int njobs, size_t count, size_t stride,
-              const uint32_t* const* in_hi = nullptr) {
+              const uint32_t* const* in_hi = nullptr, uint32_t* tab = nullptr) {
   const int ci = pick_lazy(ctx, L32, count * njobs);
   const LazyCfg cfg = lazy_cfgs(L32)[ci];
   // a double-width input enters the exponentiation below 4N: R' > 16N keeps
@@ -566,8 +620,11 @@ void run_lazy(chx_ctx* ctx, int L32, const LazySet* sets, const ExpSched* const*
     if (scheds[j]->window != window) fail(CHX_EINVAL, "jobs of one launch must share the window width");
   const int tentries = 1 << (window - 1);
   const size_t tstride = round_up(count, BLOCK_THREADS);
-  const size_t tab_words = (size_t)tentries * cfg.lp * tstride;
-  ctx->slot[S_TAB].ensure(tab_words * njobs * sizeof(uint32_t));
+  const size_t tab_words = lazy_table_words(L32, ci, window, count);
+  if (tab == nullptr) {
+    ctx->slot[S_TAB].ensure(tab_words * njobs * sizeof(uint32_t));
+    tab = ctx->slot[S_TAB].as<uint32_t>();
+  }
   LazyJob jb[2];
   for (int j = 0; j < 2; ++j) {
     const int s = j < njobs ? j : 0;
@@ -577,7 +634,7 @@ void run_lazy(chx_ctx* ctx, int L32, const LazySet* sets, const ExpSched* const*
     jb[j].in = in[s];
     jb[j].in_hi = in_hi ? in_hi[s] : nullptr;
     jb[j].out = out[s];
-    jb[j].table = ctx->slot[S_TAB].as<uint32_t>() + (size_t)s * tab_words;
+    jb[j].table = tab + (size_t)s * tab_words;
   }
   record_start(ctx, "lazy_modexp");
   CHX_DISPATCH_LAZY(L32, ci, {
@@ -744,17 +801,46 @@ void launch_encrypt(chx_pkey* k, const uint32_t* m, const uint32_t* r, uint32_t*
   check_launch("k_encrypt_finish");
 }
 
+// Scratch handed to launch_crt by the allocation-free entry point: `words`
+// 32-bit words at `p` (device memory).
+struct FixedScratch {
+  uint32_t* p;
+  size_t words;
+};
+
+// With fixed != nullptr nothing is allocated: the exponentiation uses the
+// default window and the schedules built with the key, and the residues and
+// window tables are carved from `fixed` (refused if it is too small).
 void launch_crt(chx_pkey* k, const uint32_t* c, uint32_t* m, size_t count, size_t stride,
-                uint32_t* bad = nullptr, const uint32_t* mask = nullptr, uint32_t* pop = nullptr) {
+                uint32_t* bad = nullptr, const uint32_t* mask = nullptr, uint32_t* pop = nullptr,
+                const FixedScratch* fixed = nullptr) {
   chx_ctx* ctx = k->ctx;
   if (!k->crt) fail(CHX_EINVAL, "key was created without CRT factors");
   const int H = k->H;
-  const int w = pick_window(ctx, CHX_OP_MODEXP, H, std::max(exp_bits_p(k, 2), exp_bits_p(k, 3)));
-  const ExpSched& sp = get_sched_p(k, 2, w);
-  const ExpSched& sq = get_sched_p(k, 3, w);
+  const int ebits = std::max(exp_bits_p(k, 2), exp_bits_p(k, 3));
+  const int w = fixed ? host::default_window(ebits) : pick_window(ctx, CHX_OP_MODEXP, H, ebits);
+  const ExpSched* spp = fixed ? find_sched_p(k, 2, w) : &get_sched_p(k, 2, w);
+  const ExpSched* sqp = fixed ? find_sched_p(k, 3, w) : &get_sched_p(k, 3, w);
+  if (!spp || !sqp) fail(CHX_EINVAL, "the key's CRT schedules were not built at creation");
+  const ExpSched& sp = *spp;
+  const ExpSched& sq = *sqp;
   const size_t half_words = (size_t)H * stride;
-  ctx->slot[S_MID1].ensure(2 * half_words * sizeof(uint32_t));
-  uint32_t* u = ctx->slot[S_MID1].as<uint32_t>();
+  uint32_t* u = nullptr;
+  uint32_t* tab = nullptr;
+  if (fixed) {
+    const size_t u_words = round_up(2 * half_words, 64);  // 256-byte aligned tables after it
+    const size_t t_words = 2 * lazy_table_words(H, pick_lazy(ctx, H, count * 2), w, count);
+    if (u_words + t_words > fixed->words)
+      fail(CHX_EINVAL,
+           "%zu ciphertexts (stride %zu) need %zu bytes of scratch, above the %zu-byte array of the "
+           "allocation-free entry point",
+           count, stride, (u_words + t_words) * sizeof(uint32_t), fixed->words * sizeof(uint32_t));
+    u = fixed->p;
+    tab = fixed->p + u_words;
+  } else {
+    ctx->slot[S_MID1].ensure(2 * half_words * sizeof(uint32_t));
+    u = ctx->slot[S_MID1].as<uint32_t>();
+  }
   // the finish splits each lane group into a p- and a q-half: at least 2 lanes
   const int tpi = std::max(pick_tpi(ctx, CHX_OP_MULMOD, H, count), 4);
   // both halves read the whole ciphertext (rows [0, H) and [H, 2H)) and reduce
@@ -764,7 +850,7 @@ void launch_crt(chx_pkey* k, const uint32_t* c, uint32_t* m, size_t count, size_
   const uint32_t* ins[2] = {c, c};
   const uint32_t* his[2] = {c + half_words, c + half_words};
   uint32_t* outs[2] = {u, u + half_words};
-  run_lazy(ctx, H, sets, scheds, ins, outs, 2, count, stride, his);
+  run_lazy(ctx, H, sets, scheds, ins, outs, 2, count, stride, his, tab);
   record_start(ctx, "crt_finish");
   CHX_DISPATCH_H(H, tpi, {
     hipLaunchKernelGGL((k_crt_finish<H, TPI>), grid_for(count, TPI), dim3(BLOCK_THREADS), lds_bytes(H, TPI),
@@ -981,6 +1067,7 @@ int chx_ctx_create(int device, chx_ctx** out) {
     hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
     if (device < 0 || device >= n) fail(CHX_EINVAL, "device %d not present (%d visible)", device, n);
     hip_check(hipSetDevice(device), "hipSetDevice");
+    init_scratch(device);  // the allocation-free entry point's array, once per device
     chx_ctx* c = new chx_ctx();
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
@@ -1677,6 +1764,11 @@ int chx_paillier_key_create(chx_ctx* ctx, const uint32_t* n, int n_limbs, const 
         k->e_qm1 = qb;
         host::bn_sub(k->e_qm1, one);
         k->crt = true;
+        // the default-window CRT schedules now, so that the allocation-free
+        // entry point (chx_paillier_decrypt_crt_noalloc) finds them on the device
+        const int w = host::default_window(std::max(exp_bits_p(k.get(), 2), exp_bits_p(k.get(), 3)));
+        (void)get_sched_p(k.get(), 2, w);
+        (void)get_sched_p(k.get(), 3, w);
       }
       k->lz_n2 = make_lazy_set(k->n2, ctx->stream);
       k->d = dev_upload(pd.get(), 1, ctx->stream);
@@ -1903,6 +1995,27 @@ int chx_paillier_decrypt_crt_dev(chx_ctx* ctx, const chx_pkey* kc, const uint32_
     if (count == 0) return;
     check_stride(count, stride);
     launch_crt(k, c, m, count, stride);
+  });
+}
+
+int chx_paillier_decrypt_crt_noalloc(chx_ctx* ctx, const chx_pkey* kc, const uint32_t* c, uint32_t* m,
+                                     size_t count, size_t stride) {
+  chx_pkey* k = const_cast<chx_pkey*>(kc);
+  return with_ctx(ctx, [&] {
+    if (!k || !c || !m) fail(CHX_EINVAL, "null argument");
+    if (k->ctx != ctx) fail(CHX_EINVAL, "key belongs to another context");
+    if (!k->crt) fail(CHX_EINVAL, "key was created without CRT factors");
+    if (count == 0) return;
+    check_stride(count, stride);
+    const FixedScratch fs{scratch_ptr(ctx->device), kNoAllocScratchBytes / sizeof(uint32_t)};
+    // HIP-event timing would create events: off for this call
+    struct TimingOff {
+      chx_ctx* c;
+      bool was;
+      ~TimingOff() { c->timing = was; }
+    } off{ctx, ctx->timing};
+    ctx->timing = false;
+    launch_crt(k, c, m, count, stride, nullptr, nullptr, nullptr, &fs);
   });
 }
 

@@ -212,6 +212,19 @@ int chx_paillier_decrypt_dev(chx_ctx* ctx, const chx_pkey* k, const uint32_t* c,
 int chx_paillier_decrypt_crt(chx_ctx* ctx, const chx_pkey* k, const uint32_t* c, uint32_t* m, size_t count);
 int chx_paillier_decrypt_crt_dev(chx_ctx* ctx, const chx_pkey* k, const uint32_t* c, uint32_t* m,
                                  size_t count, size_t stride);
+/* Allocation-free form of chx_paillier_decrypt_crt_dev (same arguments, same
+ * results). No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). It calls no hipMalloc or hipFree: its
+ * scratch (the CRT residues and per-item window tables, about 20 KB per
+ * ciphertext at n 2048-bit, 5 KB at n 1024-bit) comes from one 64 MiB device
+ * array per GPU, held until the process ends, and a batch needing more is
+ * refused with CHX_EINVAL. It uses the default window (context tuning of the
+ * window is ignored) and the schedules built with the key. Calls on one GPU
+ * share the array: calls that overlap on one GPU (several contexts or
+ * streams) race. HIP-event timing is off for the call. */
+int chx_paillier_decrypt_crt_noalloc(chx_ctx* ctx, const chx_pkey* k, const uint32_t* c, uint32_t* m,
+                                     size_t count, size_t stride);
 /* fused decrypt + odd-bit popcount (decode_hamming_kernel cu:592-668, never
  * launched there; decode_hamming_client cu:995-1038): counts[i] =
  * popcount(m[i] & mask), mask = `width` limbs. */
