@@ -186,7 +186,9 @@ struct chx_mod {
   ModDev* d = nullptr;
   LazySet lz;
   uint32_t* d_fixb = nullptr;  // served-multiply constants 2^(RB(i+2)) R' mod N (built on first use)
-  std::mutex mu;               // guards d_fixb: a modulus may serve several contexts of its device
+  uint32_t* d_pwz = nullptr;   // fold table of the pointwise multiply (built on first use)
+  int pw_h = -1, pw_nz = 0;    // its first folded position and row count; pw_h -2: not usable
+  std::mutex mu;               // guards d_fixb and d_pwz: a modulus may serve several contexts of its device
 };
 
 struct ExpSched {
@@ -596,15 +598,15 @@ size_t lazy_lds_bytes() {
     else fail(CHX_EUNSUPPORTED, "no lazy kernel for %d limbs (configuration %d)", l__, i__);   \
   } while (0)
 
-// One launch exponentiating one or two batches (CRT halves) of `count` items.
-// jobs[j]: input/output interleaved 32-bit buffers, schedule; the modulus
-// block of the chosen configuration is taken from sets[j].
-// in_hi (optional, per job): upper L32 limbs of double-width inputs
 // Window-table words of one job of run_lazy (configuration ci, window w).
 size_t lazy_table_words(int L32, int ci, int window, size_t count) {
   return ((size_t)1 << (window - 1)) * lazy_cfgs(L32)[ci].lp * round_up(count, BLOCK_THREADS);
 }
 
+// One launch exponentiating one or two batches (CRT halves) of `count` items.
+// jobs[j]: input/output interleaved 32-bit buffers, schedule; the modulus
+// block of the chosen configuration is taken from sets[j].
+// in_hi (optional, per job): upper L32 limbs of double-width inputs.
 // tab: the window tables' scratch (njobs * lazy_table_words words), or
 // nullptr for the context's slot.
 void run_lazy(chx_ctx* ctx, int L32, const LazySet* sets, const ExpSched* const* scheds, const uint32_t* const* in,
@@ -666,12 +668,100 @@ void launch_mulmod(chx_ctx* ctx, const ModDev* md, int L, const uint32_t* a, con
 // below that or when the context forces its lane count.
 constexpr size_t kLazyMulmodMin = 8192;
 
-void launch_mulmod_mod(chx_ctx* ctx, const chx_mod* m, const uint32_t* a, const uint32_t* b, uint32_t* out,
+// workgroups of k_fixed_mulmod: enough to fill every CU at the occupancy its
+// LDS and registers allow (the kernel strides over the tiles)
+extern "C++" {
+template <class F>
+static unsigned fixed_blocks(chx_ctx* ctx, F kernel, int threads, size_t lds, size_t tiles) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds) != hipSuccess || per_cu < 1)
+    per_cu = 1;
+  const size_t want = (size_t)device_cus(ctx->device) * per_cu;
+  return (unsigned)std::max<size_t>(1, std::min(tiles, want));
+}
+}
+
+// The one-reduction pointwise multiply (k_pw_mulmod) runs at the lazy
+// configuration of each width with 19 positions per chunk.
+#define CHX_DISPATCH_PW(L32_, ...)                                                                 \
+  do {                                                                                            \
+    const int l__ = (L32_);                                                                       \
+    if (l__ == 32) { constexpr int RB = 28, LP = 38, TPI = 2, L32 = 32; __VA_ARGS__; }             \
+    else if (l__ == 64) { constexpr int RB = 28, LP = 76, TPI = 4, L32 = 64; __VA_ARGS__; }        \
+    else if (l__ == 128) { constexpr int RB = 27, LP = 152, TPI = 8, L32 = 128; __VA_ARGS__; }     \
+    else fail(CHX_EUNSUPPORTED, "no pointwise kernel for %d limbs", l__);                         \
+  } while (0)
+
+int pw_cfg(int L32) { return L32 == 32 ? 1 : 0; }  // index in lazy_cfgs of the shape above
+
+// Fold table of k_pw_mulmod: rows r < nz hold Z = 2^(RB (h + r + 2)) mod N as
+// canonical positions; the two rows 2g, 2g + 1 of chunk w (positions
+// [w K, w K + K)) are 2 CH consecutive words at ((g TPI + w) 2 CH) (CH = K
+// rounded up to 4, zero padding). h = 2 LP - nz, with 2^(RB h) <= N / 2
+// (h <= (bits(N) - 2) / RB) and nz a multiple of 4; usable when nz <= 2 LP -
+// 12 TPI - 2 (the kernel's LDS rows), i.e. for moduli down to about half the
+// width.
+// Returns false when the modulus cannot take the kernel.
+bool pw_constants(chx_ctx* ctx, chx_mod* m) {
+  std::lock_guard<std::mutex> lock(m->mu);
+  if (m->pw_h == -2) return false;
+  if (m->d_pwz) return true;
+  const int L = m->hc.L;
+  if (lidx(L) < 0) {
+    m->pw_h = -2;
+    return false;
+  }
+  const LazyCfg c = lazy_cfgs(L)[pw_cfg(L)];
+  const int k = c.lp / c.tpi, ch = (k + 3) / 4 * 4;
+  const int h0 = (host::bn_bits(m->hc.N) - 2) / c.rb;
+  const int nz = (2 * c.lp - h0 + 3) / 4 * 4;
+  const int h = 2 * c.lp - nz;
+  if (h0 < 0 || h < 0 || nz > 2 * c.lp - 12 * c.tpi - 2) {
+    m->pw_h = -2;
+    return false;
+  }
+  std::vector<uint32_t> tab((size_t)nz * c.tpi * ch, 0u), p(c.lp);
+  BN one(L, 0u);
+  one[0] = 1;
+  BN z = host::bn_shl_mod(one, c.rb * (h + 2), m->hc.N);
+  for (int r = 0; r < nz; ++r) {
+    to_positions(z, c.rb, c.lp, p.data());
+    for (int q = 0; q < c.lp; ++q) tab[((size_t)(r / 2) * c.tpi + q / k) * 2 * ch + (r % 2) * ch + q % k] = p[q];
+    z = host::bn_shl_mod(z, c.rb, m->hc.N);
+  }
+  m->d_pwz = dev_upload(tab.data(), tab.size(), ctx->stream);
+  m->pw_h = h;
+  m->pw_nz = nz;
+  return true;
+}
+
+void launch_mulmod_mod(chx_ctx* ctx, const chx_mod* mc, const uint32_t* a, const uint32_t* b, uint32_t* out,
                        size_t count, size_t stride) {
+  chx_mod* m = const_cast<chx_mod*>(mc);
   const int L = m->hc.L;
   const int li = lidx(L);
   if (li < 0 || count < kLazyMulmodMin || ctx->tune_tpi[CHX_OP_MULMOD][li] != 0) {
     launch_mulmod(ctx, m->d, L, a, b, out, count, stride);
+    return;
+  }
+  // window 2 of CHX_OP_MULMOD selects the two-product form below
+  if (ctx->tune_window[CHX_OP_MULMOD][li] != 2 && pw_constants(ctx, m)) {
+    const size_t tiles = (count + FIX_TILE - 1) / FIX_TILE;
+    record_start(ctx, "pw_mulmod");
+    CHX_DISPATCH_PW(L, {
+      auto kern = k_pw_mulmod<RB, LP, TPI, L32>;
+      const size_t lds = (size_t)PwLds<LP, TPI>::WORDS * 4;
+      static std::once_flag lds_attr;
+      std::call_once(lds_attr, [&] {
+        hip_check(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                  "hipFuncSetAttribute");
+      });
+      hipLaunchKernelGGL(kern, dim3(fixed_blocks(ctx, kern, 64 * TPI, lds, tiles)), dim3(64 * TPI), lds,
+                         ctx->stream, m->lz.d[pw_cfg(L)], m->d_pwz, m->pw_h, m->pw_nz, a, b, out, (int)count,
+                         (int)stride);
+    });
+    record_stop(ctx);
+    check_launch("k_pw_mulmod");
     return;
   }
   const int ci = pick_lazy(ctx, L, count);
@@ -1120,6 +1210,7 @@ int chx_ctx_set_tuning(chx_ctx* ctx, int op, int limbs, int tpi, int window) {
     }
     if (window < 0 || window > 7) fail(CHX_EINVAL, "window %d out of range [1, 7]", window);
     if (op == CHX_OP_LOOKUP && window > 4) fail(CHX_EINVAL, "lookup rows per product %d out of range [1, 4]", window);
+    if (op == CHX_OP_MULMOD && window > 2) fail(CHX_EINVAL, "mulmod form %d out of range [1, 2]", window);
     ctx->tune_tpi[op][li] = tpi;
     ctx->tune_window[op][li] = window;
   });
@@ -1176,6 +1267,7 @@ int chx_mod_destroy(chx_mod* m) {
     hip_check(hipSetDevice(m->ctx->device), "hipSetDevice");
     if (m->d) (void)hipFree(m->d);
     if (m->d_fixb) (void)hipFree(m->d_fixb);
+    if (m->d_pwz) (void)hipFree(m->d_pwz);
     free_lazy_set(m->lz);
   });
   delete m;
@@ -1279,18 +1371,6 @@ int chx_mulmod_operands(chx_ctx* ctx, const chx_mod* mc, const uint32_t* y, uint
   });
 }
 
-// workgroups of k_fixed_mulmod: enough to fill every CU at the occupancy its
-// LDS and registers allow (the kernel strides over the tiles)
-extern "C++" {
-template <class F>
-static unsigned fixed_blocks(chx_ctx* ctx, F kernel, int threads, size_t lds, size_t tiles) {
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds) != hipSuccess || per_cu < 1)
-    per_cu = 1;
-  const size_t want = (size_t)device_cus(ctx->device) * per_cu;
-  return (unsigned)std::max<size_t>(1, std::min(tiles, want));
-}
-}
 
 int chx_mulmod_multi_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* x, size_t stride, size_t begin,
                          size_t count, int nq, const uint32_t* const* ops, uint32_t* const* outs,

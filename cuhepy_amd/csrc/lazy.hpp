@@ -1361,6 +1361,235 @@ __global__ __launch_bounds__(64 * TPI) __attribute__((amdgpu_waves_per_eu(4, 8))
 }
 
 // ---------------------------------------------------------------------------
+// Pointwise multiply with one reduction: out[i] = a[i] b[i] mod N for any
+// a, b < 2^(32 L32) (the evaluator's ciphertext multiply, encode_hamming_server:
+// cu:672-732, lcu:1505-1561)
+// ---------------------------------------------------------------------------
+// A workgroup of TPI waves takes tiles of 64 items (one per lane), as
+// k_fixed_mulmod; K = LP / TPI positions per chunk.
+//  1  each wave converts its K positions of a and b into LDS (as, bs);
+//  2  product T = a b in 2 LP positions: wave w sums chunks w and TPI + w
+//     (the triangle of the schoolbook product gives them w + 1 and TPI - w
+//     blocks of K rows, so every wave does LP^2 / TPI multiply-adds);
+//  3  each chunk ripples to RB-bit positions and hands its carry to the chunk
+//     above, which adds it at its bottom position (settled positions, below
+//     2^RB + 2^(64 - 2RB) + 1);
+//  4  fold: with 2^(RB h) <= N / 2, V = T_low 2^(2RB) + sum_{t >= h} T_t Z_t,
+//     Z_t = 2^(RB (t + 2)) mod N (a per-modulus table, canonical positions,
+//     read with scalar loads as the query table of k_fixed_mulmod), is
+//     = a b 2^(2RB) (mod N) and below N 2^(2RB) (1/2 + nz 2^(1 - RB)):
+//     T_low 2^(2RB) is T's positions below h shifted up by two, and
+//     the nz rows t >= h add nz multiply-adds per position;
+//  5  the finish of k_fixed_mulmod (two REDC rows, exact positions, limbs,
+//     one conditional subtraction): R = (V + m N) / 2^(2RB) < 2N.
+// About 2 LP^2 multiply-adds per item (LP^2 + nz LP), against the 4 LP^2 of
+// two Montgomery products.
+// LDS (u32 words, one region of 2 LP rows of 64): as rows [0, LP), bs rows
+// [LP, 2 LP) for steps 1-2; then the folded positions ts rows [0, nz), the
+// finish's exchange areas rows [FB, FB + 2 + 6 TPI), the chunk boundary
+// positions bd rows [2 LP - 6 TPI, 2 LP - 4 TPI) and the chunk carries cx
+// (u64) rows [2 LP - 4 TPI, 2 LP). The host guarantees nz <= FB, so the
+// finish can start while other waves still fold.
+// The fold table (nz rows of LP positions) is grouped for scalar loads: the
+// two rows 2g, 2g + 1 of wave w's chunk are 2 CH consecutive words at
+// ((g TPI + w) 2 CH) (CH = K rounded up to 4, zero padded).
+template <int LP, int TPI>
+struct PwLds {
+  static constexpr int WORDS = 2 * LP * FIX_TILE;
+  static constexpr int FB = 2 * LP - 12 * TPI - 2;  // first row of the finish's areas
+  static constexpr int BD = (2 * LP - 6 * TPI) * FIX_TILE;
+  static constexpr int CX = (2 * LP - 4 * TPI) * FIX_TILE;
+  // fixed_finish addresses its areas from XS = LP rows past the pointer it gets
+  static constexpr int FIN = (FB - LP) * FIX_TILE;
+  static_assert(FB >= LP, "finish areas below the as rows' end");
+  static_assert(FixedLds<LP, TPI>::WORDS - FixedLds<LP, TPI>::XS <= (2 * LP - 6 * TPI - FB) * FIX_TILE,
+                "finish areas overlap the boundary rows");
+};
+
+// K rows of a product chunk (step 2). Row s multiplies a_i (i = the block's
+// first row + s) by the window b_(j0 - s + p), p < K, held in a ring of K
+// registers: slot (p - s) mod K; the value entering at row s (p = 0) is read
+// from bq[(K - 1 - s) * FIX_TILE] (bq: the entering value of the block's last
+// row). MODE 0: every entering value is read. MODE 1 (last block of a low
+// chunk, j0 = 0): values entering after row 0 lie below b_0, so the terms
+// p < s are skipped. MODE 2 (first block of a high chunk, j0 = LP): the value
+// entering at row 0 lies above b_(LP-1), and terms p >= s are skipped.
+template <int K, int MODE>
+__device__ __forceinline__ void pw_block(u64 (&acc)[K], u32 (&ring)[K], const u32* ap, const u32* bq) {
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    const u32 a = ap[s * FIX_TILE];
+    const int slot = (K - s) % K;
+    if (MODE == 0 || (MODE == 1 && s == 0) || (MODE == 2 && s > 0)) ring[slot] = bq[(K - 1 - s) * FIX_TILE];
+#pragma unroll
+    for (int p = 0; p < K; ++p) {
+      if (MODE == 1 && p < s) continue;
+      if (MODE == 2 && p >= s) continue;
+      acc[p] = (u64)a * ring[(p - s + K) % K] + acc[p];
+    }
+  }
+}
+
+// Positions [c K, c K + K) of the product (chunk c; LOW: c < TPI) of the
+// operands in LDS (as_l, bs_l: column of this lane, rows of FIX_TILE words).
+template <int LP, int TPI, bool LOW>
+__device__ __forceinline__ void pw_chunk(u64 (&acc)[LP / TPI], const u32* as_l, const u32* bs_l, int c) {
+  constexpr int K = LP / TPI;
+#pragma unroll
+  for (int p = 0; p < K; ++p) acc[p] = 0;
+  u32 ring[K];
+  // rows i from i0; the value entering at row i0 is b_(j0), j0 = c K - i0
+  const int i0 = LOW ? 0 : (c - TPI) * K;
+  const int j0 = LOW ? c * K : LP;
+  const u32* ap = as_l + i0 * FIX_TILE;
+  const u32* bq = bs_l + (j0 - (K - 1)) * FIX_TILE;
+  if constexpr (LOW) {
+#pragma unroll
+    for (int p = 1; p < K; ++p) ring[p] = bs_l[(j0 + p) * FIX_TILE];  // b_(cK + p), slot p
+#pragma unroll 1
+    for (int blk = 0; blk < c; ++blk) {
+      pw_block<K, 0>(acc, ring, ap, bq);
+      ap += K * FIX_TILE;
+      bq -= K * FIX_TILE;
+    }
+    pw_block<K, 1>(acc, ring, ap, bq);
+  } else {
+#pragma unroll
+    for (int p = 0; p < K; ++p) ring[p] = 0;  // never read before written (MODE 2 skips them)
+    pw_block<K, 2>(acc, ring, ap, bq);
+#pragma unroll 1
+    for (int blk = 1; blk < 2 * TPI - c; ++blk) {
+      ap += K * FIX_TILE;
+      bq -= K * FIX_TILE;
+      pw_block<K, 0>(acc, ring, ap, bq);
+    }
+  }
+}
+
+template <int RB, int K>
+__device__ __forceinline__ u64 pw_ripple(u32 (&x)[K], const u64 (&acc)[K]) {
+  constexpr u32 MASK = (1u << RB) - 1u;
+  u64 c = 0;
+#pragma unroll
+  for (int p = 0; p < K; ++p) {
+    const u64 t = acc[p] + c;
+    x[p] = (u32)t & MASK;
+    c = t >> RB;
+  }
+  return c;
+}
+
+template <int RB, int K>
+__device__ __forceinline__ void pw_carry_in(u32 (&x)[K], u64 cin) {
+  constexpr u32 MASK = (1u << RB) - 1u;
+  const u64 t = (u64)x[0] + cin;
+  x[0] = (u32)t & MASK;
+  x[1] += (u32)(t >> RB);
+}
+
+template <int RB, int LP, int TPI, int L32>
+__global__ __launch_bounds__(64 * TPI) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_pw_mulmod(
+    const LazyDev* __restrict__ md, const u32* __restrict__ Z, int h, int nz, const u32* __restrict__ A,
+    const u32* __restrict__ B, u32* __restrict__ out, int count, int stride) {
+  extern __shared__ uint4 smem4[];
+  constexpr int K = LP / TPI, CH = FixedShape<K>::CH;
+  static_assert(LP % TPI == 0 && K >= 2, "chunks of at least two positions");
+  typedef PwLds<LP, TPI> S;
+  u32* lds = reinterpret_cast<u32*>(smem4);
+  u32* as = lds;
+  u32* bs = lds + LP * FIX_TILE;
+  u32* ts = lds;
+  u32* bd = lds + S::BD;
+  u64* cx = reinterpret_cast<u64*>(lds + S::CX);
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
+  const int lane = lane_id();
+  const int tiles = (count + FIX_TILE - 1) / FIX_TILE;
+  for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    const int base = tile * FIX_TILE;
+    {  // 1: positions of a and b
+      const int local = base + lane < count ? base + lane : count - 1;
+      const size_t so = (size_t)opaque(stride);
+      fixed_positions<RB, LP, TPI, L32>(wave, A + local, so, as + lane);
+      fixed_positions<RB, LP, TPI, L32>(wave, B + local, so, bs + lane);
+    }
+    __syncthreads();
+    // 2: product chunks wave and TPI + wave
+    u32 xl[K], xh[K];
+    u64 cl, chi;
+    {
+      u64 acc[K];
+      pw_chunk<LP, TPI, true>(acc, as + lane, bs + lane, wave);
+      cl = pw_ripple<RB, K>(xl, acc);
+      // the low chunk leaves its 64-bit accumulators here: K words stay live
+      // across the high chunk instead of 2K
+#pragma unroll
+      for (int p = 0; p < K; ++p) asm volatile("" : "+v"(xl[p]));
+      asm volatile("" : "+v"(cl));
+      __builtin_amdgcn_sched_barrier(0);
+      pw_chunk<LP, TPI, false>(acc, as + lane, bs + lane, TPI + wave);
+      chi = pw_ripple<RB, K>(xh, acc);  // the top chunk's carry is 0: a b < 2^(RB 2 LP)
+    }
+    __syncthreads();  // as, bs read by every wave
+    // 3: chunk carries
+    cx[wave * FIX_TILE + lane] = cl;
+    cx[(TPI + wave) * FIX_TILE + lane] = chi;
+    __syncthreads();
+    if (wave > 0) pw_carry_in<RB, K>(xl, cx[(wave - 1) * FIX_TILE + lane]);
+    pw_carry_in<RB, K>(xh, cx[(TPI + wave - 1) * FIX_TILE + lane]);
+#pragma unroll
+    for (int p = 0; p < K; ++p) {
+      const int t = wave * K + p;
+      if (t >= h) ts[(t - h) * FIX_TILE + lane] = xl[p];
+      ts[(LP + t - h) * FIX_TILE + lane] = xh[p];
+    }
+    bd[(2 * wave) * FIX_TILE + lane] = xl[K - 2];
+    bd[(2 * wave + 1) * FIX_TILE + lane] = xl[K - 1];
+    __syncthreads();
+    // 4: V positions [wave K, wave K + K)
+    u64 acc[K];
+#pragma unroll
+    for (int p = 0; p < K; ++p) {
+      const int t = wave * K + p - 2;  // T_t shifted up by two positions
+      u32 v;
+      if (p >= 2)
+        v = xl[p - 2];
+      else
+        v = wave > 0 ? bd[(2 * (wave - 1) + p) * FIX_TILE + lane] : 0u;
+      acc[p] = (t >= 0 && t < h) ? (u64)v : 0ull;
+    }
+    {
+      // rows in batches of 4 (two pairs of the table's layout): the batch's
+      // scalar loads and its ts reads are issued together, so one wait covers
+      // them all (a scalar load can return out of order).
+      const cu32* Zw = (const cu32*)(Z + wave * 2 * CH);
+#pragma unroll 1
+      for (int r0 = 0; r0 < nz; r0 += 4) {  // nz is a multiple of 4
+        u32 zv[4][K], xv[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const cu32* zr = Zw + (size_t)((r0 + j) >> 1) * (TPI * 2 * CH) + (j & 1) * CH;
+#pragma unroll
+          for (int p = 0; p < K; ++p) zv[j][p] = zr[p];
+          xv[j] = ts[(r0 + j) * FIX_TILE + lane];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+#pragma unroll
+          for (int p = 0; p < K; ++p) acc[p] = (u64)xv[j] * zv[j][p] + acc[p];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    // 5: reduction, exact positions, canonical limbs, store (the finish's
+    // areas lie above every ts row: no barrier before it)
+    fixed_finish<RB, LP, TPI, L32>(wave, md, acc, lds + S::FIN, lane, out + base + lane, (size_t)opaque(stride),
+                                   base + lane < count);
+    __syncthreads();  // the next tile's positions overwrite the finish's areas
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Ragged batches: one persistent launch over many sub-batches of mixed width
 // ---------------------------------------------------------------------------
 struct RaggedSub {

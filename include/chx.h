@@ -72,7 +72,11 @@ int chx_ctx_sync(chx_ctx* ctx);
  * covers: 1 = single rows, 2 = row pairs, 3 = row triples, 4 = row quads
  * (16^R-entry tables per R rows, built on first use; the next smaller group
  * when the memory for one cannot be allocated); groups are used for waves
- * whose digits are all in the bit-packing alphabet; 0 = 4. */
+ * whose digits are all in the bit-packing alphabet; 0 = 4. For CHX_OP_MULMOD
+ * the window selects the form of the pointwise multiply of batches of 8,192
+ * or more: 0 or 1 = one product and one fold (k_pw_mulmod), 2 = two
+ * Montgomery products (k_lazy_mulmod); a forced tpi selects the canonical
+ * engine at any size. Results are identical. */
 int chx_ctx_set_tuning(chx_ctx* ctx, int op, int limbs, int tpi, int window);
 /* lanes-per-bignum values built for `limbs` (returns count, fills up to cap);
  * op CHX_OP_MODEXP lists the exponentiation engine's values, other ops the

@@ -29,6 +29,18 @@ def fixed_mulmod_model(N, x, y, RB, LP, T, L32, drop_top_term=False):
     Yp = [positions((y << (RB * (i + 2))) % N, LP) for i in range(LP)]
     # B: chunk W sums positions [W K, W K + K) of V = sum_i x_i Y_i
     acc = [[sum(xs[i] * Yp[i][W * K + p] for i in range(LP)) for p in range(K)] for W in range(T)]
+    return finish_model(N, acc, RB, LP, T, L32, drop_top_term)
+
+
+def finish_model(N, acc, RB, LP, T, L32, drop_top_term=False):
+    """fixed_finish_w: chunk W's K accumulators of V (acc[W][p], V < 2N 2^(2RB))
+    -> canonical V / 2^(2RB) mod N."""
+    K = LP // T
+    M = (1 << RB) - 1
+    n0 = (-pow(N, -1, 1 << RB)) % (1 << RB)
+    Np = [(N >> (RB * p)) & M for p in range(LP)]
+    N32 = [(N >> (32 * j)) & 0xFFFFFFFF for j in range(L32)]
+    acc = [list(a) for a in acc]
     # 1: REDC digits from positions 0 and 1
     m1 = (acc[0][0] * n0) & M
     t0 = acc[0][0] + m1 * Np[0]

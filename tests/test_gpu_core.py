@@ -15,6 +15,9 @@ from oracle import paillier_oracle as O
 pytestmark = pytest.mark.gpu
 
 OP_MODEXP, OP_MULMOD, OP_LOOKUP = 0, 1, 2
+# shape (RB, LP, waves) of k_pw_mulmod per width, and the host's fold-row rule
+PW_SHAPE = {32: (28, 38, 2), 64: (28, 76, 4), 128: (27, 152, 8)}
+from test_pw_model import fold_rows  # noqa: E402
 
 
 @pytest.fixture(autouse=True)
@@ -59,24 +62,41 @@ def test_mulmod_every_variant(ctx, limbs):
 
 @pytest.mark.parametrize("limbs", [32, 64, 128])
 def test_mulmod_lazy_large_batches(ctx, limbs):
-    """Batches of 8,192 or more items go to the lazy-engine multiply (two lazy
-    products, the last modulo N, then a subtraction that takes the bit above
-    the limbs): full-width, smaller and near-2^(32 L) moduli, operands up to
-    2^(32 L) - 1, both lazy configurations of the width."""
+    """Batches of 8,192 or more items: the one-reduction multiply (k_pw_mulmod:
+    product, fold by the modulus's table, two REDC rows) and the two-product
+    form (window 2; both lazy configurations of the width). Full-width
+    moduli, moduli a few and 20 bits short of the width (more fold rows),
+    64 bits short, 3/4 of the width; half the width and 9 L bits (the fold
+    rows do not fit: the two-product form); near 2^(32 L); operands up to
+    2^(32 L) - 1."""
     from cuhepy_amd.engine import Context, Modulus
     from cuhepy_amd.limbs import ints_to_limbs, limbs_to_ints
     rng = random.Random(300 + limbs)
     R = 1 << (32 * limbs)
     count = 9000
-    for tpi in Context.supported_tpi(limbs, op=OP_MODEXP):
+    moduli = [_mod(rng, limbs, True), _mod(rng, limbs, False), R - 2 * rng.randrange(1, 1 << 20) - 1,
+              (1 << (32 * limbs - 20)) + 2 * rng.getrandbits(32 * limbs - 30) + 1,
+              (1 << (32 * limbs - 64)) + 2 * rng.getrandbits(32 * limbs - 70) + 1,
+              (1 << (24 * limbs)) + 2 * rng.getrandbits(24 * limbs - 8) + 1,  # 3/4 of the width
+              (1 << (16 * limbs)) + 2 * rng.getrandbits(16 * limbs - 8) + 1,  # half: two-product form
+              (1 << (9 * limbs)) + 2 * rng.getrandbits(9 * limbs - 8) + 1]  # too short for the fold rows
+    assert fold_rows(moduli[0], *PW_SHAPE[limbs]) and not fold_rows(moduli[-1], *PW_SHAPE[limbs])
+    forms = [(0, 0)] + [(2, t) for t in Context.supported_tpi(limbs, op=OP_MODEXP)]
+    for window, tpi in forms:
         ctx.set_tuning(OP_MODEXP, limbs, tpi, 0)
-        for n in (_mod(rng, limbs, True), _mod(rng, limbs, False), R - 2 * rng.randrange(1, 1 << 20) - 1):
+        ctx.set_tuning(OP_MULMOD, limbs, 0, window)
+        for k, n in enumerate(moduli):
             mod = Modulus(ctx, n, limbs)
             a = [0, 1, n - 1, R - 1, n] + [rng.randrange(R) for _ in range(count - 5)]
             b = [n - 1, R - 1, n - 1, R - 1, 2] + [rng.randrange(R) for _ in range(count - 5)]
+            ctx.reset_timing()
+            ctx.set_timing(True)
             got = limbs_to_ints(mod.mulmod(ints_to_limbs(a, limbs), ints_to_limbs(b, limbs)))
+            ctx.set_timing(False)
+            one_fold = window == 0 and fold_rows(n, *PW_SHAPE[limbs]) is not None
+            assert ctx.kernel_time("pw_mulmod")[1] == int(one_fold), (window, k)
             bad = [i for i, (g, x, y) in enumerate(zip(got, a, b)) if g != x * y % n]
-            assert not bad, f"tpi={tpi} n bits {n.bit_length()}: first bad index {bad[:5]}"
+            assert not bad, f"window={window} tpi={tpi} n bits {n.bit_length()}: first bad index {bad[:5]}"
 
 
 @pytest.mark.parametrize("limbs", [32, 64, 128])
