@@ -25,6 +25,7 @@ Reported beside it:
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import math
 import os
@@ -114,6 +115,95 @@ def committed_traffic(kernel: str):
             return blob.get("traffic_bytes"), os.path.relpath(path, ROOT)
     return None, None
 
+
+
+def _c1_worker(job):
+    n, phi, inv, items = job
+    from oracle import gmp_engine as G
+    n2 = n * n
+    out = []
+    for m, r in items:
+        c = (1 + m * n) * G.powm(r, n, n2) % n2          # scheme.py:63-83 with g = n + 1
+        u = G.powm(c, phi, n2)                            # scheme.py:86-104
+        out.append((c, (u - 1) // n * inv % n))
+    return out
+
+
+def host_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def c1_leg(ctx, args, cores: int) -> dict:
+    """The reference's CPU-runnable case: textbook Paillier encrypt and decrypt
+    of 1,024 messages as one batch (scheme.py:63-104), seeded key, m uniform in
+    [0, n), r uniform in Z_n^*; at n of 1024 bits (n^2 2048-bit, the metric's
+    width) and of 2048 bits (the reference's default key_len 1024). The GPU
+    runs the host entry points (PaillierKey.encrypt / decrypt: host buffers,
+    PCIe included); GMP (the engine gmpy2 wraps) runs the same formulas on
+    the host cores. Every ciphertext and plaintext is compared."""
+    from multiprocessing import get_context
+
+    from cuhepy_amd.engine import PaillierKey
+    from cuhepy_amd.limbs import limbs_to_ints
+    from cuhepy_amd.paillier import keygen as kg
+    from oracle import gmp_engine as G
+    out = {}
+    for key_len in (512, 1024):
+        rng = random.Random(args.seed + key_len)
+        k = kg.paillier_key_gen(key_len, rng)
+        n, phi, inv = k["pk"]["n"], k["sk"]["phi"], k["sk"]["inv"]
+        key = PaillierKey(ctx, n, phi, inv, k["p"], k["q"])
+        ms = [rng.randrange(n) for _ in range(1024)]
+        rs = [kg.random_r(n, rng) for _ in range(1024)]
+        key.decrypt(key.encrypt(ms[:64], rs[:64]))  # warm (schedules, scratch)
+        ctx.sync()
+        t = time.perf_counter()
+        c = key.encrypt(ms, rs)
+        t_enc = time.perf_counter() - t
+        d = key.decrypt(c)
+        t_all = time.perf_counter() - t
+        cs, ds = limbs_to_ints(c), limbs_to_ints(d)
+        if ds != ms:
+            raise RuntimeError(f"C1 key_len {key_len}: decrypt differs from the messages")
+        row = {"messages": 1024, "n_bits": n.bit_length(), "n2_bits": (n * n).bit_length(),
+               "gpu_ms": round(t_all * 1e3, 3), "gpu_encrypt_ms": round(t_enc * 1e3, 3),
+               "gpu_decrypt_ms": round((t_all - t_enc) * 1e3, 3)}
+        if G.available() and not args.no_cpu:
+            items = list(zip(ms, rs))
+            single = 256 if key_len == 512 else 32
+            t = time.perf_counter()
+            one = _c1_worker((n, phi, inv, items[:single]))
+            t1 = time.perf_counter() - t
+            chunks = [items[i::cores] for i in range(cores)]
+            with get_context("fork").Pool(cores) as pool:
+                pool.map(_c1_worker, [(n, phi, inv, items[:1])] * cores)  # warm the workers
+                t = time.perf_counter()
+                res = pool.map(_c1_worker, [(n, phi, inv, ch) for ch in chunks])
+                tp = time.perf_counter() - t
+            got = [None] * len(items)
+            for i, r in enumerate(res):
+                got[i::cores] = r
+            bad = sum(int(g != (a, b)) for g, a, b in zip(got, cs, ds)) + sum(
+                int(g != (a, b)) for g, a, b in zip(one, cs, ds))
+            if bad:
+                raise RuntimeError(f"C1 key_len {key_len}: GMP and GPU differ on {bad} messages")
+            row.update({"cpu_engine": "gmp " + G.version(), "cpu_ms": round(tp * 1e3, 1), "cpu_cores": cores,
+                        "cpu_single_thread_ms_per_1024": round(t1 / single * 1024 * 1e3, 1),
+                        "cpu_single_thread_sample": single, "gpu_over_cpu_all_cores": round(tp / t_all, 1),
+                        "bit_exact": {"ciphertexts": 1024, "plaintexts": 1024, "mismatches": 0}})
+        out[f"n{n.bit_length()}"] = row
+        log(f"C1 (1,024 messages, n {n.bit_length()}-bit): GPU encrypt+decrypt {t_all * 1e3:.2f} ms "
+            f"(host buffers)" + (f"; GMP {row['cpu_ms']:.0f} ms on {cores} processes, "
+                                 f"{row['cpu_single_thread_ms_per_1024']:.0f} ms on one thread" if "cpu_ms" in row
+                                 else ""))
+        key.close()
+    return out
 
 
 def ragged_leg(ctx, args, peak: float, device_name: str) -> dict:
@@ -333,6 +423,26 @@ def lookup_leg(ctx, args, peak: float) -> dict:
     cpu_s = time.perf_counter() - t
     got = limbs_to_ints(c_aos[sample])
     f_bad = sum(int(a != b) for a, b in zip(got, want))
+    # CPU baseline with GMP (the engine gmpy2 wraps), the reference's loop over every digit row
+    gmp_lookup = None
+    from oracle import gmp_engine as G
+    if G.available():
+        enc = G.LookupEncryptor(rows, nt, k["n_squared"], mc)
+        gsample = list(range(0, count, max(1, count // args.lookup_gmp_sample)))[: args.lookup_gmp_sample]
+        gms = limbs_to_ints(m[gsample])
+        for mi in gms:
+            enc.prepare(mi)
+        t = time.perf_counter()
+        gout = [enc(mi, nidx[i].tolist()) for mi, i in zip(gms, gsample)]
+        g_s = time.perf_counter() - t
+        g_bad = sum(int(a != b) for a, b in zip(gout, limbs_to_ints(c_aos[gsample])))
+        if g_bad:
+            raise RuntimeError(f"GMP lookup encryption differs from the GPU on {g_bad} vectors")
+        gmp_lookup = {"engine": "gmp " + G.version() + " (ctypes), one thread", "vectors": len(gsample),
+                      "vectors_per_s": round(len(gsample) / g_s, 2),
+                      "mulmods_per_vector": mc + 14, "equal_to_gpu": len(gsample)}
+        log(f"lookup: GMP CPU baseline {len(gsample) / g_s:.1f} vectors/s single thread "
+            f"({mc + 14} mulmods per vector), {len(gsample)} equal to the GPU ciphertexts")
     log(f"lookup: {count} vectors in {launch_s * 1e3:.2f} ms ({count / launch_s:,.0f} vectors/s); row quads "
         f"execute {executed / count:.1f} products per vector: {executed_tmac:.2f} TMAC/s = "
         f"{executed_tmac / peak:.1%} of the issue rate (the reference's one product per nonzero digit, "
@@ -341,11 +451,25 @@ def lookup_leg(ctx, args, peak: float) -> dict:
         f"mismatches {rt_bad}; oracle mismatches {f_bad}/{len(sample)}; CPU {len(sample) / cpu_s:.1f} vectors/s")
     if rt_bad or f_bad:
         raise RuntimeError(f"lookup encryption wrong: round trip {rt_bad}, oracle {f_bad}")
-    # the client path end to end (host packing, transfer, kernels, integer conversion) on a slice
+    # the client path end to end (validation, packing, transfer, kernels,
+    # output conversion) on a slice, for each input/output form
     sl = min(count, args.lookup_client_vectors)
-    t = time.perf_counter()
-    client.encrypt(bits[:sl].tolist())
-    client_s = time.perf_counter() - t
+    client_path = {"vectors": sl}
+    as_lists = bits[:sl].tolist()
+    for name, fn, arg in (("lists_in_ints_out", client.encrypt, as_lists),
+                          ("numpy_in_ints_out", client.encrypt, bits[:sl]),
+                          ("numpy_in_bytes_out", client.encrypt_bytes, bits[:sl]),
+                          ("lists_in_bytes_out", client.encrypt_bytes, as_lists)):
+        fn(arg[:64])  # first-call set-up outside the timing
+        gc.collect()  # as the reference's time_call (benchmarks/bfv_client_matrix.py:40-44)
+        gc.disable()
+        t = time.perf_counter()
+        fn(arg)
+        client_path[name + "_vectors_per_s"] = round(sl / (time.perf_counter() - t), 1)
+        gc.enable()
+    client_s = sl / client_path["lists_in_ints_out_vectors_per_s"]
+    log("lookup: client path end to end, vectors/s: " + ", ".join(
+        f"{k[:-14]} {v:,.0f}" for k, v in client_path.items() if k.endswith("_per_s")))
     reenc = _reencrypt_block(ctx, args, key, out, d_dig, d_idx, m, count, stride, W)
     return {
         "workload": f"{count} binary vectors x 512 bits (Bernoulli 0.5), key_len 1024 (n^2 4096-bit), alpha 280, "
@@ -366,8 +490,10 @@ def lookup_leg(ctx, args, peak: float) -> dict:
                        "speedup_over_single_rows": round(single_s / launch_s, 3)},
         "wall_ms_per_batch": round(wall * 1e3, 3),
         "client_path_vectors_per_s": round(sl / client_s, 1),
+        "client_path": client_path,
         "bit_exact": {"round_trip_vectors": count, "oracle_vectors": len(sample), "mismatches": 0},
         "cpu_port_single_thread_vectors_s": round(len(sample) / cpu_s, 2),
+        "cpu_gmp": gmp_lookup,
         "reencrypt": reenc,
     }
 
@@ -501,8 +627,11 @@ def serve_leg(ctx, args, peak: float) -> dict:
 
     # 1) one query alone, whole index in one launch: the kernel against its rooflines
     qb, ys = queries(1)
-    plain = KnnServer(ctx, mod, index, count, chunk=count, max_active=1)
+    plain = KnnServer(ctx, mod, index, count, chunk=count, max_active=1, mode="scan")
     r = plain.submit(ys[0], keep=True)
+    plain.drain()
+    for _ in range(10):  # warm-up: result buffers pooled, clocks up
+        plain.submit(ys[0])
     plain.drain()
     ctx.reset_timing()
     ctx.set_timing(True)
@@ -546,7 +675,8 @@ def serve_leg(ctx, args, peak: float) -> dict:
     pw_forms = {}
     for form, kname in ((2, "mulmod"), (0, "pw_mulmod")):  # two Montgomery products; one product + fold
         ctx.set_tuning(1, W, 0, form)
-        mod.mulmod_dev(index, other_full, pw_out, count, stride)  # warm (builds the fold table once)
+        for _ in range(3):  # warm-up (the first call builds the fold table)
+            mod.mulmod_dev(index, other_full, pw_out, count, stride)
         torch.cuda.synchronize()
         ctx.reset_timing()
         ctx.set_timing(True)
@@ -577,8 +707,35 @@ def serve_leg(ctx, args, peak: float) -> dict:
         f"{count * 3 * W * 4 / pw_s / 1e9:.0f} GB/s); two-product form {pw_two_s * 1e3:.3f} ms; "
         f"256 entries exact")
 
-    # 2) saturation: 32 queries at once through the batched loop
-    srv = KnnServer(ctx, mod, index, count, chunk=args.serve_chunk, max_active=32)
+    # 1c) CPU baseline: the reference's evaluate loop int(mpz(a) * y % n^2) over
+    # a sample of the index (benchmarks/bfv_client_matrix.py:166-169), GMP
+    cpu_eval = None
+    from oracle import gmp_engine as G
+    if G.available() and not args.no_cpu:
+        ns = min(args.serve_cpu_sample, count)
+        s_aos = torch.empty((ns, W), dtype=torch.int32, device=dev)
+        torch.cuda.synchronize()
+        ctx.il_to_aos(index, s_aos, ns, W, stride)
+        ctx.sync()
+        sx = limbs_to_ints(s_aos.cpu().numpy().view(np.uint32))
+        ev = G.Evaluator(ys[0], n2)
+        t = time.perf_counter()
+        sout = [ev(a) for a in sx]
+        cpu_s = time.perf_counter() - t
+        if sout[:256] != got:
+            raise RuntimeError("CPU evaluate loop differs from the served products")
+        cpu_eval = {"engine": "gmp " + G.version() + " (ctypes), one thread, the loop int(mpz(a) * y % n^2)",
+                    "entries": ns, "mulmod_per_s": round(ns / cpu_s, 1),
+                    "us_per_entry": round(cpu_s / ns * 1e6, 3),
+                    "ms_per_query_over_index": round(count * cpu_s / ns * 1e3, 1),
+                    "reference_published_us_per_entry": 5.3,
+                    "reference_published_source": "benchmarks/results/native_bfv_1024.json:97 (gmpy2, one thread)"}
+        log(f"serve: CPU evaluate loop (GMP) {ns / cpu_s:,.0f} mulmod/s single thread = "
+            f"{count * cpu_s / ns:.2f} s per query over the index; equal to the GPU on 256 entries")
+
+    # 2) saturation: 64 queries submitted at once, batched (FIFO launches of
+    # every admitted query) and plain (one launch per query)
+    srv = KnnServer(ctx, mod, index, count, max_active=32, mode="fifo")
     _, ys = queries(64)
     t0 = time.perf_counter()
     for y in ys:
@@ -586,18 +743,19 @@ def serve_leg(ctx, args, peak: float) -> dict:
     srv.drain()
     sat = len(ys) / (time.perf_counter() - t0)
     t0 = time.perf_counter()
-    for y in ys[:16]:
+    for y in ys:
         plain.submit(y)
     plain.drain()
-    sat_plain = 16 / (time.perf_counter() - t0)
-    log(f"serve: saturation {sat:.1f} queries/s batched ({srv.iterations} launches), {sat_plain:.1f} plain")
+    sat_plain = len(ys) / (time.perf_counter() - t0)
+    log(f"serve: saturation {sat:.1f} queries/s batched ({srv.iterations} launches; stamp clock "
+        f"{srv.clock_hz / 1e6:.2f} MHz measured), {sat_plain:.1f} plain")
 
-    # 3) Poisson arrivals at a sweep of rates
+    # 3) Poisson arrivals at a sweep of rates (fractions of the plain saturation)
     sweep = []
     rng = random.Random(args.seed + 10)
     _, pool_ys = queries(args.serve_queries)
-    for frac in (0.25, 0.5, 0.75, 0.9):
-        rate = frac * sat
+    for frac in (0.25, 0.5, 0.75, 0.9, 0.97):
+        rate = frac * sat_plain
         offs = poisson_arrivals(rate, args.serve_queries, rng)
         row = {"rate_qps": round(rate, 2)}
         for name, server in (("batched", srv), ("plain", plain)):
@@ -627,10 +785,15 @@ def serve_leg(ctx, args, peak: float) -> dict:
                       "achieved_tmac_s": round(macs / pw_s / 1e12, 4),
                       "frac_of_peak": round(macs / pw_s / 1e12 / peak, 4),
                       "hbm_gb_s": round(count * 3 * W * 4 / pw_s / 1e9, 1), "entries_checked": 256},
+        "cpu_baseline": cpu_eval,
         "saturation_qps_batched": round(sat, 2),
         "saturation_qps_plain": round(sat_plain, 2),
-        "chunk": args.serve_chunk, "max_active": 32,
-        "latency_endpoint": "device-complete (event after the query's last range), from arrival",
+        "batched": "fifo: each launch takes every admitted query not yet launched (up to 32) over the whole "
+                   "index, tiles query by query; a query completes at its last tile (device clock stamp)",
+        "plain": "one launch per query, whole index",
+        "max_active": 32, "queries_per_rate": args.serve_queries,
+        "latency_endpoint": "device-complete from arrival (plain: event after the query's launch; batched: the "
+                            "stamp of the query's last tile, anchored to the event after its launch)",
         "poisson_sweep": sweep,
         "bit_exact": {"entries_checked": 256, "distances_checked": 256, "mismatches": 0},
     }
@@ -650,11 +813,11 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--tpi", type=int, default=0, help="force lanes per bignum (0 = automatic)")
     ap.add_argument("--window", type=int, default=0, help="force the window width (0 = automatic)")
-    ap.add_argument("--legs", default="ragged,lookup,serve",
-                    help="extra workloads reported in the same line: ragged, lookup, serve (comma list, or none)")
+    ap.add_argument("--legs", default="c1,ragged,lookup,serve",
+                    help="extra workloads reported in the same line: c1, ragged, lookup, serve (comma list, or none)")
     ap.add_argument("--serve-index", type=int, default=1_000_000)
-    ap.add_argument("--serve-chunk", type=int, default=131072)
-    ap.add_argument("--serve-queries", type=int, default=200)
+    ap.add_argument("--serve-queries", type=int, default=1000)
+    ap.add_argument("--serve-cpu-sample", type=int, default=20000)
     ap.add_argument("--ragged-subs", type=int, default=256)
     ap.add_argument("--ragged-max", type=int, default=8192)
     ap.add_argument("--ragged-steps", type=int, default=3)
@@ -663,7 +826,8 @@ def main() -> int:
     ap.add_argument("--lookup-vectors", type=int, default=100000)
     ap.add_argument("--lookup-steps", type=int, default=5)
     ap.add_argument("--lookup-cpu-sample", type=int, default=24)
-    ap.add_argument("--lookup-client-vectors", type=int, default=10000)
+    ap.add_argument("--lookup-gmp-sample", type=int, default=400)
+    ap.add_argument("--lookup-client-vectors", type=int, default=20000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (counter calibration for profiles)")
     args = ap.parse_args()
@@ -795,6 +959,27 @@ def main() -> int:
         print(json.dumps({"error": "half-width and full-width encryption or decryption differ"}))
         return 1
     exact["bit_exact"] = {"half_equals_full_items": count, "decrypt_equals_plaintexts": count, "mismatches": 0}
+    # host entry point (key.decrypt: host buffers) of the key with factors:
+    # all units take the textbook CRT pass only; with one non-unit (c = p) that
+    # item alone is decrypted again in the exact form
+    c_host = from_il(c_il, W)
+    key.decrypt(c_host[:64])
+    for name, batch in (("units", c_host), ("one_nonunit", c_host.copy())):
+        if name == "one_nonunit":
+            batch[0] = ints_to_limbs([p], W)[0]
+        t = time.perf_counter()
+        dh = key.decrypt(batch)
+        exact[f"host_decrypt_{name}_ms"] = round((time.perf_counter() - t) * 1e3, 3)
+        ok = bool(np.array_equal(dh[1:], m_aos[1:]))
+        if name == "units":
+            ok = ok and bool(np.array_equal(dh[:1], m_aos[:1]))
+        else:
+            ok = ok and bool(np.array_equal(dh[:1], full_key.decrypt(batch[:1])))
+        if not ok:
+            print(json.dumps({"error": f"host decrypt ({name}) differs"}))
+            return 1
+    log(f"host decrypt of {count} (PCIe included): units {exact['host_decrypt_units_ms']:.1f} ms, "
+        f"with one non-unit {exact['host_decrypt_one_nonunit_ms']:.1f} ms")
     del full_key
 
     # roofline of the dominant kernel
@@ -834,16 +1019,17 @@ def main() -> int:
         plain = modexp_per_step / plain_s
         log(f"plain per-call form ({chunk}/call): {plain:,.0f} modexp/s")
 
+    try:
+        host_cores_visible = len(os.sched_getaffinity(0))
+    except AttributeError:
+        host_cores_visible = os.cpu_count() or 1
     cpu = None
     if not args.no_cpu:
         c_ints = limbs_to_ints(from_il(c_il, W)[: args.cpu_sample])
         cp = {"p": p, "q": q}
         cc = crt_constants(p, q)
         cp.update({"hp": cc["hp"], "hq": cc["hq"], "q_inv_p": cc["qinv"]})
-        try:
-            cores = min(16, len(os.sched_getaffinity(0)))
-        except AttributeError:
-            cores = min(16, os.cpu_count() or 1)
+        cores = min(16, host_cores_visible)
         t = time.perf_counter()
         cpu = cpu_baseline(cp, c_ints, ms[: args.cpu_sample], cores)
         log(f"cpu baseline ({cpu['engine']}): {cpu['single']:,.0f} modexp/s single thread, "
@@ -903,8 +1089,13 @@ def main() -> int:
                       f"(one process per core); single thread {cpu['single']:.1f} modexp/s on "
                       f"{cpu['single_n']} of them",
             "single_thread": round(cpu["single"], 1),
+            "model": host_model(),
+            "host_cores_visible": host_cores_visible,
+            "pool_note": "at most 16 processes: the host cores a one-GPU box allots",
         }
     legs = [x for x in args.legs.split(",") if x and x != "none"]
+    if "c1" in legs:
+        line["c1_encrypt_decrypt"] = c1_leg(ctx, args, min(16, host_cores_visible))
     if "ragged" in legs:
         line["ragged"] = ragged_leg(ctx, args, peak, torch.cuda.get_device_name(0))
     if "lookup" in legs:
@@ -918,8 +1109,10 @@ def main() -> int:
             dst.copy_(src)
         torch.cuda.synchronize()
         # the same bytes with the lane-group pattern of the exponentiation and
-        # mulmod kernels (k_copy_groups), one launch per (limbs, lanes) shape
-        for limbs, tpi in ((64, 4), (128, 8)):
+        # mulmod kernels (k_copy_groups), one launch per (limbs, lanes) shape;
+        # (128, 1): one lane per item, 64 consecutive items per limb row (the
+        # fixed and pointwise kernels' index loads)
+        for limbs, tpi in ((64, 4), (128, 8), (128, 1)):
             cnt = args.copy_calibration * (1 << 20) // (4 * limbs)
             st = stride_for(cnt)
             a = torch.ones((limbs, st), dtype=torch.int32, device=dev)
@@ -928,7 +1121,7 @@ def main() -> int:
             ctx.probe_copy(a, b, cnt, st, limbs, tpi)
             torch.cuda.synchronize()
             del a, b
-        log(f"copy calibration: 3 device copies of {args.copy_calibration} MiB, lane-group copies (64, 4), (128, 8)")
+        log(f"copy calibration: 3 device copies of {args.copy_calibration} MiB, lane-group copies (64, 4), (128, 8), (128, 1)")
     print(json.dumps(line), flush=True)
     return 0
 

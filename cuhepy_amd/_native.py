@@ -75,6 +75,9 @@ SIGNATURES = {
     "chx_mulmod_operands": (c_int, [vp, vp, u32p, vp, c_size_t]),
     "chx_mulmod_multi_dev": (c_int, [vp, vp, vp, c_size_t, c_size_t, c_size_t, c_int, POINTER(vp), POINTER(vp),
                                      c_size_t]),
+    "chx_mulmod_multi_stamped_dev": (c_int, [vp, vp, vp, c_size_t, c_size_t, c_size_t, c_int, POINTER(vp),
+                                             POINTER(vp), c_size_t, vp, vp]),
+    "chx_ctx_wall_clock_hz": (c_int, [vp, POINTER(c_double)]),
     "chx_modexp": (c_int, [vp, vp, u32p, c_size_t, u32p, c_int, u32p]),
     "chx_modexp_dev": (c_int, [vp, vp, vp, vp, c_size_t, c_size_t, u32p, c_int]),
     "chx_modexp_each": (c_int, [vp, vp, u32p, c_size_t, u32p, c_int, u32p]),

@@ -2,6 +2,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <thread>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -14,6 +16,7 @@
 #include <vector>
 
 #include "../../include/chx.h"
+#include "host_api.hpp"
 #include "host_bn.hpp"
 #include "kernels.hpp"
 #include "lazy.hpp"
@@ -22,48 +25,12 @@ using namespace chx;
 using chx::host::BN;
 
 // ---------------------------------------------------------------------------
-// errors
+// errors: ChxError, fail() and guarded() live in host_api.hpp
 // ---------------------------------------------------------------------------
-static thread_local std::string g_last_error;
-
 namespace {
-
-struct ChxError : std::runtime_error {
-  int code;
-  ChxError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-[[noreturn]] void fail(int code, const char* fmt, ...) {
-  char buf[512];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof buf, fmt, ap);
-  va_end(ap);
-  throw ChxError(code, buf);
-}
 
 void hip_check(hipError_t e, const char* what) {
   if (e != hipSuccess) fail(CHX_EDEVICE, "%s: %s", what, hipGetErrorString(e));
-}
-
-template <class F>
-int guarded(F&& f) {
-  try {
-    f();
-    return CHX_OK;
-  } catch (const ChxError& e) {
-    g_last_error = e.what();
-    return e.code;
-  } catch (const std::invalid_argument& e) {
-    g_last_error = e.what();
-    return CHX_EINVAL;
-  } catch (const std::bad_alloc&) {
-    g_last_error = "host allocation failed";
-    return CHX_ENOMEM;
-  } catch (const std::exception& e) {
-    g_last_error = e.what();
-    return CHX_EDEVICE;
-  }
 }
 
 size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
@@ -152,7 +119,7 @@ uint32_t* scratch_ptr(int device) {
 }
 }  // namespace
 
-enum Slot { S_IN0, S_IN1, S_OUT, S_AOS, S_TAB, S_TMP, S_AUX, S_MID0, S_MID1, S_NSLOT };
+enum Slot { S_IN0, S_IN1, S_OUT, S_AOS, S_TAB, S_TMP, S_AUX, S_MID0, S_MID1, S_QUEUE, S_NSLOT };
 
 struct TimedLaunch {
   std::string name;
@@ -681,6 +648,16 @@ static unsigned fixed_blocks(chx_ctx* ctx, F kernel, int threads, size_t lds, si
 }
 }
 
+// Head of a work queue in the context's memory, zeroed on its stream (a fill
+// kernel) for the launch that follows: launches on one stream take turns.
+u32* queue_head(chx_ctx* ctx) {
+  ctx->slot[S_QUEUE].ensure(256);
+  u32* h = ctx->slot[S_QUEUE].as<u32>();
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, h, 0u, 1);
+  check_launch("k_fill_u32");
+  return h;
+}
+
 // The one-reduction pointwise multiply (k_pw_mulmod) runs at the lazy
 // configuration of each width with 19 positions per chunk.
 #define CHX_DISPATCH_PW(L32_, ...)                                                                 \
@@ -747,6 +724,7 @@ void launch_mulmod_mod(chx_ctx* ctx, const chx_mod* mc, const uint32_t* a, const
   // window 2 of CHX_OP_MULMOD selects the two-product form below
   if (ctx->tune_window[CHX_OP_MULMOD][li] != 2 && pw_constants(ctx, m)) {
     const size_t tiles = (count + FIX_TILE - 1) / FIX_TILE;
+    u32* head = queue_head(ctx);
     record_start(ctx, "pw_mulmod");
     CHX_DISPATCH_PW(L, {
       auto kern = k_pw_mulmod<RB, LP, TPI, L32>;
@@ -758,7 +736,7 @@ void launch_mulmod_mod(chx_ctx* ctx, const chx_mod* mc, const uint32_t* a, const
       });
       hipLaunchKernelGGL(kern, dim3(fixed_blocks(ctx, kern, 64 * TPI, lds, tiles)), dim3(64 * TPI), lds,
                          ctx->stream, m->lz.d[pw_cfg(L)], m->d_pwz, m->pw_h, m->pw_nz, a, b, out, (int)count,
-                         (int)stride);
+                         (int)stride, head);
     });
     record_stop(ctx);
     check_launch("k_pw_mulmod");
@@ -1141,15 +1119,6 @@ extern "C" {
 
 const char* chx_version(void) { return "cuhepy_amd-chx 0.1 (gfx950)"; }
 
-int chx_last_error(char* buf, size_t size) {
-  if (buf && size) {
-    size_t n = std::min(size - 1, g_last_error.size());
-    memcpy(buf, g_last_error.data(), n);
-    buf[n] = 0;
-  }
-  return (int)g_last_error.size();
-}
-
 int chx_ctx_create(int device, chx_ctx** out) {
   return guarded([&] {
     if (!out) fail(CHX_EINVAL, "null out pointer");
@@ -1372,43 +1341,101 @@ int chx_mulmod_operands(chx_ctx* ctx, const chx_mod* mc, const uint32_t* y, uint
 }
 
 
+// One multi-query launch (k_fixed_mulmod); done/stamps optional (both or neither).
+static void multi_launch(chx_ctx* ctx, const chx_mod* m, const uint32_t* x, size_t stride, size_t begin,
+                         size_t count, int nq, const uint32_t* const* ops, uint32_t* const* outs,
+                         size_t out_stride, uint32_t* done, uint64_t* stamps) {
+  if (!m || !x || !ops || !outs) fail(CHX_EINVAL, "null argument");
+  if (m->ctx->device != ctx->device) fail(CHX_EINVAL, "modulus lives on another device");
+  if (nq < 1 || nq > MQ_MAX) fail(CHX_EINVAL, "nq=%d out of range [1, %d]", nq, MQ_MAX);
+  if (count == 0) return;
+  if (begin + count > stride) fail(CHX_EINVAL, "range [%zu, %zu) exceeds the index stride %zu", begin,
+                                   begin + count, stride);
+  if (begin + count > out_stride) fail(CHX_EINVAL, "range exceeds the output stride %zu", out_stride);
+  if (stride > kMaxStride || out_stride > kMaxStride) fail(CHX_EINVAL, "stride exceeds the maximum of %zu", kMaxStride);
+  MultiQuery mq;
+  memset(&mq, 0, sizeof(mq));
+  mq.nq = nq;
+  for (int q = 0; q < nq; ++q) {
+    if (!ops[q] || !outs[q]) fail(CHX_EINVAL, "query %d: null pointer", q);
+    mq.op[q] = ops[q];
+    mq.out[q] = outs[q];
+  }
+  const int L = m->hc.L;
+  const size_t tiles = (count + FIX_TILE - 1) / FIX_TILE;
+  if (done || stamps) {
+    if (!done || !stamps) fail(CHX_EINVAL, "done and stamps go together");
+    mq.done = done;
+    mq.stamp = reinterpret_cast<u64*>(stamps);
+    // counters and the unit queue's head from zero (a fill kernel, no memset
+    // node if the caller captures)
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, done, 0u, nq + 1);
+    check_launch("k_fill_u32");
+    mq.head = done + nq;
+  } else {
+    mq.head = queue_head(ctx);
+  }
+  record_start(ctx, "mulmod_multi");
+  CHX_DISPATCH_LAZY(L, kServeCfg, {
+    auto kern = k_fixed_mulmod<RB, LP, TPI, L32>;
+    const size_t lds = (size_t)FixedLds<LP, TPI>::WORDS * 4;
+    static std::once_flag lds_attr;
+    std::call_once(lds_attr, [&] {
+      hip_check(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                "hipFuncSetAttribute");
+    });
+    hipLaunchKernelGGL(kern, dim3(fixed_blocks(ctx, kern, 64 * TPI, lds, tiles * nq)), dim3(64 * TPI), lds,
+                       ctx->stream, m->lz.d[kServeCfg], x, (int)begin, (int)count, (int)stride, mq,
+                       (int)out_stride);
+  });
+  record_stop(ctx);
+  check_launch("k_fixed_mulmod");
+}
+
 int chx_mulmod_multi_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* x, size_t stride, size_t begin,
                          size_t count, int nq, const uint32_t* const* ops, uint32_t* const* outs,
                          size_t out_stride) {
   return with_ctx(ctx, [&] {
-    if (!m || !x || !ops || !outs) fail(CHX_EINVAL, "null argument");
-    if (m->ctx->device != ctx->device) fail(CHX_EINVAL, "modulus lives on another device");
-    if (nq < 1 || nq > MQ_MAX) fail(CHX_EINVAL, "nq=%d out of range [1, %d]", nq, MQ_MAX);
-    if (count == 0) return;
-    if (begin + count > stride) fail(CHX_EINVAL, "range [%zu, %zu) exceeds the index stride %zu", begin,
-                                     begin + count, stride);
-    if (begin + count > out_stride) fail(CHX_EINVAL, "range exceeds the output stride %zu", out_stride);
-    if (stride > kMaxStride || out_stride > kMaxStride) fail(CHX_EINVAL, "stride exceeds the maximum of %zu", kMaxStride);
-    MultiQuery mq;
-    memset(&mq, 0, sizeof(mq));
-    mq.nq = nq;
-    for (int q = 0; q < nq; ++q) {
-      if (!ops[q] || !outs[q]) fail(CHX_EINVAL, "query %d: null pointer", q);
-      mq.op[q] = ops[q];
-      mq.out[q] = outs[q];
-    }
-    const int L = m->hc.L;
-    const size_t tiles = (count + FIX_TILE - 1) / FIX_TILE;
-    record_start(ctx, "mulmod_multi");
-    CHX_DISPATCH_LAZY(L, kServeCfg, {
-      auto kern = k_fixed_mulmod<RB, LP, TPI, L32>;
-      const size_t lds = (size_t)FixedLds<LP, TPI>::WORDS * 4;
-      static std::once_flag lds_attr;
-      std::call_once(lds_attr, [&] {
-        hip_check(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
-                  "hipFuncSetAttribute");
-      });
-      hipLaunchKernelGGL(kern, dim3(fixed_blocks(ctx, kern, 64 * TPI, lds, tiles)), dim3(64 * TPI), lds,
-                         ctx->stream, m->lz.d[kServeCfg], x, (int)begin, (int)count, (int)stride, mq,
-                         (int)out_stride);
-    });
-    record_stop(ctx);
-    check_launch("k_fixed_mulmod");
+    multi_launch(ctx, m, x, stride, begin, count, nq, ops, outs, out_stride, nullptr, nullptr);
+  });
+}
+
+int chx_mulmod_multi_stamped_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* x, size_t stride, size_t begin,
+                                 size_t count, int nq, const uint32_t* const* ops, uint32_t* const* outs,
+                                 size_t out_stride, uint32_t* done, uint64_t* stamps) {
+  return with_ctx(ctx, [&] {
+    if (!done || !stamps) fail(CHX_EINVAL, "null argument");
+    multi_launch(ctx, m, x, stride, begin, count, nq, ops, outs, out_stride, done, stamps);
+  });
+}
+
+// The stamps' clock rate, measured: two reads of the device clock (k_clock)
+// about 20 ms apart against the host's steady clock (the rate the runtime
+// reports as the wall clock rate need not be this counter's).
+int chx_ctx_wall_clock_hz(chx_ctx* ctx, double* hz) {
+  return with_ctx(ctx, [&] {
+    if (!hz) fail(CHX_EINVAL, "null argument");
+    ctx->slot[S_AUX].ensure(16);
+    u64* d = ctx->slot[S_AUX].as<u64>();
+    auto read = [&](u64* host) {
+      hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, ctx->stream, d);
+      check_launch("k_clock");
+      hip_check(hipMemcpyAsync(host, d, 8, hipMemcpyDeviceToHost, ctx->stream), "D2H");
+      hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    };
+    u64 a = 0, b = 0;
+    read(&a);  // warm
+    const auto t0 = std::chrono::steady_clock::now();
+    read(&a);
+    const auto t1 = std::chrono::steady_clock::now();
+    std::this_thread::sleep_for(std::chrono::milliseconds(20));
+    const auto t2 = std::chrono::steady_clock::now();
+    read(&b);
+    const auto t3 = std::chrono::steady_clock::now();
+    // each read lies between its bracketing host times: take the midpoints
+    const double host = std::chrono::duration<double>((t2 - t0) + (t3 - t1)).count() / 2;
+    if (b <= a || host <= 0) fail(CHX_EDEVICE, "device clock did not advance");
+    *hz = (double)(b - a) / host;
   });
 }
 
@@ -1752,18 +1779,6 @@ int chx_ragged_destroy(chx_ragged* rg) {
 // ---------------------------------------------------------------------------
 // Paillier
 // ---------------------------------------------------------------------------
-namespace {
-// Host check of a Paillier key given with its factors (host::check_paillier_factors).
-void check_factors(const uint32_t* n, int n_limbs, const uint32_t* phi, const uint32_t* inv, const uint32_t* p,
-                   const uint32_t* q, int half_limbs, const uint32_t* hp, const uint32_t* hq, const uint32_t* qinv) {
-  if (!n || !phi || !inv || !p || !q || !hp || !hq || !qinv || n_limbs <= 0 || n_limbs > 64 || half_limbs <= 0 ||
-      half_limbs > 64)
-    fail(CHX_EINVAL, "bad key arguments");
-  auto v = [](const uint32_t* x, int l) { return host::bn_from(x, l, l); };
-  host::check_paillier_factors(v(n, n_limbs), v(phi, n_limbs), v(inv, n_limbs), v(p, half_limbs), v(q, half_limbs),
-                               v(hp, half_limbs), v(hq, half_limbs), v(qinv, half_limbs));
-}
-}  // namespace
 
 int chx_paillier_key_create(chx_ctx* ctx, const uint32_t* n, int n_limbs, const uint32_t* phi,
                             const uint32_t* inv, const uint32_t* p, const uint32_t* q, int half_limbs,
@@ -2430,32 +2445,6 @@ int chx_il_to_aos_dev(chx_ctx* ctx, const uint32_t* il, uint32_t* aos, size_t co
   });
 }
 
-int chx_host_schedule(const uint32_t* e, int e_limbs, int window, uint32_t* out, int cap) {
-  int n = -1;
-  int rc = guarded([&] {
-    if (!e || e_limbs <= 0 || window < 1 || window > 7) fail(CHX_EINVAL, "bad argument");
-    std::vector<uint32_t> s = host::sliding_schedule(host::bn_from(e, e_limbs, e_limbs), window);
-    for (int i = 0; i < (int)s.size() && i < cap; ++i) out[i] = s[i];
-    n = (int)s.size();
-  });
-  return rc == CHX_OK ? n : -rc;
-}
-
-int chx_host_paillier_check(const uint32_t* n, int n_limbs, const uint32_t* phi, const uint32_t* inv,
-                            const uint32_t* p, const uint32_t* q, int half_limbs, const uint32_t* hp,
-                            const uint32_t* hq, const uint32_t* qinv) {
-  return guarded([&] { check_factors(n, n_limbs, phi, inv, p, q, half_limbs, hp, hq, qinv); });
-}
-
-int chx_host_mont_consts(const uint32_t* n, int limbs, uint32_t* n0inv, uint32_t* r2) {
-  return guarded([&] {
-    if (!n || !n0inv || !r2 || limbs <= 0) fail(CHX_EINVAL, "bad argument");
-    host::MontConsts c = host::mont_consts(host::bn_from(n, limbs, limbs));
-    *n0inv = c.n0inv;
-    for (int i = 0; i < limbs; ++i) r2[i] = c.R2[i];
-  });
-}
-
 int chx_probe_rate(chx_ctx* ctx, int kind, int blocks, int iters, double* seconds, double* ops) {
   return with_ctx(ctx, [&] {
     if (blocks <= 0 || iters <= 0 || !seconds || !ops || kind < 0 || kind > 7) fail(CHX_EINVAL, "bad argument");
@@ -2504,6 +2493,7 @@ int chx_probe_copy(chx_ctx* ctx, const uint32_t* src, uint32_t* dst, size_t coun
                          dst, (int)count, (int)stride);                                                        \
       done = true;                                                                                             \
     }
+    CHX_COPY_CASE(128, 1)  // one lane per item: the row pattern of the fixed and pointwise kernels' loads
     CHX_COPY_CASE(64, 4)
     CHX_COPY_CASE(64, 8)
     CHX_COPY_CASE(128, 8)

@@ -987,6 +987,9 @@ struct MultiQuery {
   int nq;
   const u32* op[MQ_MAX];  // operand tables (k_fixed_operand), device
   u32* out[MQ_MAX];       // interleaved [L32][out_stride], indexed by the global entry
+  u32* head;              // head of the queue of (query, tile) units, zeroed before the launch
+  u32* done;              // optional (nullptr): finished units per query, zeroed before the launch
+  u64* stamp;             // with done: device wall clock when each query's last unit finished
 };
 
 // ---------------------------------------------------------------------------
@@ -1308,7 +1311,16 @@ __global__ __launch_bounds__(64 * TPI) __attribute__((amdgpu_waves_per_eu(4, 8))
   // (97 KB each at 4096-bit moduli) instead of all of them. Each unit converts
   // its tile's positions again (a small share of the unit's multiply-adds).
   const int units = tiles * mq.nq;
-  for (int u = blockIdx.x; u < units; u += gridDim.x) {
+  // The units come from a queue: every workgroup takes the next unit in
+  // order, so a fast workgroup takes more units than a slow one (strided units
+  // leave the launch waiting for its slowest workgroups, 14% slower measured)
+  // and the queries complete one after another at the pace of the grid.
+  __shared__ int next_unit;
+  for (;;) {
+    if (threadIdx.x == 0) next_unit = (int)atomicAdd(mq.head, 1u);
+    __syncthreads();
+    const int u = next_unit;  // the previous unit ended with a barrier: nobody still reads the old value
+    if (u >= units) break;
     const int q = u / tiles, tile = u - q * tiles;
     const int base = tile * FIX_TILE;
     // A: wave w converts positions [w K, w K + K) of all 64 entries (one
@@ -1357,6 +1369,12 @@ __global__ __launch_bounds__(64 * TPI) __attribute__((amdgpu_waves_per_eu(4, 8))
       fixed_finish<RB, LP, TPI, L32>(wave, md, acc, lds, lane, mq.out[q] + (size_t)begin + base + lane, so,
                                      base + lane < count);
     }
+    __syncthreads();  // every wave has issued the unit's stores (and read next_unit)
+    // a query is complete when its last unit is: stamp the device clock then
+    // (a timing record, not a hand-over of data, which is read after the
+    // launch: no fence, which would write the L2 back on every unit)
+    if (mq.done != nullptr && threadIdx.x == 0 && atomicAdd(mq.done + q, 1u) == (u32)tiles - 1u)
+      mq.stamp[q] = __builtin_amdgcn_s_memrealtime();
   }
 }
 
@@ -1490,7 +1508,7 @@ __device__ __forceinline__ void pw_carry_in(u32 (&x)[K], u64 cin) {
 template <int RB, int LP, int TPI, int L32>
 __global__ __launch_bounds__(64 * TPI) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_pw_mulmod(
     const LazyDev* __restrict__ md, const u32* __restrict__ Z, int h, int nz, const u32* __restrict__ A,
-    const u32* __restrict__ B, u32* __restrict__ out, int count, int stride) {
+    const u32* __restrict__ B, u32* __restrict__ out, int count, int stride, u32* __restrict__ head) {
   extern __shared__ uint4 smem4[];
   constexpr int K = LP / TPI, CH = FixedShape<K>::CH;
   static_assert(LP % TPI == 0 && K >= 2, "chunks of at least two positions");
@@ -1504,7 +1522,14 @@ __global__ __launch_bounds__(64 * TPI) __attribute__((amdgpu_waves_per_eu(4, 8))
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
   const int lane = lane_id();
   const int tiles = (count + FIX_TILE - 1) / FIX_TILE;
-  for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+  // tiles from a queue (head zeroed before the launch): a fast workgroup takes
+  // more tiles than a slow one
+  __shared__ int next_tile;
+  for (;;) {
+    if (threadIdx.x == 0) next_tile = (int)atomicAdd(head, 1u);
+    __syncthreads();
+    const int tile = next_tile;  // the previous tile ended with a barrier
+    if (tile >= tiles) break;
     const int base = tile * FIX_TILE;
     {  // 1: positions of a and b
       const int local = base + lane < count ? base + lane : count - 1;
@@ -1601,6 +1626,11 @@ struct RaggedSub {
   int count, stride, e_limbs, nwin;
   int tile0;          // first queue tile of this sub-batch
 };
+
+// the device wall clock the multi-query stamps count, for calibration
+__global__ void k_clock(u64* __restrict__ out) {
+  if (threadIdx.x == 0) *out = __builtin_amdgcn_s_memrealtime();
+}
 
 __global__ void k_fill_u32(u32* __restrict__ p, u32 v, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;

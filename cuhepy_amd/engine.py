@@ -18,7 +18,7 @@ from typing import Optional, Sequence
 import numpy as np
 
 from . import _native as N
-from .limbs import ints_to_limbs, limb_count, width_for_bits
+from .limbs import ints_to_limbs, limb_count, limbs_below, limbs_to_ints, width_for_bits
 
 _u32p = N.u32p
 
@@ -108,6 +108,12 @@ class Context:
     def sync(self) -> None:
         N.check(N.lib.chx_ctx_sync(self.handle))
 
+    def wall_clock_hz(self) -> float:
+        """Frequency of the device wall clock (the stamps of mulmod_multi_stamped_dev)."""
+        hz = ctypes.c_double()
+        N.check(N.lib.chx_ctx_wall_clock_hz(self.handle, ctypes.byref(hz)))
+        return hz.value
+
     def set_tuning(self, op: int, limbs: int, tpi: int = 0, window: int = 0) -> None:
         N.check(N.lib.chx_ctx_set_tuning(self.handle, op, limbs, tpi, window))
 
@@ -185,6 +191,21 @@ class Modulus:
             N.check(N.lib.chx_mulmod(self.ctx.handle, self.handle, _p32(A), _p32(B), _p32(out), len(A)))
         return out
 
+    def product(self, a) -> int:
+        """The product of all rows mod n, as a tree of batched mulmods (log2
+        count launches); 1 for an empty batch. A product is a unit mod n
+        exactly when every factor is, so one gcd with n checks a batch."""
+        A = _batch(a, self.limbs)
+        if len(A) == 0:
+            return 1
+        if not limbs_below(A, self.n).all():
+            raise ValueError("factors must be below the modulus")
+        while len(A) > 1:
+            h = len(A) // 2
+            P = self.mulmod(A[:h], A[h:2 * h])
+            A = np.concatenate([P, A[2 * h:]]) if len(A) % 2 else P
+        return limbs_to_ints(A)[0]
+
     def modexp(self, bases, e: int) -> np.ndarray:
         if e < 0:
             raise ValueError("negative exponent")
@@ -252,6 +273,21 @@ class Modulus:
         O = (ctypes.c_void_p * max(nq, 1))(*[int(o.data_ptr()) for o in outs])
         N.check(N.lib.chx_mulmod_multi_dev(self.ctx.handle, self.handle, _dptr(x), stride, begin, count, nq, P, O,
                                            out_stride))
+
+    def mulmod_multi_stamped_dev(self, x, stride: int, begin: int, count: int, ops: Sequence, outs: Sequence,
+                                 out_stride: int, done, stamps) -> None:
+        """mulmod_multi_dev, and stamps[q] (int64 device tensor) = the device
+        clock when query q's last tile finished (done: int32 device scratch of
+        at least nq + 1 words: tile counters and the unit queue's head)."""
+        if done.numel() < len(ops) + 1 or stamps.numel() < len(ops):
+            raise ValueError("done needs nq + 1 words and stamps nq")
+        nq = len(ops)
+        if nq != len(outs):
+            raise ValueError("ops and outs differ in length")
+        P = (ctypes.c_void_p * max(nq, 1))(*[int(o.data_ptr()) for o in ops])
+        O = (ctypes.c_void_p * max(nq, 1))(*[int(o.data_ptr()) for o in outs])
+        N.check(N.lib.chx_mulmod_multi_stamped_dev(self.ctx.handle, self.handle, _dptr(x), stride, begin, count, nq,
+                                                   P, O, out_stride, _dptr(done), _dptr(stamps)))
 
     def modexp_dev(self, base, out, count: int, stride: int, e: int) -> None:
         E = ints_to_limbs([e], limb_count(e))

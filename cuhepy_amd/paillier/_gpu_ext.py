@@ -13,23 +13,36 @@ semantics (paillier/scheme.py):
     (cu:376-404); a ``random.Random`` can be injected for reproducibility;
   * hex inputs wider than the key width raise instead of being truncated
     (cu:235-254).
-Added (no reference counterpart): batched ``encrypt_cts``/``decrypt_cts``/
-``decrypt_cts_crt`` over integers, so ``Paillier.encrypt``/``decrypt`` can run
-on the GPU.
+Added (no reference counterpart):
+  * batched ``encrypt_cts``/``decrypt_cts``/``decrypt_cts_crt`` over integers,
+    so ``Paillier.encrypt``/``decrypt`` (scheme.py:63-104) can run on the GPU;
+  * the same path in the little-endian bytes convention of
+    hamming/paillier.py:101, 162 and of the msgpack packets of
+    benchmarks/bfv_client_matrix.py:49-69 (minimal LE bytes per ciphertext):
+    ``encrypt_bytes``, ``encode_hamming_server_bytes``, ``encrypt_cts_bytes``,
+    ``decrypt_cts_bytes``; ``decode_hamming_client`` takes bytes too. These
+    never build Python integers: bytes <-> limb arrays in numpy.
+Host conversions are batched numpy work (bit packing, range checks, hex and
+bytes codecs); without an injected ``random.Random`` the r values are drawn
+in one batch from the OS and their coprimality with n is checked with one
+gcd of their product mod n (a tree of device mulmods), redrawing one at a
+time only if that gcd is not 1.
 """
 
 from __future__ import annotations
 
 import json
+import math
 import random
 from typing import Optional, Sequence
 
 import numpy as np
 
 from ..engine import Context, Modulus, PaillierKey
-from ..limbs import ints_to_limbs, limbs_to_ints, width_for_bits
+from ..limbs import (DoesNotFit, bytes_to_limbs, hex_to_limbs, ints_to_limbs, limbs_below, limbs_to_bytes,
+                     limbs_to_hex, limbs_to_ints, width_for_bits)
 from . import keygen as kg
-from .packing import chunk_geometry, embeddings_to_plaintexts, odd_mask, validate_embeddings
+from .packing import chunk_geometry, embeddings_to_plaintexts, odd_mask, plaintexts_below, validate_embeddings
 
 KEY_BITS = 1024  # largest prime size: n^2 of 4096 bits = 128 limbs
 
@@ -59,15 +72,41 @@ def _parse_hex(s) -> int:
         raise ValueError(f"Invalid hex digit in {s!r}") from e
 
 
+def cipher_limbs(values: Sequence, limbs: int) -> np.ndarray:
+    """Ciphertexts as hex strings (the reference's format), LE bytes or ints
+    -> (count, limbs) uint32; hex and bytes batches are converted in bulk."""
+    if values and all(type(v) is str for v in values):
+        return hex_to_limbs(values, limbs)
+    if values and all(isinstance(v, (bytes, bytearray, memoryview)) for v in values):
+        return bytes_to_limbs(values, limbs)
+    return ints_to_limbs([_parse_hex(v) for v in values], limbs)
+
+
+def pointwise_mulmod_limbs(a: np.ndarray, b: np.ndarray, n_squared: int) -> np.ndarray:
+    """Row-wise a * b mod n^2 on the GPU (gpu_pointwise_mulmod_hex, cu:672-732)."""
+    if len(a) != len(b):
+        raise RuntimeError("encode_hamming_*: ct1 and ct2 must have the same length")
+    mod = _modulus(n_squared)
+    if len(a) == 0:
+        return np.zeros((0, mod.limbs), dtype=np.uint32)
+    return mod.mulmod(a, b)
+
+
 def pointwise_mulmod(ct1: Sequence[int], ct2: Sequence[int], n_squared: int) -> list[int]:
     """c1[i] * c2[i] mod n^2 on the GPU (gpu_pointwise_mulmod_hex, cu:672-732)."""
     if len(ct1) != len(ct2):
         raise RuntimeError("encode_hamming_*: ct1 and ct2 must have the same length")
     if not ct1:
         return []
-    mod = _modulus(n_squared)
-    out = mod.mulmod(ints_to_limbs(ct1, mod.limbs), ints_to_limbs(ct2, mod.limbs))
-    return limbs_to_ints(out)
+    W = _modulus(n_squared).limbs
+    return limbs_to_ints(pointwise_mulmod_limbs(ints_to_limbs(ct1, W), ints_to_limbs(ct2, W), n_squared))
+
+
+def _server_operands(ct1: Sequence, ct2: Sequence, n2: int) -> tuple[np.ndarray, np.ndarray]:
+    if len(ct1) != len(ct2):
+        raise RuntimeError("encode_hamming_*: ct1 and ct2 must have the same length")
+    W = width_for_bits(n2.bit_length())
+    return cipher_limbs(ct1, W), cipher_limbs(ct2, W)
 
 
 class PaillierGPUClient:
@@ -168,97 +207,157 @@ class PaillierGPUClient:
         self._chunk_len, self._chunk_num = chunk_geometry(self._embed_len, self._key_len)
 
     # ---- batched integer API (added) ----
-    def _draw_r(self, count: int) -> list[int]:
-        return [kg.random_r(self._n, self._rng) for _ in range(count)]
+    def _n_modulus(self) -> Modulus:
+        return _modulus(self._n)
+
+    def _draw_r(self, count: int, width: int) -> np.ndarray:
+        """(count, width) r values uniform in [1, n-1] with gcd(r, n) = 1
+        (scheme.py:47-60)."""
+        if self._rng is not None:  # reproducible: one value at a time from the injected generator
+            return ints_to_limbs([kg.random_r(self._n, self._rng) for _ in range(count)], width)
+        R = kg.random_below_limbs(self._n, count, width)
+        if count and math.gcd(self._n_modulus().product(R[:, :self._n_modulus().limbs]), self._n) != 1:
+            # some r shares a factor with n: redraw those, one at a time
+            for i, r in enumerate(limbs_to_ints(R)):
+                if math.gcd(r, self._n) != 1:
+                    R[i] = ints_to_limbs([kg.random_r(self._n)], width)[0]
+        return R
+
+    def _plaintext_limbs(self, m: np.ndarray) -> np.ndarray:
+        if not limbs_below(m, self._n).all():
+            raise ValueError("plaintext must be in range [0, n)")
+        return m
+
+    def _cipher_batch(self, values: Sequence, width: int) -> np.ndarray:
+        try:
+            C = cipher_limbs(values, width)
+        except DoesNotFit as e:
+            raise ValueError("ciphertext must be in range [0, n^2)") from e
+        if not limbs_below(C, self._n * self._n).all():
+            raise ValueError("ciphertext must be in range [0, n^2)")
+        return C
+
+    def _encrypt_limbs(self, key: PaillierKey, m: np.ndarray, rs: Optional[Sequence[int]] = None) -> np.ndarray:
+        self._plaintext_limbs(m)
+        if rs is None:
+            r = self._draw_r(len(m), key.width)
+        else:
+            if len(rs) != len(m):
+                raise ValueError("rs must match plaintexts")
+            r = ints_to_limbs([int(v) for v in rs], key.width)
+        return key.encrypt(m, r)
 
     def encrypt_cts(self, plaintexts: Sequence[int], rs: Optional[Sequence[int]] = None) -> list[int]:
         """c_i = g^m_i r_i^n mod n^2 for m_i in [0, n) (scheme.py:63-83)."""
         key = self._require_keys("encrypt_cts")
         ms = [int(m) for m in plaintexts]
-        for m in ms:
-            if not 0 <= m < self._n:
-                raise ValueError("plaintext must be in range [0, n)")
-        rr = self._draw_r(len(ms)) if rs is None else [int(r) for r in rs]
-        if len(rr) != len(ms):
-            raise ValueError("rs must match plaintexts")
+        if any(m < 0 for m in ms):
+            raise ValueError("plaintext must be in range [0, n)")
         if not ms:
+            if rs:
+                raise ValueError("rs must match plaintexts")
             return []
-        out = key.encrypt(ints_to_limbs(ms, key.width), ints_to_limbs(rr, key.width))
-        return limbs_to_ints(out)
+        try:
+            m = ints_to_limbs(ms, key.width)
+        except DoesNotFit as e:
+            raise ValueError("plaintext must be in range [0, n)") from e
+        return limbs_to_ints(self._encrypt_limbs(key, m, rs))
+
+    def encrypt_cts_bytes(self, plaintexts: Sequence[bytes]) -> list[bytes]:
+        """``encrypt_cts`` with LE bytes in and minimal LE bytes out."""
+        key = self._require_keys("encrypt_cts_bytes")
+        if not plaintexts:
+            return []
+        try:
+            m = bytes_to_limbs(plaintexts, key.width)
+        except DoesNotFit as e:
+            raise ValueError("plaintext must be in range [0, n)") from e
+        return limbs_to_bytes(self._encrypt_limbs(key, m))
 
     def decrypt_cts(self, ciphers: Sequence[int]) -> list[int]:
         """m_i = L(c_i^phi mod n^2) * inv mod n (scheme.py:86-104)."""
         key = self._require_keys("decrypt_cts")
         cs = [int(c) for c in ciphers]
-        n2 = self._n * self._n
-        for c in cs:
-            if not 0 <= c < n2:
-                raise ValueError("ciphertext must be in range [0, n^2)")
         if not cs:
             return []
-        return limbs_to_ints(key.decrypt(ints_to_limbs(cs, key.width)))
+        if any(c < 0 for c in cs):
+            raise ValueError("ciphertext must be in range [0, n^2)")
+        return limbs_to_ints(key.decrypt(self._cipher_batch(cs, key.width)))
+
+    def decrypt_cts_bytes(self, ciphers: Sequence[bytes]) -> list[bytes]:
+        """``decrypt_cts`` with LE bytes in and minimal LE bytes out."""
+        key = self._require_keys("decrypt_cts_bytes")
+        if not ciphers:
+            return []
+        return limbs_to_bytes(key.decrypt(self._cipher_batch(ciphers, key.width)))
 
     def decrypt_cts_crt(self, ciphers: Sequence[int]) -> list[int]:
         """CRT decryption (two half-width modexps per ciphertext)."""
         key = self._require_keys("decrypt_cts_crt")
         cs = [int(c) for c in ciphers]
-        n2 = self._n * self._n
-        for c in cs:
-            if not 0 <= c < n2:
-                raise ValueError("ciphertext must be in range [0, n^2)")
         if not cs:
             return []
-        return limbs_to_ints(key.decrypt_crt(ints_to_limbs(cs, key.width)))
+        if any(c < 0 for c in cs):
+            raise ValueError("ciphertext must be in range [0, n^2)")
+        return limbs_to_ints(key.decrypt_crt(self._cipher_batch(cs, key.width)))
 
     # ---- reference surface ----
-    def encrypt(self, embeddings: Sequence[Sequence[int]]) -> list[list[str]]:
-        """[batch][embed_len] bits -> [batch][chunk_num] hex ciphertexts (cu:872-974)."""
+    def _encrypt_embeddings(self, embeddings) -> tuple[int, np.ndarray]:
         key = self._require_keys("encrypt")
         bits = validate_embeddings(embeddings, self._embed_len)
-        batch = bits.shape[0]
-        if batch == 0:
-            return []
+        if bits.shape[0] == 0:
+            return 0, np.zeros((0, key.width), dtype=np.uint32)
         m = embeddings_to_plaintexts(bits, self._key_len, key.width)
-        ms = limbs_to_ints(m)
-        for v in ms:
-            if v >= self._n:
-                raise ValueError("plaintext must be in range [0, n)")
-        r = ints_to_limbs(self._draw_r(len(ms)), key.width)
-        cts = limbs_to_ints(key.encrypt(m, r))
-        cn = self._chunk_num
-        return [[format(c, "x") for c in cts[i * cn:(i + 1) * cn]] for i in range(batch)]
+        if not plaintexts_below(self._embed_len, self._key_len, self._n):
+            self._plaintext_limbs(m)
+        r = self._draw_r(len(m), key.width)
+        return bits.shape[0], key.encrypt(m, r)
+
+    def encrypt(self, embeddings: Sequence[Sequence[int]]) -> list[list[str]]:
+        """[batch][embed_len] bits -> [batch][chunk_num] hex ciphertexts (cu:872-974).
+        A (batch, embed_len) numpy array of 0/1 is taken as it is."""
+        batch, c = self._encrypt_embeddings(embeddings)
+        hx, cn = limbs_to_hex(c), self._chunk_num
+        return [hx[i * cn:(i + 1) * cn] for i in range(batch)]
+
+    def encrypt_bytes(self, embeddings) -> list[list[bytes]]:
+        """``encrypt`` with minimal LE bytes per ciphertext."""
+        batch, c = self._encrypt_embeddings(embeddings)
+        by, cn = limbs_to_bytes(c), self._chunk_num
+        return [by[i * cn:(i + 1) * cn] for i in range(batch)]
 
     def encode_hamming_client(self, ct1: Sequence[str], ct2: Sequence[str]) -> list[str]:
         """ct1[i] * ct2[i] mod n^2 with the client's own key (cu:976-983)."""
         self._require_keys("encode_hamming_client")
-        out = pointwise_mulmod([_parse_hex(c) for c in ct1], [_parse_hex(c) for c in ct2], self._n * self._n)
-        return [format(v, "x") for v in out]
+        n2 = self._n * self._n
+        return limbs_to_hex(pointwise_mulmod_limbs(*_server_operands(ct1, ct2, n2), n2))
 
     @staticmethod
     def encode_hamming_server(ct1: Sequence[str], ct2: Sequence[str], pk: dict) -> list[str]:
         """Evaluator: ct1[i] * ct2[i] mod pk["n_squared"] (hex), cu:986-992."""
         n2 = _parse_hex(pk["n_squared"])
-        out = pointwise_mulmod([_parse_hex(c) for c in ct1], [_parse_hex(c) for c in ct2], n2)
-        return [format(v, "x") for v in out]
+        return limbs_to_hex(pointwise_mulmod_limbs(*_server_operands(ct1, ct2, n2), n2))
+
+    @staticmethod
+    def encode_hamming_server_bytes(ct1: Sequence[bytes], ct2: Sequence[bytes], pk: dict) -> list[bytes]:
+        """``encode_hamming_server`` with LE bytes in and out."""
+        n2 = _parse_hex(pk["n_squared"])
+        return limbs_to_bytes(pointwise_mulmod_limbs(*_server_operands(ct1, ct2, n2), n2))
 
     def decode_hamming_client(self, ciphers_hex_batch: Sequence[Sequence[str]]) -> list[int]:
-        """Decrypt each chunk and count odd-position bits (cu:995-1038), fused on the GPU."""
+        """Decrypt each chunk and count odd-position bits (cu:995-1038), fused on
+        the GPU. Ciphertexts are hex strings (the reference's format), LE bytes
+        or ints."""
         key = self._require_keys("decode_hamming_client")
         batch = len(ciphers_hex_batch)
         if batch == 0:
             return []
         cn = self._chunk_num
-        flat: list[int] = []
-        for row in ciphers_hex_batch:
-            if len(row) != cn:
-                raise RuntimeError("decode_hamming_client: each inner list must have length == chunk_num")
-            flat.extend(_parse_hex(c) for c in row)
-        n2 = self._n * self._n
-        for c in flat:
-            if not 0 <= c < n2:
-                raise ValueError("ciphertext must be in range [0, n^2)")
-        counts = key.decode_hamming(ints_to_limbs(flat, key.width), odd_mask(self._chunk_len))
-        return [int(counts[i * cn:(i + 1) * cn].sum()) for i in range(batch)]
+        if any(len(row) != cn for row in ciphers_hex_batch):
+            raise RuntimeError("decode_hamming_client: each inner list must have length == chunk_num")
+        flat = [c for row in ciphers_hex_batch for c in row]
+        counts = key.decode_hamming(self._cipher_batch(flat, key.width), odd_mask(self._chunk_len))
+        return counts.reshape(batch, cn).sum(axis=1, dtype=np.int64).tolist()
 
 
 # the reference module also exports the compile-time width

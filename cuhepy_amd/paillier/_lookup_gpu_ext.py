@@ -20,6 +20,14 @@ Work placement:
     re-encrypt under key B, both on the GPU.
 Noise indices are uniform bytes (the reference takes u32 words mod 256,
 lcu:933-934, which is uniform too); a ``random.Random`` can be injected.
+
+Host conversions are batched numpy work (bit packing, range checks, limb
+codecs). Added (no reference counterpart): the little-endian bytes forms
+``encrypt_bytes``, ``encrypt_cts_bytes``, ``decrypt_cts_bytes`` and
+``encode_hamming_server_bytes`` (minimal LE bytes per ciphertext, the
+convention of hamming/paillier.py:101, 162), which never build Python
+integers; every method that takes ciphertexts also takes LE bytes
+(big_from_py, lcu:1301-1320, takes int, decimal and 0x-hex strings).
 """
 
 from __future__ import annotations
@@ -32,10 +40,12 @@ from typing import Optional, Sequence
 import numpy as np
 
 from ..engine import Context, LookupKey, Modulus
-from ..limbs import ints_to_limbs, limbs_to_ints, width_for_bits
+from ..limbs import (DoesNotFit, bytes_to_limbs, ints_to_limbs, limbs_below, limbs_to_bytes, limbs_to_ints,
+                     width_for_bits)
 from . import keygen as kg
-from ._gpu_ext import KEY_BITS, pointwise_mulmod
-from .packing import chunk_geometry, embeddings_to_plaintexts, odd_mask, plaintexts_to_digits, validate_embeddings
+from ._gpu_ext import KEY_BITS, pointwise_mulmod_limbs
+from .packing import (chunk_geometry, embeddings_to_plaintexts, odd_mask, plaintexts_below, plaintexts_to_digits,
+                      validate_embeddings)
 
 ALPHA_LEN = kg.DEFAULT_ALPHA_LEN
 
@@ -57,6 +67,16 @@ def _to_int(v) -> int:
         t = v.strip()
         return int(t, 16) if t[:2].lower() == "0x" else int(t, 10)
     return int(v)
+
+
+def _value_limbs(values: Sequence, limbs: int) -> np.ndarray:
+    """Values as LE bytes, ints, decimal or 0x-hex strings (big_from_py,
+    lcu:1301-1320) -> (count, limbs) uint32; bytes and int batches in bulk."""
+    if values and all(isinstance(v, (bytes, bytearray, memoryview)) for v in values):
+        return bytes_to_limbs(values, limbs)
+    if values and all(type(v) is int for v in values):
+        return ints_to_limbs(values, limbs)
+    return ints_to_limbs([_to_int(v) for v in values], limbs)
 
 
 def _with_crt(key: LookupKey, phi) -> LookupKey:
@@ -278,47 +298,80 @@ class PaillierGPULookupClient:
         idx = self._noise_idx(len(m)) if noise_idx is None else np.ascontiguousarray(noise_idx, dtype=np.uint8)
         return dev.encrypt_digits(digits, idx)
 
-    def encrypt(self, embeddings: Sequence[Sequence[int]]) -> list:
-        """[batch][embed_len] bits -> [batch][chunk_num] ciphertexts (lcu:1873-2002)."""
+    def _width(self) -> int:
+        return width_for_bits(self._keys["n_squared"].bit_length())
+
+    def _encrypt_embeddings(self, embeddings) -> tuple[int, np.ndarray]:
         self._require_keys("encrypt")
         bits = validate_embeddings(embeddings, self._embed_len)
-        batch = bits.shape[0]
-        if batch == 0:
-            return []
-        W = width_for_bits(self._keys["n_squared"].bit_length())
+        W = self._width()
+        if bits.shape[0] == 0:
+            return 0, np.zeros((0, W), dtype=np.uint32)
         m = embeddings_to_plaintexts(bits, self._key_len, W)
-        for v in limbs_to_ints(m):
-            if v >= self._keys["n"]:
-                raise ValueError("plaintext must be in range [0, n)")
-        cts = limbs_to_ints(self.encrypt_limbs(m))
-        cn = self._chunk_num
-        return [[_out(c) for c in cts[i * cn:(i + 1) * cn]] for i in range(batch)]
+        if not plaintexts_below(self._embed_len, self._key_len, self._keys["n"]) and \
+                not limbs_below(m, self._keys["n"]).all():
+            raise ValueError("plaintext must be in range [0, n)")
+        return bits.shape[0], self.encrypt_limbs(m)
+
+    def encrypt(self, embeddings: Sequence[Sequence[int]]) -> list:
+        """[batch][embed_len] bits -> [batch][chunk_num] ciphertexts (lcu:1873-2002).
+        A (batch, embed_len) numpy array of 0/1 is taken as it is."""
+        batch, c = self._encrypt_embeddings(embeddings)
+        cts, cn = [_out(v) for v in limbs_to_ints(c)], self._chunk_num
+        return [cts[i * cn:(i + 1) * cn] for i in range(batch)]
+
+    def encrypt_bytes(self, embeddings) -> list[list[bytes]]:
+        """``encrypt`` with minimal LE bytes per ciphertext."""
+        batch, c = self._encrypt_embeddings(embeddings)
+        by, cn = limbs_to_bytes(c), self._chunk_num
+        return [by[i * cn:(i + 1) * cn] for i in range(batch)]
+
+    def _plaintext_batch(self, values: Sequence) -> np.ndarray:
+        try:
+            m = _value_limbs(values, self._width())
+        except DoesNotFit as e:
+            raise RuntimeError("encrypt_cts: plaintext out of range [0, n)") from e
+        if not limbs_below(m, self._keys["n"]).all():
+            raise RuntimeError("encrypt_cts: plaintext out of range [0, n)")
+        return m
 
     def encrypt_cts(self, plaintexts: Sequence, noise_idx: Optional[np.ndarray] = None) -> list:
         """Integers in [0, n) -> ciphertexts (lcu:2006-2131)."""
         self._require_keys("encrypt_cts")
-        ms = [_to_int(p) for p in plaintexts]
-        if not ms:
+        if not len(plaintexts):
             return []
-        n = self._keys["n"]
-        for m in ms:
-            if not 0 <= m < n:
-                raise RuntimeError("encrypt_cts: plaintext out of range [0, n)")
-        W = width_for_bits(self._keys["n_squared"].bit_length())
-        return [_out(c) for c in limbs_to_ints(self.encrypt_limbs(ints_to_limbs(ms, W), noise_idx))]
+        return [_out(c) for c in limbs_to_ints(self.encrypt_limbs(self._plaintext_batch(plaintexts), noise_idx))]
+
+    def encrypt_cts_bytes(self, plaintexts: Sequence[bytes], noise_idx: Optional[np.ndarray] = None) -> list[bytes]:
+        """``encrypt_cts`` with LE bytes in and minimal LE bytes out."""
+        self._require_keys("encrypt_cts_bytes")
+        if not len(plaintexts):
+            return []
+        return limbs_to_bytes(self.encrypt_limbs(self._plaintext_batch(plaintexts), noise_idx))
 
     # ---- decryption ----
+    def _cipher_batch(self, values: Sequence) -> np.ndarray:
+        try:
+            C = _value_limbs(values, self._width())
+        except DoesNotFit as e:
+            raise ValueError("ciphertext must be in range [0, n^2)") from e
+        if not limbs_below(C, self._keys["n_squared"]).all():
+            raise ValueError("ciphertext must be in range [0, n^2)")
+        return C
+
     def decrypt_cts(self, ciphers: Sequence) -> list[int]:
         """m = L(c^a mod n^2) * g_a_inv mod n (lcu:2440-2499; lookup.py:257-278)."""
         self._require_keys("decrypt_cts")
-        cs = [_to_int(c) for c in ciphers]
-        if not cs:
+        if not len(ciphers):
             return []
-        n2 = self._keys["n_squared"]
-        for c in cs:
-            if not 0 <= c < n2:
-                raise ValueError("ciphertext must be in range [0, n^2)")
-        return limbs_to_ints(self._decrypt_key().decrypt(cs))
+        return limbs_to_ints(self._decrypt_key().decrypt(self._cipher_batch(ciphers)))
+
+    def decrypt_cts_bytes(self, ciphers: Sequence[bytes]) -> list[bytes]:
+        """``decrypt_cts`` with LE bytes in and minimal LE bytes out."""
+        self._require_keys("decrypt_cts_bytes")
+        if not len(ciphers):
+            return []
+        return limbs_to_bytes(self._decrypt_key().decrypt(self._cipher_batch(ciphers)))
 
     def _decrypt_key(self) -> LookupKey:
         if self._dev is not None:
@@ -335,17 +388,11 @@ class PaillierGPULookupClient:
         if batch == 0:
             return []
         cn = self._chunk_num
-        flat: list[int] = []
-        for row in ciphers_batch:
-            if len(row) != cn:
-                raise RuntimeError("decode_hamming_client: each inner list must have length == chunk_num")
-            flat.extend(_to_int(c) for c in row)
-        n2 = self._keys["n_squared"]
-        for c in flat:
-            if not 0 <= c < n2:
-                raise ValueError("ciphertext must be in range [0, n^2)")
-        counts = self._decrypt_key().decode_hamming(flat, odd_mask(self._chunk_len))
-        return [int(counts[i * cn:(i + 1) * cn].sum()) for i in range(batch)]
+        if any(len(row) != cn for row in ciphers_batch):
+            raise RuntimeError("decode_hamming_client: each inner list must have length == chunk_num")
+        C = self._cipher_batch([c for row in ciphers_batch for c in row])
+        counts = self._decrypt_key().decode_hamming(C, odd_mask(self._chunk_len))
+        return counts.reshape(batch, cn).sum(axis=1, dtype=np.int64).tolist()
 
     def decrypt_then_reencrypt(self, ciphers: Sequence, key_a: dict, key_b: dict) -> list:
         """Decrypt under key A, re-encrypt under key B's tables (lcu:2134-2285)."""
@@ -386,22 +433,37 @@ class PaillierGPULookupClient:
         return [_out(c) for c in limbs_to_ints(out)]
 
     # ---- evaluator ----
+    @staticmethod
+    def _server(ct1: Sequence, ct2: Sequence, n2: int, what: str) -> np.ndarray:
+        if len(ct1) != len(ct2):
+            raise RuntimeError(f"{what}: ct1 and ct2 must have the same length")
+        W = width_for_bits(n2.bit_length())
+        return pointwise_mulmod_limbs(_value_limbs(ct1, W), _value_limbs(ct2, W), n2)
+
     def encode_hamming_client(self, ct1: Sequence, ct2: Sequence) -> list:
         self._require_keys("encode_hamming_client")
-        if len(ct1) != len(ct2):
-            raise RuntimeError("encode_hamming_client: ct1 and ct2 must have the same length")
-        res = pointwise_mulmod([_to_int(c) for c in ct1], [_to_int(c) for c in ct2], self._keys["n_squared"])
-        return [_out(v) for v in res]
+        res = self._server(ct1, ct2, self._keys["n_squared"], "encode_hamming_client")
+        return [_out(v) for v in limbs_to_ints(res)]
+
+    @staticmethod
+    def _server_modulus(pk: dict, what: str) -> int:
+        if "n_squared" not in pk:
+            raise RuntimeError(f"{what}: pk must contain n_squared")
+        return _to_int(pk["n_squared"])
 
     @staticmethod
     def encode_hamming_server(ct1: Sequence, ct2: Sequence, pk: dict) -> list:
         """ct1[i] * ct2[i] mod pk["n_squared"] (int, decimal or 0x-hex), lcu:2312-2335."""
-        if len(ct1) != len(ct2):
-            raise RuntimeError("encode_hamming_server: ct1 and ct2 must have the same length")
-        if "n_squared" not in pk:
-            raise RuntimeError("encode_hamming_server: pk must contain n_squared")
-        res = pointwise_mulmod([_to_int(c) for c in ct1], [_to_int(c) for c in ct2], _to_int(pk["n_squared"]))
-        return [_out(v) for v in res]
+        C = PaillierGPULookupClient
+        n2 = C._server_modulus(pk, "encode_hamming_server")
+        return [_out(v) for v in limbs_to_ints(C._server(ct1, ct2, n2, "encode_hamming_server"))]
+
+    @staticmethod
+    def encode_hamming_server_bytes(ct1: Sequence[bytes], ct2: Sequence[bytes], pk: dict) -> list[bytes]:
+        """``encode_hamming_server`` with LE bytes in and minimal LE bytes out."""
+        C = PaillierGPULookupClient
+        n2 = C._server_modulus(pk, "encode_hamming_server_bytes")
+        return limbs_to_bytes(C._server(ct1, ct2, n2, "encode_hamming_server_bytes"))
 
 
 # Backwards-compatible alias (lcu:2669-2670, 2720)

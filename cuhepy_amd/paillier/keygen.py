@@ -11,9 +11,14 @@ Here it is plain Python integers. Randomness comes from ``secrets`` unless a
 from __future__ import annotations
 
 import math
+import os
 import random
 import secrets
 from typing import Optional
+
+import numpy as np
+
+from ..limbs import limbs_below
 
 MSG_BITS = 8                 # lookup.py:11
 NOISE_TABLE_SIZE = 2 ** 8    # lookup.py:12
@@ -106,6 +111,29 @@ def random_r(n: int, rng: Optional[random.Random] = None) -> int:
         v = r.randrange(n - 1) + 1
         if math.gcd(v, n) == 1:
             return v
+
+
+def random_below_limbs(n: int, count: int, limbs: int) -> np.ndarray:
+    """`count` values uniform in [1, n-1] as (count, limbs) uint32, from the
+    operating system's random bytes: draws of bit_length(n - 1) bits,
+    rejecting 0 and values >= n (at most half of them). Coprimality with n
+    is the caller's check (``Modulus.product`` over the batch)."""
+    kb = (n - 1).bit_length()
+    words = (kb + 31) // 32
+    if words > limbs:
+        raise ValueError("n does not fit the width")
+    top = np.uint32((1 << (kb - 32 * (words - 1))) - 1)
+    out = np.zeros((count, limbs), dtype=np.uint32)
+    have = 0
+    while have < count:
+        draw = (count - have) * 2 + 8
+        x = np.zeros((draw, limbs), dtype=np.uint32)
+        x[:, :words] = np.frombuffer(os.urandom(draw * words * 4), dtype="<u4").reshape(draw, words)
+        x[:, words - 1] &= top
+        x = x[limbs_below(x, n) & x.any(axis=1)][: count - have]
+        out[have:have + len(x)] = x
+        have += len(x)
+    return out
 
 
 # ---------------------------------------------------------------------------

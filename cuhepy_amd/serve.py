@@ -6,25 +6,37 @@ paillier_GPU_lookup_client.cu:2312-2329): a query ciphertext y is multiplied
 into every index ciphertext x_i mod n^2, giving the encrypted Hamming
 distances that the client decrypts. The reference runs one call per query
 from Python; here a request loop keeps the index resident in HBM and serves
-many queries at once:
+many queries at once. Two batching modes:
 
-* cyclic scan: the index is cut into ``nchunks`` ranges; every scheduler
-  iteration launches ONE kernel that multiplies the next range by the
-  operands of all active queries (``Modulus.mulmod_multi_dev``; inside the
-  launch the work runs query by query over the range, so the workgroups share
-  one or two operand tables at a time);
-* continuous batching: a query that arrives while others are on the GPU joins
-  at the next iteration, at whatever range the scan is on, and is complete
-  after ``nchunks`` iterations (it has then seen every range once); finished
-  queries leave at iteration boundaries;
-* memory admission: each active query holds a result buffer (one ciphertext
-  per index entry); a waiting query is admitted only when a released buffer
-  can be reused or free device memory (minus a reserve) covers a new one,
-  and at most ``max_active`` (<= 32 operands per launch) are active;
-* latency is measured to device completion (the event recorded after the
-  query's last iteration), from the request's arrival time.
+* ``fifo`` (the default): admitted queries run in arrival order. Whenever the
+  device queue has room (``depth`` launches in flight), ONE launch takes every
+  admitted query not yet launched (up to ``max_active``) over the whole
+  index. Inside the launch the workgroups take the tiles query by query, so
+  the queries complete one after another (each overlapping only the start of
+  the next), with no gap between them as between launches. Each query
+  is complete when its last tile is: the kernel stamps the device clock then
+  (``Modulus.mulmod_multi_stamped_dev``), and the stamps are converted to
+  host time against the event recorded after the launch. A query arriving
+  while a launch runs joins the next one, which is queued behind it on the
+  stream.
+* ``scan``: a cyclic scan over ``nchunks`` index ranges. Every iteration
+  multiplies the next range by all active queries; a query arriving while
+  others run joins at the next iteration, at whatever range the scan is on,
+  and completes after ``nchunks`` iterations (processor sharing).
 
-The scheduler and the admission policy are plain Python, independent of the
+The evaluator is bound by the integer multiply-add rate (32 multiply-adds per
+byte), so a query costs the same in a batch as alone. With equal work per
+query, FIFO order gives the lowest latency; processor sharing (``scan``)
+makes every active query wait for all the others. ``fifo`` keeps each
+query's own completion time and removes the gaps between launches.
+
+Memory admission: each active query holds a result buffer (one ciphertext per
+index entry). A waiting query is admitted only when a released buffer can be
+reused, or free device memory minus a reserve covers a new one. At most
+``max_active`` (<= 32 operands per launch) are active.
+
+Latency is measured to device completion from the request's arrival time.
+The schedulers and the admission policy are plain Python, independent of the
 device, so their invariants are tested on the CPU.
 """
 
@@ -138,29 +150,81 @@ class ScanScheduler:
         return not self.waiting and not self.active
 
 
+class FifoBatcher:
+    """FIFO continuous batching: admitted requests are launched in arrival
+    order; a launch takes every admitted request not yet launched (up to
+    `max_active` active in all, launched or not)."""
+
+    def __init__(self, max_active: int = MQ_MAX):
+        if not 1 <= max_active <= MQ_MAX:
+            raise ValueError(f"max_active must be in [1, {MQ_MAX}]")
+        self.max_active = max_active
+        self.waiting: collections.deque[Request] = collections.deque()
+        self.pending: list[Request] = []   # admitted, not launched
+        self.running = 0                   # launched, not complete
+
+    def submit(self, req: Request) -> None:
+        self.waiting.append(req)
+
+    def admit(self, admission: MemoryAdmission, now: float, on_admit: Callable[[Request, str], None],
+              limit: int = MQ_MAX) -> int:
+        n = 0
+        while self.waiting and len(self.pending) + self.running < self.max_active and n < limit:
+            how = admission.try_reserve()
+            if how is None:
+                break
+            req = self.waiting.popleft()
+            req.admitted = now
+            on_admit(req, how)
+            self.pending.append(req)
+            n += 1
+        return n
+
+    def next_batch(self) -> list[Request]:
+        batch, self.pending = self.pending, []
+        self.running += len(batch)
+        return batch
+
+    def complete(self, batch: list[Request]) -> None:
+        self.running -= len(batch)
+        assert self.running >= 0
+
+    @property
+    def active(self) -> list[Request]:
+        return self.pending
+
+    def idle(self) -> bool:
+        return not self.waiting and not self.pending and self.running == 0
+
+
 class KnnServer:
-    """Resident encrypted index + the scheduler above driving the multi-query kernel.
+    """Resident encrypted index + a scheduler driving the multi-query kernel.
 
     index: device tensor (limbs, stride) int32, interleaved ciphertexts under
-    modulus n^2 (a ``Modulus``); `count` entries. One launch covers one range
-    of `chunk` entries for all active queries.
+    modulus n^2 (a ``Modulus``); `count` entries. mode "fifo": one launch
+    covers the whole index for a batch of queries in arrival order; mode
+    "scan": one launch covers one range of `chunk` entries for all active
+    queries.
     """
 
     def __init__(self, ctx, modulus, index, count: int, chunk: Optional[int] = None, max_active: int = MQ_MAX,
                  reserve_bytes: int = 4 << 30, free_bytes: Optional[Callable[[], int]] = None, depth: int = 2,
-                 clock: Callable[[], float] = time.perf_counter):
+                 clock: Callable[[], float] = time.perf_counter, mode: str = "fifo"):
         import torch
 
+        if mode not in ("fifo", "scan"):
+            raise ValueError("mode must be 'fifo' or 'scan'")
         self.torch = torch
+        self.mode = mode
         self.ctx, self.mod, self.index = ctx, modulus, index
         self.limbs = modulus.limbs
         self.count = int(count)
         self.stride = int(index.shape[1])
         if index.shape[0] != self.limbs or self.stride < self.count:
             raise ValueError("index must be (limbs, stride >= count)")
-        self.chunk = int(chunk or self.count)
+        self.chunk = self.count if mode == "fifo" else int(chunk or self.count)
         self.nchunks = math.ceil(self.count / self.chunk)
-        self.sched = ScanScheduler(self.nchunks, max_active)
+        self.sched = FifoBatcher(max_active) if mode == "fifo" else ScanScheduler(self.nchunks, max_active)
         self.device = torch.device(f"cuda:{ctx.device}")
         self.result_bytes = self.limbs * self.stride * 4
         if free_bytes is None:
@@ -182,6 +246,21 @@ class KnnServer:
         self.base_event.record(self.stream)
         self.base_event.synchronize()
         self.t_base = clock()
+        # query ciphertexts are staged in pinned host rows, so the copy that
+        # feeds a query's operand preparation is asynchronous (a pageable
+        # source would make the host wait for the launches in flight); a row
+        # is reused once its query retires
+        import numpy as np
+        rows = (depth + 1) * MQ_MAX + max_active
+        self.y_host = torch.empty((rows, self.limbs), dtype=torch.int32, pin_memory=True)
+        self.y_np = self.y_host.numpy().view(np.uint32)
+        self.y_free = list(range(rows))
+        if mode == "fifo":  # per-launch completion counters and stamps, reused once retired
+            self.clock_hz = ctx.wall_clock_hz()
+            with torch.cuda.stream(self.stream):
+                self.stamp_sets = [(torch.zeros(MQ_MAX + 1, dtype=torch.int32, device=self.device),
+                                    torch.zeros(MQ_MAX, dtype=torch.int64, device=self.device))
+                                   for _ in range(depth + 1)]
 
     # ---- requests ----
     def submit(self, y: int, arrival: Optional[float] = None, keep: bool = False) -> Request:
@@ -201,28 +280,38 @@ class KnnServer:
             op = torch.empty((self.mod.operand_words,), dtype=torch.int32, device=self.device)
         # the query's operand is prepared on the library's stream, behind the
         # iterations in flight and ahead of the next launch: no synchronisation
-        y = ints_to_limbs([req.y], self.limbs)
-        self.mod.operands(y, op)
-        req._buf, req._op, req._y = buf, op, y
+        row = self.y_free.pop()
+        self.y_np[row] = ints_to_limbs([req.y], self.limbs)[0]
+        self.mod.operands(self.y_np[row:row + 1], op)
+        req._buf, req._op, req._row = buf, op, row
 
     # ---- loop ----
     def _retire(self, block: bool) -> int:
         n = 0
         while self.inflight:
-            ev, finished = self.inflight[0]
+            ev, finished, stamps = self.inflight[0]
             if not block and not ev.query():
                 break
             ev.synchronize()
             self.inflight.popleft()
             t = self.t_base + self.base_event.elapsed_time(ev) / 1e3
+            if stamps is not None:  # fifo: each query completed at its own stamp, the last one at the event
+                self.stamp_sets.append(stamps)
+                st = stamps[1][:len(finished)].cpu().tolist()
+                last = max(st)
+                for r, s_q in zip(finished, st):
+                    r.done = t - (last - s_q) / self.clock_hz
+                self.sched.complete(finished)
             for r in finished:
-                r.done = t
+                if stamps is None:
+                    r.done = t
                 if r.keep:
                     r.result = r._buf
                 else:
                     self.pool.append(r._buf)
                     self.admission.release()
-                r._buf = r._op = r._y = None
+                self.y_free.append(r._row)
+                r._buf = r._op = r._row = None
                 self.completed.append(r)
             n += 1
             block = False
@@ -240,18 +329,33 @@ class KnnServer:
         queue depth allows; returns whether anything was launched."""
         self._retire(block=False)
         if len(self.inflight) >= self.depth:
+            if self.mode == "fifo":  # prepare the queries that will share the next launch
+                self.sched.admit(self.admission, self.clock(), self._on_admit)
             return False
-        self.sched.admit(self.admission, self.clock(), self._on_admit)
+        if self.mode == "fifo" and not self.sched.pending:
+            # the device queue has room: admit and launch the next query now
+            # (queries admitted while the queue is full wait as one batch)
+            self.sched.admit(self.admission, self.clock(), self._on_admit, limit=1)
+        else:
+            self.sched.admit(self.admission, self.clock(), self._on_admit)
         if not self.sched.active:
             return False
-        chunk, batch, finished = self.sched.next_iteration()
-        begin = chunk * self.chunk
-        cnt = min(self.chunk, self.count - begin)
-        self.mod.mulmod_multi_dev(self.index, self.stride, begin, cnt, [r._op for r in batch],
-                                  [r._buf for r in batch], self.stride)
+        if self.mode == "fifo":
+            batch = finished = self.sched.next_batch()
+            begin, cnt = 0, self.count
+            stamps = self.stamp_sets.pop()
+            self.mod.mulmod_multi_stamped_dev(self.index, self.stride, begin, cnt, [r._op for r in batch],
+                                              [r._buf for r in batch], self.stride, stamps[0], stamps[1])
+        else:
+            chunk, batch, finished = self.sched.next_iteration()
+            begin = chunk * self.chunk
+            cnt = min(self.chunk, self.count - begin)
+            stamps = None
+            self.mod.mulmod_multi_dev(self.index, self.stride, begin, cnt, [r._op for r in batch],
+                                      [r._buf for r in batch], self.stride)
         ev = self.torch.cuda.Event(enable_timing=True)
         ev.record(self.stream)
-        self.inflight.append((ev, finished))
+        self.inflight.append((ev, finished, stamps))
         self.iterations += 1
         self.launched_pairs += len(batch) * cnt
         return True

@@ -119,6 +119,20 @@ int chx_mulmod_operands(chx_ctx* ctx, const chx_mod* m, const uint32_t* y, uint3
 int chx_mulmod_multi_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* x, size_t stride, size_t begin,
                          size_t count, int nq, const uint32_t* const* ops, uint32_t* const* outs,
                          size_t out_stride);
+/* chx_mulmod_multi_dev that also reports when each query is complete: done
+ * (device, nq + 1 words, zeroed by the call on the stream) counts each
+ * query's finished tiles and holds the head of a queue of (query, tile)
+ * units, and stamps[q] (device, nq u64) receives the device clock
+ * (chx_ctx_wall_clock_hz) when query q's last tile finished. The workgroups
+ * take the units from the queue in order, query by query, so the queries
+ * complete one after another, each overlapping only the start of the next
+ * (no reference counterpart: the reference serves one query per call). */
+int chx_mulmod_multi_stamped_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* x, size_t stride, size_t begin,
+                                 size_t count, int nq, const uint32_t* const* ops, uint32_t* const* outs,
+                                 size_t out_stride, uint32_t* done, uint64_t* stamps);
+/* frequency of the device clock the stamps above count (Hz), measured
+ * against the host's steady clock over about 20 ms */
+int chx_ctx_wall_clock_hz(chx_ctx* ctx, double* hz);
 /* out[i] = x[i]*y mod n with y shared (one query, canonical engine) */
 int chx_mulmod_bcast_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* x, const uint32_t* y_host,
                          uint32_t* out, size_t count, size_t stride);

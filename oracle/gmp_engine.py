@@ -177,6 +177,62 @@ class CrtDecryptor:
         return t3.get()
 
 
+class Evaluator:
+    """The reference's CPU evaluate loop, int(mpz(a) * y % n^2) per index
+    entry (benchmarks/bfv_client_matrix.py:166-169), with GMP: y and n^2 are
+    converted once; each call converts a, multiplies, reduces and returns an
+    int, as that loop does."""
+
+    def __init__(self, y: int, n_squared: int):
+        self.lib = _load()
+        self.y, self.m = Mpz(y), Mpz(n_squared)
+        self.a, self.r = Mpz(), Mpz()
+
+    def __call__(self, a: int) -> int:
+        lib = self.lib
+        self.a.set(a)
+        lib.mul(self.r.ref, self.a.ref, self.y.ref)
+        lib.mod(self.r.ref, self.r.ref, self.m.ref)
+        return self.r.get()
+
+
+class LookupEncryptor:
+    """PaillierLookup.encrypt (lookup.py:213-252) with GMP: c = prod_i
+    g_table[i][digit_i(m)] * prod_k noise[idx_k] mod n^2, one table mulmod per
+    8-bit digit (zero digits included, as the reference loops over every
+    chunk) and one per noise factor. Table entries are converted to mpz once
+    and cached (the reference's tables are mpz already)."""
+
+    def __init__(self, table, noise: list[int], n_squared: int, chunks: int):
+        self.lib = _load()
+        self.table, self.chunks = table, chunks
+        self.noise = [Mpz(v) for v in noise]
+        self.m = Mpz(n_squared)
+        self.cache: dict = {}
+        self.acc = Mpz()
+
+    def entry(self, i: int, d: int) -> "Mpz":
+        e = self.cache.get((i, d))
+        if e is None:
+            e = self.cache[(i, d)] = Mpz(self.table[i][d])
+        return e
+
+    def prepare(self, m: int) -> None:
+        for i in range(self.chunks):
+            self.entry(i, (m >> (8 * i)) & 0xFF)
+
+    def __call__(self, m: int, noise_idx) -> int:
+        lib, acc, mod = self.lib, self.acc, self.m.ref
+        acc.set(1)
+        for i in range(self.chunks):
+            lib.mul(acc.ref, acc.ref, self.entry(i, (m >> (8 * i)) & 0xFF).ref)
+            lib.mod(acc.ref, acc.ref, mod)
+        for k in noise_idx:
+            lib.mul(acc.ref, acc.ref, self.noise[k].ref)
+            lib.mod(acc.ref, acc.ref, mod)
+        return acc.get()
+
+
 def host_cores() -> int:
     """Host cores this process may use (affinity-aware)."""
     try:

@@ -105,7 +105,50 @@ def _server(ctx, mod, il, count, **kw):
     return KnnServer(ctx, mod, il, count, **kw)
 
 
-def test_server_alone_equals_mix(ctx):
+def _active(srv):
+    return len(srv.sched.active) + getattr(srv.sched, "running", 0)
+
+
+def test_mulmod_multi_stamped(ctx):
+    """The stamped launch gives the unstamped launch's products; every query
+    gets a stamp, and the queries complete in order (the first before the last)."""
+    import torch
+    from cuhepy_amd.engine import Modulus
+    from cuhepy_amd.limbs import ints_to_limbs
+    N = _modulus(2048, 21)
+    mod = Modulus(ctx, N, 128)
+    rng = random.Random(21)
+    count = 60000
+    xs = [rng.randrange(N) for _ in range(count)]
+    il, stride = _index(ctx, N, 128, xs)
+    ys = [rng.randrange(N) for _ in range(3)]
+    dev = torch.device("cuda:0")
+    ops = [torch.empty(mod.operand_words, dtype=torch.int32, device=dev) for _ in ys]
+    for y, op in zip(ys, ops):
+        mod.operands(ints_to_limbs([y], 128), op)
+    outs_a = [torch.full((128, stride), -1, dtype=torch.int32, device=dev) for _ in ys]
+    outs_b = [torch.full((128, stride), -1, dtype=torch.int32, device=dev) for _ in ys]
+    done = torch.full((32,), 7, dtype=torch.int32, device=dev)
+    stamps = torch.zeros(32, dtype=torch.int64, device=dev)
+    torch.cuda.synchronize()  # the fills above ran on torch's stream, the library uses its own
+    ctx.sync()
+    mod.mulmod_multi_dev(il, stride, 0, count, ops, outs_a, stride)
+    mod.mulmod_multi_stamped_dev(il, stride, 0, count, ops, outs_b, stride, done, stamps)
+    ctx.sync()
+    for a, b in zip(outs_a, outs_b):
+        assert torch.equal(a[:, :count], b[:, :count])
+    tiles = (count + 63) // 64
+    # counters complete; the queue head passed every unit; the words beyond untouched
+    assert done[:3].tolist() == [tiles] * 3 and done[3].item() >= 3 * tiles and done[4:].tolist() == [7] * 28
+    st = stamps[:3].tolist()
+    assert all(v > 0 for v in st) and st[0] < st[1] < st[2]
+    assert ctx.wall_clock_hz() > 1e6
+    got = _read(ctx, outs_b[1], 64, 128, stride)
+    assert got == [x * ys[1] % N for x in xs[:64]]
+
+
+@pytest.mark.parametrize("mode", ["scan", "fifo"])
+def test_server_alone_equals_mix(ctx, mode):
     """A query's result is identical served alone and admitted into a running mix."""
     from cuhepy_amd.engine import Modulus
     N = _modulus(2048, 11)
@@ -115,12 +158,12 @@ def test_server_alone_equals_mix(ctx):
     xs = [rng.randrange(N) for _ in range(count)]
     il, stride = _index(ctx, N, 128, xs)
     ys = [rng.randrange(N) for _ in range(6)]
-    srv = _server(ctx, mod, il, count, chunk=512, max_active=4, reserve_bytes=0)
+    srv = _server(ctx, mod, il, count, chunk=512, max_active=4, reserve_bytes=0, mode=mode)
     alone = srv.submit(ys[0], keep=True)
     srv.drain()
     ref = _read(ctx, alone.result, count, 128, stride)
     assert ref == [x * ys[0] % N for x in xs]
-    # the mix: two queries start, the tested query joins mid-scan, more arrive after it
+    # the mix: two queries start, the tested query joins while they run, more arrive after it
     reqs = [srv.submit(ys[1], keep=True), srv.submit(ys[2], keep=True)]
     srv.pump()
     srv.pump()
@@ -128,8 +171,9 @@ def test_server_alone_equals_mix(ctx):
     srv.pump()
     reqs += [srv.submit(y, keep=True) for y in ys[3:]]
     srv.drain()
-    assert mixed.start_chunk != 0  # joined mid-scan
-    assert sorted(mixed.chunks_seen) == list(range(srv.nchunks))
+    if mode == "scan":
+        assert mixed.start_chunk != 0  # joined mid-scan
+        assert sorted(mixed.chunks_seen) == list(range(srv.nchunks))
     assert _read(ctx, mixed.result, count, 128, stride) == ref
     for r in reqs:
         got = _read(ctx, r.result, count, 128, stride)
@@ -137,7 +181,8 @@ def test_server_alone_equals_mix(ctx):
     assert all(r.latency is not None and r.latency >= 0 for r in reqs + [mixed])
 
 
-def test_server_memory_admission(ctx):
+@pytest.mark.parametrize("mode", ["scan", "fifo"])
+def test_server_memory_admission(ctx, mode):
     """With room for two result buffers, five queries still all complete,
     never more than two at once, reusing the pooled buffers."""
     from cuhepy_amd.engine import Modulus
@@ -149,7 +194,8 @@ def test_server_memory_admission(ctx):
     il, stride = _index(ctx, N, 64, xs)
     per = 64 * stride * 4
     state = {"free": 2 * per}
-    srv = _server(ctx, mod, il, count, chunk=256, max_active=8, reserve_bytes=0, free_bytes=lambda: state["free"])
+    srv = _server(ctx, mod, il, count, chunk=256, max_active=8, reserve_bytes=0, free_bytes=lambda: state["free"],
+                  mode=mode)
     orig = srv._on_admit
 
     def on_admit(req, how):
@@ -164,7 +210,7 @@ def test_server_memory_admission(ctx):
     while not srv.sched.idle() or srv.inflight:
         if not srv.pump() and srv.inflight:
             srv._retire(block=True)
-        peak = max(peak, len(srv.sched.active))
+        peak = max(peak, _active(srv))
     assert peak <= 2 and srv.admission.allocated == 2
     assert all(r.done is not None for r in reqs)
 

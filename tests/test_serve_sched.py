@@ -1,11 +1,12 @@
-"""Served-kNN host logic on the CPU: cyclic-scan continuous batching and
-memory-based admission (no device)."""
+"""Served-kNN host logic on the CPU: FIFO and cyclic-scan continuous batching
+and memory-based admission (no device)."""
 
 import random
 
 import pytest
 
-from cuhepy_amd.serve import MQ_MAX, MemoryAdmission, Request, ScanScheduler, latency_stats, poisson_arrivals
+from cuhepy_amd.serve import (MQ_MAX, FifoBatcher, MemoryAdmission, Request, ScanScheduler, latency_stats,
+                               poisson_arrivals)
 
 
 def _drive(sched, adm, arrivals_per_iter, iters, release_finished=True):
@@ -109,3 +110,65 @@ def test_poisson_and_percentiles():
         reqs.append(r)
     s = latency_stats(reqs)
     assert s["n"] == 100 and abs(s["p50_ms"] - 50.0) < 1e-9 and abs(s["p99_ms"] - 99.0) < 1e-9
+
+
+def test_fifo_batcher_launches_in_arrival_order():
+    """Every request is launched exactly once, in arrival order; a launch takes
+    every admitted request not yet launched; at most max_active are active."""
+    rng = random.Random(3)
+    sched = FifoBatcher(max_active=5)
+    adm = MemoryAdmission(lambda: 1 << 40, request_bytes=1)
+    rid, launched, inflight = 0, [], []
+    for step in range(300):
+        for _ in range(rng.randrange(4) if step < 200 else 0):
+            sched.submit(Request(rid, 0, float(step)))
+            rid += 1
+        sched.admit(adm, float(step), lambda r, how: None)
+        assert len(sched.pending) + sched.running <= 5
+        if len(inflight) < 2 and sched.pending:
+            batch = sched.next_batch()
+            assert batch and not sched.pending
+            launched += [r.rid for r in batch]
+            inflight.append(batch)
+        if inflight and rng.random() < 0.6:  # the oldest launch completes
+            done = inflight.pop(0)
+            sched.complete(done)
+            for _ in done:
+                adm.release()
+    while inflight:
+        sched.complete(inflight.pop(0))
+    while not sched.idle():  # drain
+        sched.admit(adm, 0.0, lambda r, how: None)
+        b = sched.next_batch()
+        launched += [r.rid for r in b]
+        sched.complete(b)
+        for _ in b:
+            adm.release()
+    assert launched == list(range(rid))
+    assert sched.idle() and adm.held == 0
+
+
+def test_fifo_batcher_memory_and_bad_parameters():
+    free = {"b": 250}
+    sched = FifoBatcher(max_active=MQ_MAX)
+    adm = MemoryAdmission(lambda: free["b"], request_bytes=100, reserve=0)
+
+    def on_admit(r, how):
+        if how == "new":
+            free["b"] -= 100
+
+    for i in range(4):
+        sched.submit(Request(i, 0, 0.0))
+    assert sched.admit(adm, 0.0, on_admit) == 2 and len(sched.waiting) == 2
+    b = sched.next_batch()
+    assert [r.rid for r in b] == [0, 1] and sched.running == 2
+    sched.complete(b)
+    adm.release()
+    adm.release()
+    hows = []
+    sched.admit(adm, 1.0, lambda r, how: hows.append(how))
+    assert hows == ["pool", "pool"] and adm.allocated == 2
+    with pytest.raises(ValueError):
+        FifoBatcher(0)
+    with pytest.raises(ValueError):
+        FifoBatcher(MQ_MAX + 1)
