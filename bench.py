@@ -733,42 +733,92 @@ def serve_leg(ctx, args, peak: float) -> dict:
         log(f"serve: CPU evaluate loop (GMP) {ns / cpu_s:,.0f} mulmod/s single thread = "
             f"{count * cpu_s / ns:.2f} s per query over the index; equal to the GPU on 256 entries")
 
-    # 2) saturation: 64 queries submitted at once, batched (FIFO launches of
-    # every admitted query) and plain (one launch per query)
-    srv = KnnServer(ctx, mod, index, count, max_active=32, mode="fifo")
-    _, ys = queries(64)
-    t0 = time.perf_counter()
-    for y in ys:
-        srv.submit(y)
-    srv.drain()
-    sat = len(ys) / (time.perf_counter() - t0)
-    t0 = time.perf_counter()
-    for y in ys:
-        plain.submit(y)
-    plain.drain()
-    sat_plain = len(ys) / (time.perf_counter() - t0)
-    log(f"serve: saturation {sat:.1f} queries/s batched ({srv.iterations} launches; stamp clock "
-        f"{srv.clock_hz / 1e6:.2f} MHz measured), {sat_plain:.1f} plain")
+    # 1d) the kernel alone: B queries in one stamped launch against B single
+    # launches (host synchronised around each form; best of four)
+    from cuhepy_amd.limbs import ints_to_limbs
+    _, kys = queries(32)
+    kops = [torch.empty((mod.operand_words,), dtype=torch.int32, device=dev) for _ in kys]
+    for y, op in zip(kys, kops):
+        mod.operands(ints_to_limbs([y], W), op)
+    kouts = [torch.empty((W, stride), dtype=torch.int32, device=dev) for _ in kys]
+    kdone = torch.zeros(64, dtype=torch.int32, device=dev)
+    kst = torch.zeros(32, dtype=torch.int64, device=dev)
+    torch.cuda.synchronize()
+    ctx.sync()
+    batching_kernel = {}
+    for B in (1, 4, 32):
+        ts, tb = [], []
+        for _ in range(4):
+            ctx.sync()
+            t0 = time.perf_counter()
+            for q in range(B):
+                mod.mulmod_multi_dev(index, stride, 0, count, [kops[q]], [kouts[q]], stride)
+            ctx.sync()
+            ts.append((time.perf_counter() - t0) / B)
+            t0 = time.perf_counter()
+            mod.mulmod_multi_stamped_dev(index, stride, 0, count, kops[:B], kouts[:B], stride, kdone, kst)
+            ctx.sync()
+            tb.append((time.perf_counter() - t0) / B)
+        batching_kernel[str(B)] = {"single_launches_ms_per_query": round(min(ts) * 1e3, 4),
+                                   "one_launch_ms_per_query": round(min(tb) * 1e3, 4)}
+    del kouts
+    log(f"serve: kernel-level batching, ms per query (B single launches | one launch of B): "
+        + ", ".join(f"B={b}: {v['single_launches_ms_per_query']:.3f} | {v['one_launch_ms_per_query']:.3f}"
+                    for b, v in batching_kernel.items()))
 
-    # 3) Poisson arrivals at a sweep of rates (fractions of the plain saturation)
+    # 2) saturation: 128 queries submitted at once, batched (FIFO launches of
+    # up to 32 admitted queries, 64 admitted at most) and plain (one launch per
+    # query); the two forms alternate, twice each, and each keeps its best run
+    srv = KnnServer(ctx, mod, index, count, max_active=64, mode="fifo")
+    _, ys = queries(128)
+    for server in (srv, plain):  # warm-up: every result buffer of the pools allocated
+        for y in ys:
+            server.submit(y)
+        server.drain()
+    sat_runs = {"batched": [], "plain": []}
+    launches = 0
+    for _ in range(2):
+        for name, server in (("batched", srv), ("plain", plain)):
+            it0 = server.iterations
+            t0 = time.perf_counter()
+            for y in ys:
+                server.submit(y)
+            server.drain()
+            sat_runs[name].append(round(len(ys) / (time.perf_counter() - t0), 2))
+            if name == "batched":
+                launches = server.iterations - it0
+    sat, sat_plain = max(sat_runs["batched"]), max(sat_runs["plain"])
+    log(f"serve: saturation {sat:.1f} queries/s batched ({launches} launches for {len(ys)} queries; stamp clock "
+        f"{srv.clock_hz / 1e6:.2f} MHz measured), {sat_plain:.1f} plain; runs {sat_runs}")
+
+    # 3) Poisson arrivals at a sweep of rates (fractions of the plain
+    # saturation): the same arrival trace for both forms, twice each in
+    # alternating order, latencies pooled per form
     sweep = []
     rng = random.Random(args.seed + 10)
     _, pool_ys = queries(args.serve_queries)
-    for frac in (0.25, 0.5, 0.75, 0.9, 0.97):
+    for k, frac in enumerate((0.25, 0.5, 0.75, 0.9, 0.97)):
         rate = frac * sat_plain
         offs = poisson_arrivals(rate, args.serve_queries, rng)
         row = {"rate_qps": round(rate, 2)}
-        for name, server in (("batched", srv), ("plain", plain)):
-            server.completed.clear()
-            t0 = time.perf_counter()
-            reqs = server.run_trace(list(zip(offs, pool_ys)))
-            wall = time.perf_counter() - t0
-            st = latency_stats(reqs)
+        pooled = {"batched": [], "plain": []}
+        walls = {"batched": 0.0, "plain": 0.0}
+        order = (("batched", srv), ("plain", plain))
+        for rep in range(2):
+            for name, server in (order if (k + rep) % 2 == 0 else order[::-1]):
+                server.completed.clear()
+                t0 = time.perf_counter()
+                pooled[name] += server.run_trace(list(zip(offs, pool_ys)))
+                walls[name] += time.perf_counter() - t0
+        for name in ("batched", "plain"):
+            st = latency_stats(pooled[name])
             row[name] = {"p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
-                         "mean_ms": round(st["mean_ms"], 3), "throughput_qps": round(len(reqs) / wall, 2)}
+                         "mean_ms": round(st["mean_ms"], 3), "queries": st["n"],
+                         "throughput_qps": round(len(pooled[name]) / walls[name], 2)}
         sweep.append(row)
         log(f"serve: {rate:7.1f} q/s: batched p50 {row['batched']['p50_ms']:.2f} p99 {row['batched']['p99_ms']:.2f} "
-            f"ms | plain p50 {row['plain']['p50_ms']:.2f} p99 {row['plain']['p99_ms']:.2f} ms")
+            f"ms | plain p50 {row['plain']['p50_ms']:.2f} p99 {row['plain']['p99_ms']:.2f} ms "
+            f"({row['plain']['queries']} queries each)")
     return {
         "workload": f"{count} lookup-encrypted 512-bit vectors (key_len 1024, n^2 4096-bit) resident in HBM; "
                     "each query = one mulmod per index entry (encode_hamming_server)",
@@ -788,10 +838,15 @@ def serve_leg(ctx, args, peak: float) -> dict:
         "cpu_baseline": cpu_eval,
         "saturation_qps_batched": round(sat, 2),
         "saturation_qps_plain": round(sat_plain, 2),
-        "batched": "fifo: each launch takes every admitted query not yet launched (up to 32) over the whole "
-                   "index, tiles query by query; a query completes at its last tile (device clock stamp)",
+        "saturation_runs": sat_runs,
+        "batching_kernel": batching_kernel,
+        "saturation_queries": len(ys),
+        "batched": "fifo: each launch takes the admitted queries not yet launched (up to 32; up to 64 admitted "
+                   "and prepared) over the whole index, tiles query by query; a query completes at its last tile "
+                   "(device clock stamp); a launch of one query is the plain launch",
         "plain": "one launch per query, whole index",
-        "max_active": 32, "queries_per_rate": args.serve_queries,
+        "max_active": 64, "queries_per_launch_max": 32, "queries_per_rate": 2 * args.serve_queries,
+        "sweep_method": "one Poisson trace per rate, replayed twice per form in alternating order, latencies pooled",
         "latency_endpoint": "device-complete from arrival (plain: event after the query's launch; batched: the "
                             "stamp of the query's last tile, anchored to the event after its launch)",
         "poisson_sweep": sweep,

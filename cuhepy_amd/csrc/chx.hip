@@ -1367,9 +1367,11 @@ static void multi_launch(chx_ctx* ctx, const chx_mod* m, const uint32_t* x, size
     if (!done || !stamps) fail(CHX_EINVAL, "done and stamps go together");
     mq.done = done;
     mq.stamp = reinterpret_cast<u64*>(stamps);
-    // counters and the unit queue's head from zero (a fill kernel, no memset
-    // node if the caller captures)
+    // counters, the unit queue's head and the stamps from zero (fill kernels,
+    // no memset node if the caller captures)
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, done, 0u, nq + 1);
+    check_launch("k_fill_u32");
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, reinterpret_cast<u32*>(stamps), 0u, 2 * nq);
     check_launch("k_fill_u32");
     mq.head = done + nq;
   } else {

@@ -989,7 +989,7 @@ struct MultiQuery {
   u32* out[MQ_MAX];       // interleaved [L32][out_stride], indexed by the global entry
   u32* head;              // head of the queue of (query, tile) units, zeroed before the launch
   u32* done;              // optional (nullptr): finished units per query, zeroed before the launch
-  u64* stamp;             // with done: device wall clock when each query's last unit finished
+  u64* stamp;             // with done: device wall clock when each query's last unit finished (zeroed before the launch)
 };
 
 // ---------------------------------------------------------------------------
@@ -1136,9 +1136,16 @@ typedef __attribute__((address_space(4))) const u32 cu32;  // read-only, wave-un
 //     above for limbs that straddle the boundary;
 //  5  R - N on those limbs; a borrow lookahead over the chunks decides R >= N
 //     (or bit 32 L32 of R set), and each chunk stores R or R - N.
-template <int RB, int LP, int TPI, int L32, int W>
+// `mid` runs right after the first barrier: every wave of the workgroup has
+// then finished whatever it did before the finish (the served kernel converts
+// its next tile into the position rows there).
+struct NoMid {
+  __device__ __forceinline__ void operator()() const {}
+};
+template <int RB, int LP, int TPI, int L32, int W, class Mid = NoMid>
 __device__ __forceinline__ void fixed_finish_w(const LazyDev* __restrict__ md, u64 (&acc)[LP / TPI], u32* lds,
-                                               int lane, u32* __restrict__ ob, size_t so, bool store) {
+                                               int lane, u32* __restrict__ ob, size_t so, bool store,
+                                               const Mid& mid = Mid()) {
   constexpr int K = LP / TPI, T = TPI;
   constexpr u32 MASK = (1u << RB) - 1u;
   typedef FixedLds<LP, TPI> S;
@@ -1161,6 +1168,7 @@ __device__ __forceinline__ void fixed_finish_w(const LazyDev* __restrict__ md, u
     mdg[FIX_TILE + lane] = m2;
   }
   __syncthreads();
+  mid();
   // 2: add (m1 + m2 2^RB) N on this chunk (the top chunk's m2 N_(LP-1) lies at
   // position LP, above every chunk: it joins the top carry in step 3)
   const u32 m2 = mdg[FIX_TILE + lane];
@@ -1294,88 +1302,154 @@ __device__ __forceinline__ void fixed_finish(int wave, const LazyDev* __restrict
   }
 }
 
-template <int RB, int LP, int TPI, int L32>
-__global__ __launch_bounds__(64 * TPI) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_fixed_mulmod(const LazyDev* __restrict__ md,
-                                                           const u32* __restrict__ x, int begin, int count,
-                                                           int stride, MultiQuery mq, int out_stride) {
-  extern __shared__ uint4 smem4[];
+// Index limbs of chunk W of one entry: the limbs [J0, J0 + NL) that positions
+// [W K, W K + K) span, plus one above (zero past the top limb).
+template <int RB, int LP, int TPI, int L32, int W>
+struct FixedChunk {
+  static constexpr int K = LP / TPI;
+  static constexpr int J0 = (RB * W * K) >> 5;
+  static constexpr int NL = ((RB * (W * K + K) - 1) >> 5) - J0 + 2;
+};
+
+// The units of one launch, run by wave W of every workgroup (W a template
+// argument: the limb indices, bit offsets and the prefetch registers are
+// constants). Two units are in hand at any time: the workgroup works on unit u
+// while the index limbs of the next unit un are on their way into registers
+// (loaded before the row loop, converted into the position rows by the
+// finish's hook, after its first barrier, when every wave has left the row
+// loop). The queue head is bumped mid-unit, so the grab of the unit after un
+// is hidden behind the rest of the finish. A unit costs six barriers.
+template <int RB, int LP, int TPI, int L32, int W>
+__device__ __forceinline__ void fixed_units(const LazyDev* __restrict__ md, const u32* __restrict__ x, int begin,
+                                            int count, int stride, const MultiQuery& mq, int out_stride,
+                                            u32* lds, int* slot) {
   constexpr int K = LP / TPI, CH = FixedShape<K>::CH;
-  static_assert(LP % TPI == 0, "LP must be a multiple of TPI");
-  u32* lds = reinterpret_cast<u32*>(smem4);
+  constexpr u32 MASK = (1u << RB) - 1u;
+  typedef FixedChunk<RB, LP, TPI, L32, W> C;
   u32* xs = lds;  // [LP][FIX_TILE]
-  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
   const int lane = lane_id();
   const int tiles = (count + FIX_TILE - 1) / FIX_TILE;
   // Work units in query-major order: the workgroups sweep the range for one
   // query before the next, so at any time they read one or two operand tables
-  // (97 KB each at 4096-bit moduli) instead of all of them. Each unit converts
-  // its tile's positions again (a small share of the unit's multiply-adds).
+  // (97 KB each at 4096-bit moduli) instead of all of them. Units come from a
+  // queue, so a fast workgroup takes more units than a slow one (strided
+  // units leave the launch waiting for its slowest workgroups, 14% slower
+  // measured) and the queries complete one after another at the pace of the
+  // grid.
   const int units = tiles * mq.nq;
-  // The units come from a queue: every workgroup takes the next unit in
-  // order, so a fast workgroup takes more units than a slow one (strided units
-  // leave the launch waiting for its slowest workgroups, 14% slower measured)
-  // and the queries complete one after another at the pace of the grid.
-  __shared__ int next_unit;
-  for (;;) {
-    if (threadIdx.x == 0) next_unit = (int)atomicAdd(mq.head, 1u);
-    __syncthreads();
-    const int u = next_unit;  // the previous unit ended with a barrier: nobody still reads the old value
-    if (u >= units) break;
+  u32 lim[C::NL];
+  // an opaque stride keeps the per-limb addresses from being hoisted out of
+  // the unit loop (they would be held, or spilled, across it)
+  auto fetch = [&](int unit) {
+    const int base = (unit % tiles) * FIX_TILE;
+    const int local = base + lane < count ? base + lane : count - 1;
+    const size_t so = (size_t)opaque(stride);
+    const u32* xb = x + (size_t)begin + local;
+#pragma unroll
+    for (int t = 0; t < C::NL; ++t) lim[t] = C::J0 + t < L32 ? xb[(size_t)(C::J0 + t) * so] : 0u;
+  };
+  auto convert = [&]() {
+#pragma unroll
+    for (int p = 0; p < K; ++p) {
+      const int q = W * K + p;
+      const int b = RB * q, t = (b >> 5) - C::J0, o = b & 31;
+      xs[q * FIX_TILE + lane] = __builtin_amdgcn_alignbit(lim[t + 1], lim[t], o) & MASK;
+    }
+  };
+  int u = slot[0], un = slot[1];
+  if (u < units) {
+    fetch(u);
+    convert();
+  }
+  if (un < units) fetch(un);
+  __syncthreads();  // position rows of unit u
+  while (u < units) {
     const int q = u / tiles, tile = u - q * tiles;
     const int base = tile * FIX_TILE;
-    // A: wave w converts positions [w K, w K + K) of all 64 entries (one
-    // entry per lane) straight from the index limbs into xs
-    {
-      const int local = base + lane < count ? base + lane : count - 1;
-      // an opaque stride keeps the per-limb addresses from being hoisted out
-      // of the tile loop (they would be held, or spilled, across it)
-      const size_t so = (size_t)opaque(stride);
-      const u32* xb = x + (size_t)begin + local;
-      fixed_positions<RB, LP, TPI, L32>(wave, xb, so, xs + lane);
-    }
-    __syncthreads();
-    {
-      // B: positions [wave K, wave K + K) of V for entry `lane`
-      // the table does not change during the launch: read it through the
-      // constant address space, so the wave-uniform loads are scalar
-      const cu32* Y = (const cu32*)(mq.op[q] + wave * CH);
-      u64 acc[K];
+    // positions [W K, W K + K) of V for entry `lane`; the table does not
+    // change during the launch: read through the constant address space, so
+    // the wave-uniform loads are scalar
+    const cu32* Y = (const cu32*)(mq.op[q] + W * CH);
+    u64 acc[K];
 #pragma unroll
-      for (int p = 0; p < K; ++p) acc[p] = 0;
-      // Rows in batches of FIX_ROWS: the batch's scalar loads (and its x
-      // reads) are issued together, so one wait covers them all; a scalar
-      // load can return out of order, so every use of one waits for all.
-      constexpr int FIX_ROWS = FixedRows<LP>::R;
+    for (int p = 0; p < K; ++p) acc[p] = 0;
+    // Rows in batches of FIX_ROWS: the batch's scalar loads (and its x
+    // reads) are issued together, so one wait covers them all; a scalar
+    // load can return out of order, so every use of one waits for all.
+    constexpr int FIX_ROWS = FixedRows<LP>::R;
 #pragma unroll 1
-      for (int i0 = 0; i0 < LP; i0 += FIX_ROWS) {
-        u32 yv[FIX_ROWS][K], xv[FIX_ROWS];
+    for (int i0 = 0; i0 < LP; i0 += FIX_ROWS) {
+      u32 yv[FIX_ROWS][K], xv[FIX_ROWS];
 #pragma unroll
-        for (int j = 0; j < FIX_ROWS; ++j) {
-          const cu32* yr = Y + (size_t)(i0 + j) * (TPI * CH);
+      for (int j = 0; j < FIX_ROWS; ++j) {
+        const cu32* yr = Y + (size_t)(i0 + j) * (TPI * CH);
 #pragma unroll
-          for (int p = 0; p < K; ++p) yv[j][p] = yr[p];
-          xv[j] = xs[(i0 + j) * FIX_TILE + lane];
-        }
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int j = 0; j < FIX_ROWS; ++j) {
-#pragma unroll
-          for (int p = 0; p < K; ++p) acc[p] = (u64)xv[j] * yv[j][p] + acc[p];
-        }
-        __builtin_amdgcn_sched_barrier(0);
+        for (int p = 0; p < K; ++p) yv[j][p] = yr[p];
+        xv[j] = xs[(i0 + j) * FIX_TILE + lane];
       }
-      // C..: reduction, exact positions, canonical 32-bit limbs, store
-      const size_t so = (size_t)opaque(out_stride);
-      fixed_finish<RB, LP, TPI, L32>(wave, md, acc, lds, lane, mq.out[q] + (size_t)begin + base + lane, so,
-                                     base + lane < count);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int j = 0; j < FIX_ROWS; ++j) {
+#pragma unroll
+        for (int p = 0; p < K; ++p) acc[p] = (u64)xv[j] * yv[j][p] + acc[p];
+      }
+      __builtin_amdgcn_sched_barrier(0);
     }
-    __syncthreads();  // every wave has issued the unit's stores (and read next_unit)
-    // a query is complete when its last unit is: stamp the device clock then
-    // (a timing record, not a hand-over of data, which is read after the
-    // launch: no fence, which would write the L2 back on every unit)
-    if (mq.done != nullptr && threadIdx.x == 0 && atomicAdd(mq.done + q, 1u) == (u32)tiles - 1u)
-      mq.stamp[q] = __builtin_amdgcn_s_memrealtime();
+    // reduction, exact positions, canonical 32-bit limbs, store; after the
+    // finish's first barrier the position rows are free: the next unit's
+    // positions go there, and the unit after it is taken from the queue
+    const bool more = un < units;
+    auto mid = [&]() {
+      if (more) convert();
+      if (threadIdx.x == 0) slot[0] = (int)atomicAdd(mq.head, 1u);
+    };
+    const size_t so = (size_t)opaque(out_stride);
+    fixed_finish_w<RB, LP, TPI, L32, W>(md, acc, lds, lane, mq.out[q] + (size_t)begin + base + lane, so,
+                                        base + lane < count, mid);
+    __syncthreads();  // every wave has issued the unit's stores; slot[0] and the next positions are set
+    // a query is complete when its last unit is: each unit counts itself and
+    // raises its query's stamp to the device clock at its end, so the stamp
+    // ends as the clock of the query's last unit. Both atomics return nothing:
+    // no unit waits for them. (Timing records, read after the launch, not a
+    // hand-over of data: no fence, which would write the L2 back on every unit.)
+    if (mq.done != nullptr && threadIdx.x == 0) {
+      atomicAdd(mq.done + q, 1u);
+      atomicMax(reinterpret_cast<unsigned long long*>(mq.stamp + q),
+                (unsigned long long)__builtin_amdgcn_s_memrealtime());
+    }
+    u = un;
+    un = slot[0];  // written before the barrier above, rewritten only after the next unit's first finish barrier
+    if (u < units && un < units) fetch(un);
   }
+}
+
+template <int RB, int LP, int TPI, int L32, int W = 0>
+__device__ __forceinline__ void fixed_units_for(int wave, const LazyDev* __restrict__ md, const u32* __restrict__ x,
+                                                int begin, int count, int stride, const MultiQuery& mq,
+                                                int out_stride, u32* lds, int* slot) {
+  if constexpr (W < TPI) {
+    if (wave == W)
+      fixed_units<RB, LP, TPI, L32, W>(md, x, begin, count, stride, mq, out_stride, lds, slot);
+    else
+      fixed_units_for<RB, LP, TPI, L32, W + 1>(wave, md, x, begin, count, stride, mq, out_stride, lds, slot);
+  }
+}
+
+template <int RB, int LP, int TPI, int L32>
+__global__ __launch_bounds__(64 * TPI) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_fixed_mulmod(const LazyDev* __restrict__ md,
+                                                           const u32* __restrict__ x, int begin, int count,
+                                                           int stride, MultiQuery mq, int out_stride) {
+  extern __shared__ uint4 smem4[];
+  static_assert(LP % TPI == 0, "LP must be a multiple of TPI");
+  __shared__ int slot[2];  // the unit in hand and the next one
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
+  if (threadIdx.x == 0) {
+    slot[0] = (int)atomicAdd(mq.head, 1u);
+    slot[1] = (int)atomicAdd(mq.head, 1u);
+  }
+  __syncthreads();
+  fixed_units_for<RB, LP, TPI, L32>(wave, md, x, begin, count, stride, mq, out_stride,
+                                    reinterpret_cast<u32*>(smem4), slot);
 }
 
 // ---------------------------------------------------------------------------

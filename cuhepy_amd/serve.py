@@ -9,16 +9,19 @@ from Python; here a request loop keeps the index resident in HBM and serves
 many queries at once. Two batching modes:
 
 * ``fifo`` (the default): admitted queries run in arrival order. Whenever the
-  device queue has room (``depth`` launches in flight), ONE launch takes every
-  admitted query not yet launched (up to ``max_active``) over the whole
-  index. Inside the launch the workgroups take the tiles query by query, so
-  the queries complete one after another (each overlapping only the start of
-  the next), with no gap between them as between launches. Each query
-  is complete when its last tile is: the kernel stamps the device clock then
-  (``Modulus.mulmod_multi_stamped_dev``), and the stamps are converted to
-  host time against the event recorded after the launch. A query arriving
-  while a launch runs joins the next one, which is queued behind it on the
-  stream.
+  device queue has room (``depth`` launches in flight), ONE launch takes the
+  admitted queries not yet launched (up to ``MQ_MAX``) over the whole index.
+  Inside the launch the workgroups take the tiles query by query, so the
+  queries complete one after another (each overlapping only the start of the
+  next), with no gap between them as between launches, and the launch pays
+  one tail instead of one per query. A launch of several queries stamps the
+  device clock when each query's last tile is done
+  (``Modulus.mulmod_multi_stamped_dev``); the stamps are converted to host
+  time against the event recorded after the launch. A launch of one query is
+  the plain launch (completion = its event). Queries arriving while the queue
+  is full are admitted and prepared (operand tables) up to ``max_active``
+  (which may exceed one launch's ``MQ_MAX``), so a prepared batch is ready
+  when a slot frees.
 * ``scan``: a cyclic scan over ``nchunks`` index ranges. Every iteration
   multiplies the next range by all active queries; a query arriving while
   others run joins at the next iteration, at whatever range the scan is on,
@@ -33,7 +36,9 @@ query's own completion time and removes the gaps between launches.
 Memory admission: each active query holds a result buffer (one ciphertext per
 index entry). A waiting query is admitted only when a released buffer can be
 reused, or free device memory minus a reserve covers a new one. At most
-``max_active`` (<= 32 operands per launch) are active.
+``max_active`` are active (``MQ_MAX`` operands per launch; fifo mode admits
+up to ``FIFO_MAX_ACTIVE``, scan mode ``MQ_MAX``, since every active query of a
+scan joins each launch).
 
 Latency is measured to device completion from the request's arrival time.
 The schedulers and the admission policy are plain Python, independent of the
@@ -49,6 +54,7 @@ from dataclasses import dataclass, field
 from typing import Callable, Optional
 
 MQ_MAX = 32  # operands per launch (MultiQuery in the kernel)
+FIFO_MAX_ACTIVE = 4 * MQ_MAX  # admitted (launched or waiting for a launch) in fifo mode
 
 
 @dataclass
@@ -60,6 +66,7 @@ class Request:
     start_chunk: int = -1
     left: int = 0
     admitted: Optional[float] = None
+    launched: Optional[float] = None  # host clock when its (last) launch was queued
     done: Optional[float] = None  # device completion, host clock
     result: object = None         # (limbs, stride) device tensor when kept
     chunks_seen: list = field(default_factory=list)
@@ -152,12 +159,12 @@ class ScanScheduler:
 
 class FifoBatcher:
     """FIFO continuous batching: admitted requests are launched in arrival
-    order; a launch takes every admitted request not yet launched (up to
-    `max_active` active in all, launched or not)."""
+    order; a launch takes the admitted requests not yet launched, at most
+    MQ_MAX; at most `max_active` are admitted (launched or not)."""
 
-    def __init__(self, max_active: int = MQ_MAX):
-        if not 1 <= max_active <= MQ_MAX:
-            raise ValueError(f"max_active must be in [1, {MQ_MAX}]")
+    def __init__(self, max_active: int = 2 * MQ_MAX):
+        if not 1 <= max_active <= FIFO_MAX_ACTIVE:
+            raise ValueError(f"max_active must be in [1, {FIFO_MAX_ACTIVE}]")
         self.max_active = max_active
         self.waiting: collections.deque[Request] = collections.deque()
         self.pending: list[Request] = []   # admitted, not launched
@@ -167,7 +174,7 @@ class FifoBatcher:
         self.waiting.append(req)
 
     def admit(self, admission: MemoryAdmission, now: float, on_admit: Callable[[Request, str], None],
-              limit: int = MQ_MAX) -> int:
+              limit: int = FIFO_MAX_ACTIVE) -> int:
         n = 0
         while self.waiting and len(self.pending) + self.running < self.max_active and n < limit:
             how = admission.try_reserve()
@@ -181,7 +188,7 @@ class FifoBatcher:
         return n
 
     def next_batch(self) -> list[Request]:
-        batch, self.pending = self.pending, []
+        batch, self.pending = self.pending[:MQ_MAX], self.pending[MQ_MAX:]
         self.running += len(batch)
         return batch
 
@@ -209,7 +216,7 @@ class KnnServer:
 
     def __init__(self, ctx, modulus, index, count: int, chunk: Optional[int] = None, max_active: int = MQ_MAX,
                  reserve_bytes: int = 4 << 30, free_bytes: Optional[Callable[[], int]] = None, depth: int = 2,
-                 clock: Callable[[], float] = time.perf_counter, mode: str = "fifo"):
+                 clock: Callable[[], float] = time.perf_counter, mode: str = "fifo", prep_per_pump: int = 4):
         import torch
 
         if mode not in ("fifo", "scan"):
@@ -232,6 +239,8 @@ class KnnServer:
         self.admission = MemoryAdmission(free_bytes, self.result_bytes, reserve_bytes)
         self.pool: list = []
         self.depth = depth
+        self.prep_per_pump = max(1, int(prep_per_pump))
+        self.prepared = 0
         self.clock = clock
         self.stream = torch.cuda.ExternalStream(ctx.stream_ptr, device=self.device) if ctx.stream_ptr else \
             torch.cuda.default_stream(self.device)
@@ -295,12 +304,13 @@ class KnnServer:
             ev.synchronize()
             self.inflight.popleft()
             t = self.t_base + self.base_event.elapsed_time(ev) / 1e3
-            if stamps is not None:  # fifo: each query completed at its own stamp, the last one at the event
+            if stamps is not None:  # fifo batch: each query completed at its own stamp, the last one at the event
                 self.stamp_sets.append(stamps)
                 st = stamps[1][:len(finished)].cpu().tolist()
                 last = max(st)
                 for r, s_q in zip(finished, st):
                     r.done = t - (last - s_q) / self.clock_hz
+            if self.mode == "fifo":
                 self.sched.complete(finished)
             for r in finished:
                 if stamps is None:
@@ -326,16 +336,24 @@ class KnnServer:
 
     def pump(self) -> bool:
         """Retire finished iterations, admit, and launch one iteration if the
-        queue depth allows; returns whether anything was launched."""
+        queue depth allows; returns whether anything was launched
+        (``self.prepared``: how many queries this call admitted without a
+        launch, which a caller keeps pumping for before it waits)."""
         self._retire(block=False)
+        self.prepared = 0
         if len(self.inflight) >= self.depth:
-            if self.mode == "fifo":  # prepare the queries that will share the next launch
-                self.sched.admit(self.admission, self.clock(), self._on_admit)
+            if self.mode == "fifo":
+                # prepare a few of the queries that will share the next launch,
+                # then return, so the caller sees a finished launch soon
+                self.prepared = self.sched.admit(self.admission, self.clock(), self._on_admit,
+                                                 limit=self.prep_per_pump)
             return False
-        if self.mode == "fifo" and not self.sched.pending:
-            # the device queue has room: admit and launch the next query now
-            # (queries admitted while the queue is full wait as one batch)
-            self.sched.admit(self.admission, self.clock(), self._on_admit, limit=1)
+        if self.mode == "fifo":
+            # the device queue has room: launch what is prepared now; with
+            # nothing prepared, admit the next query and launch it alone
+            # (queries admitted while the queue is full leave as one batch)
+            if not self.sched.pending:
+                self.sched.admit(self.admission, self.clock(), self._on_admit, limit=1)
         else:
             self.sched.admit(self.admission, self.clock(), self._on_admit)
         if not self.sched.active:
@@ -343,9 +361,14 @@ class KnnServer:
         if self.mode == "fifo":
             batch = finished = self.sched.next_batch()
             begin, cnt = 0, self.count
-            stamps = self.stamp_sets.pop()
-            self.mod.mulmod_multi_stamped_dev(self.index, self.stride, begin, cnt, [r._op for r in batch],
-                                              [r._buf for r in batch], self.stride, stamps[0], stamps[1])
+            if len(batch) > 1:
+                stamps = self.stamp_sets.pop()
+                self.mod.mulmod_multi_stamped_dev(self.index, self.stride, begin, cnt, [r._op for r in batch],
+                                                  [r._buf for r in batch], self.stride, stamps[0], stamps[1])
+            else:  # one query: the plain launch, complete at its event
+                stamps = None
+                self.mod.mulmod_multi_dev(self.index, self.stride, begin, cnt, [batch[0]._op], [batch[0]._buf],
+                                          self.stride)
         else:
             chunk, batch, finished = self.sched.next_iteration()
             begin = chunk * self.chunk
@@ -355,6 +378,9 @@ class KnnServer:
                                       [r._buf for r in batch], self.stride)
         ev = self.torch.cuda.Event(enable_timing=True)
         ev.record(self.stream)
+        now = self.clock()
+        for r in batch:
+            r.launched = now
         self.inflight.append((ev, finished, stamps))
         self.iterations += 1
         self.launched_pairs += len(batch) * cnt
@@ -362,7 +388,7 @@ class KnnServer:
 
     def drain(self) -> None:
         while not self.sched.idle() or self.inflight:
-            if not self.pump():
+            if not self.pump() and not self.prepared:
                 if self.inflight:
                     self._retire(block=True)
                 elif self.sched.waiting and not self.sched.active:
@@ -380,7 +406,7 @@ class KnnServer:
                 reqs.append(self.submit(arrivals[i][1], t0 + arrivals[i][0], keep(i)))
                 i += 1
             launched = self.pump()
-            if not launched:
+            if not launched and not self.prepared:
                 if self.inflight:
                     # the device is busy: wait for the oldest iteration unless an arrival is due first
                     nxt = t0 + arrivals[i][0] if i < len(arrivals) else math.inf

@@ -122,8 +122,9 @@ int chx_mulmod_multi_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* x, size
 /* chx_mulmod_multi_dev that also reports when each query is complete: done
  * (device, nq + 1 words, zeroed by the call on the stream) counts each
  * query's finished tiles and holds the head of a queue of (query, tile)
- * units, and stamps[q] (device, nq u64) receives the device clock
- * (chx_ctx_wall_clock_hz) when query q's last tile finished. The workgroups
+ * units, and stamps[q] (device, nq u64, zeroed by the call) receives the
+ * device clock (chx_ctx_wall_clock_hz) when query q's last tile finished
+ * (the largest of its tiles' finishing clocks). The workgroups
  * take the units from the queue in order, query by query, so the queries
  * complete one after another, each overlapping only the start of the next
  * (no reference counterpart: the reference serves one query per call). */

@@ -129,7 +129,7 @@ def test_mulmod_multi_stamped(ctx):
     outs_a = [torch.full((128, stride), -1, dtype=torch.int32, device=dev) for _ in ys]
     outs_b = [torch.full((128, stride), -1, dtype=torch.int32, device=dev) for _ in ys]
     done = torch.full((32,), 7, dtype=torch.int32, device=dev)
-    stamps = torch.zeros(32, dtype=torch.int64, device=dev)
+    stamps = torch.full((32,), 5, dtype=torch.int64, device=dev)
     torch.cuda.synchronize()  # the fills above ran on torch's stream, the library uses its own
     ctx.sync()
     mod.mulmod_multi_dev(il, stride, 0, count, ops, outs_a, stride)
@@ -141,7 +141,7 @@ def test_mulmod_multi_stamped(ctx):
     # counters complete; the queue head passed every unit; the words beyond untouched
     assert done[:3].tolist() == [tiles] * 3 and done[3].item() >= 3 * tiles and done[4:].tolist() == [7] * 28
     st = stamps[:3].tolist()
-    assert all(v > 0 for v in st) and st[0] < st[1] < st[2]
+    assert all(v > 5 for v in st) and st[0] < st[1] < st[2] and stamps[3:].tolist() == [5] * 29
     assert ctx.wall_clock_hz() > 1e6
     got = _read(ctx, outs_b[1], 64, 128, stride)
     assert got == [x * ys[1] % N for x in xs[:64]]

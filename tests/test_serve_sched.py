@@ -5,7 +5,7 @@ import random
 
 import pytest
 
-from cuhepy_amd.serve import (MQ_MAX, FifoBatcher, MemoryAdmission, Request, ScanScheduler, latency_stats,
+from cuhepy_amd.serve import (FIFO_MAX_ACTIVE, MQ_MAX, FifoBatcher, MemoryAdmission, Request, ScanScheduler, latency_stats,
                                poisson_arrivals)
 
 
@@ -171,4 +171,24 @@ def test_fifo_batcher_memory_and_bad_parameters():
     with pytest.raises(ValueError):
         FifoBatcher(0)
     with pytest.raises(ValueError):
-        FifoBatcher(MQ_MAX + 1)
+        FifoBatcher(FIFO_MAX_ACTIVE + 1)
+
+
+def test_fifo_batcher_caps_a_launch_and_prepares_the_next():
+    """More admitted requests than one launch holds: each launch takes the
+    oldest MQ_MAX, the rest stay admitted (prepared) for the next launch."""
+    sched = FifoBatcher(max_active=3 * MQ_MAX)
+    adm = MemoryAdmission(lambda: 1 << 40, request_bytes=1)
+    for i in range(3 * MQ_MAX + 5):
+        sched.submit(Request(i, 0, 0.0))
+    assert sched.admit(adm, 0.0, lambda r, how: None) == 3 * MQ_MAX and len(sched.waiting) == 5
+    b1 = sched.next_batch()
+    b2 = sched.next_batch()
+    assert [r.rid for r in b1] == list(range(MQ_MAX)) and [r.rid for r in b2] == list(range(MQ_MAX, 2 * MQ_MAX))
+    assert len(sched.pending) == MQ_MAX and sched.running == 2 * MQ_MAX
+    assert sched.admit(adm, 0.0, lambda r, how: None) == 0  # at the cap: the rest waits
+    sched.complete(b1)
+    assert sched.admit(adm, 0.0, lambda r, how: None) == 5
+    b3 = sched.next_batch()
+    assert [r.rid for r in b3] == list(range(2 * MQ_MAX, 3 * MQ_MAX))
+    assert [r.rid for r in sched.next_batch()] == list(range(3 * MQ_MAX, 3 * MQ_MAX + 5))
