@@ -285,9 +285,9 @@ class PaillierGPULookupClient:
 
     # ---- encryption ----
     def _noise_idx(self, count: int) -> np.ndarray:
-        if self._rng is not None:
-            return np.array([[self._rng.randrange(256) for _ in range(kg.NOISE_MULTIPLES)] for _ in range(count)],
-                            dtype=np.uint8)
+        if self._rng is not None:  # seeded: one randbytes call per batch (row-major count x 14 bytes)
+            return np.frombuffer(self._rng.randbytes(count * kg.NOISE_MULTIPLES), dtype=np.uint8).reshape(
+                count, kg.NOISE_MULTIPLES).copy()
         return np.frombuffer(os.urandom(count * kg.NOISE_MULTIPLES), dtype=np.uint8).reshape(
             count, kg.NOISE_MULTIPLES).copy()
 

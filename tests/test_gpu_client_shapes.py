@@ -86,7 +86,7 @@ def test_lookup_client_c3_shape_every_row_group(ctx):
     client._rng = random.Random(8)
     clone = random.Random(8)
     got = [int(c[0]) for c in client.encrypt(embs[:16])]
-    idx16 = [[clone.randrange(256) for _ in range(14)] for _ in range(16)]
+    idx16 = np.frombuffer(clone.randbytes(16 * 14), dtype=np.uint8).reshape(16, 14).tolist()
     assert got == [O.lookup_encrypt(m, keys["pk"], gt, nt, mc, ix) for m, ix in zip(ms[:16], idx16)]
     assert client.decrypt_cts(got) == ms[:16]
 
@@ -136,9 +136,11 @@ def test_lookup_client_multi_chunk(embed_len, key_len):
     clone = random.Random(78)
     cts = client.encrypt(embs)
     assert len(cts) == len(embs) and all(len(row) == chunk_num for row in cts)
+    # the client draws the batch's noise indices in one call, row-major (ciphertext, factor)
+    idx = iter(np.frombuffer(clone.randbytes(len(embs) * chunk_num * 14), dtype=np.uint8).reshape(-1, 14).tolist())
     for e, row in zip(embs, cts):
         ms = O.embed_to_plaintexts(e, embed_len, key_len)
-        want = [O.lookup_encrypt(m, keys["pk"], gt, nt, mc, [clone.randrange(256) for _ in range(14)]) for m in ms]
+        want = [O.lookup_encrypt(m, keys["pk"], gt, nt, mc, next(idx)) for m in ms]
         assert [int(c) for c in row] == want
     pk = json.loads(client.stringify_pk())
     encs = [PaillierGPULookupClient.encode_hamming_server(cts[i], cts[i + 1], pk) for i in range(len(embs) - 1)]

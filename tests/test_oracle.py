@@ -310,3 +310,40 @@ def test_planted_reversed_pairs_are_caught():
     # ... but the closed form (and so the oracle) disagrees
     assert ra != packing_closed_form(a, 1024)
     assert ra != O.embed_to_plaintexts(a, 512, 1024)
+
+
+# --------------------------------------------------------------------------
+# randomness and noise: what the scheme requires of them
+# --------------------------------------------------------------------------
+
+def test_noise_entries_encrypt_zero(golden):
+    """lookup.py:113-117: every noise entry is an encryption of 0 (an n-th power
+    residue g^(n r)), so it is killed by a and decrypts to 0; the mistake of
+    dropping the exponent n (g^r) is caught."""
+    g = golden["lookup_w32"]
+    keys = O.lookup_key_gen(g["key_len"], g["alpha_len"], g["seed"], with_tables=False)
+    n, n2, gg, a = keys["pk"]["n"], keys["pk"]["n_squared"], keys["pk"]["g"], keys["sk"]["a"]
+    rs = [H(x) for x in g["noise_r"]]
+    noise = O.precompute_noise_table(gg, n, rs)
+    assert len(noise) == len(rs)
+    for v in noise:
+        assert pow(v, a, n2) == 1 and O.lookup_decrypt(v, keys) == 0 and v != 1
+    # a message times noise decrypts to the message
+    for m in (0, 1, n - 1, 12345):
+        c = pow(gg, m, n2) * noise[m % len(noise)] % n2
+        assert O.lookup_decrypt(c, keys) == m
+    wrong = [pow(gg, r, n2) for r in rs[:8]]  # exponent n dropped
+    assert any(O.lookup_decrypt(v, keys) != 0 for v in wrong)
+
+
+def test_random_r_draws_units_in_range():
+    """scheme.py:47-60 and lookup.py:198-210: r is a unit of Z_n in the stated
+    range, drawn afresh (no fixed fallback); checked on a modulus with small
+    factors, where non-units are common and must be rejected."""
+    n = 3 * 5 * 7 * 11 * 13 * 17  # 255,255: about 62% of [0, n) are non-units
+    rng = random.Random(12)
+    plain = [O.random_r(n, rng) for _ in range(400)]
+    look = [O.lookup_random_r(n, rng) for _ in range(400)]
+    assert all(0 < r < n and math.gcd(r, n) == 1 for r in plain)
+    assert all(1 <= r <= n and math.gcd(r, n) == 1 for r in look)
+    assert len(set(plain)) > 350 and len(set(look)) > 350
