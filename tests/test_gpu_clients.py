@@ -218,3 +218,33 @@ def test_invalid_inputs(clients):
         client.decode_hamming_client([["1", "2"]])
     assert client.encrypt([]) == []
     assert client.decode_hamming_client([]) == []
+
+
+def test_lookup_tables_survive_the_keys_file_format(tmp_path):
+    """keys.save(include_tables=True) writes {"type", "pk", "sk", "config",
+    "tables": dump_tables()} as JSON and keys.load hands the tables back to
+    load_config (keys.py:57-120): a JSON round trip turns the g_table keys into
+    strings; the reloaded client encrypts exactly as the original with the same
+    noise indices and decrypts its ciphertexts."""
+    from cuhepy_amd.paillier._lookup_gpu_ext import PaillierGPULookupClient
+    client = PaillierGPULookupClient(embed_len=_EMBED_LEN, key_len=512, rng=random.Random(404))
+    blob = {"type": "PaillierLookupClient", "pk": client.stringify_pk(), "sk": client.stringify_sk(),
+            "config": json.loads(client.stringify_config()), "tables": client.dump_tables()}
+    path = tmp_path / "key.json"
+    path.write_text(json.dumps(blob))
+    back = json.loads(path.read_text())
+    assert all(isinstance(k, str) for k in back["tables"]["g_table"])
+    cfg = back["config"]
+    clone = PaillierGPULookupClient(embed_len=cfg["embed_len"], key_len=cfg["key_len"], alpha_len=cfg["alpha_len"],
+                                    skip_key_gen=True)
+    clone.load_stringified_keys(back["pk"], back["sk"])
+    clone.load_config(cfg, back["tables"])
+    n = int(json.loads(client.stringify_pk())["n"])
+    rng = random.Random(5)
+    ms = [rng.randrange(n) for _ in range(40)] + [0, 1, n - 1]
+    idx = np.array([[rng.randrange(256) for _ in range(14)] for _ in ms], dtype=np.uint8)
+    a = [int(c) for c in client.encrypt_cts(ms, noise_idx=idx)]
+    b = [int(c) for c in clone.encrypt_cts(ms, noise_idx=idx)]
+    assert a == b
+    assert clone.decrypt_cts(a) == ms
+    assert clone.dump_tables()["noise_table"] == client.dump_tables()["noise_table"]
