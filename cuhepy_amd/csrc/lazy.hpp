@@ -1068,6 +1068,26 @@ struct FixedRows {
   static constexpr int R = LP % 4 == 0 ? 4 : LP % 2 == 0 ? 2 : 1;
 };
 
+// Rows between accumulator folds of the served multiply. A row adds x_i Y_p
+// with x_i < 2^RB and a settled table position Y_p < D = 2^RB + 2^(64-2RB) + 1;
+// after a fold a position holds less than 2^RB + 2^(64-RB) (its low bits and
+// the carry of the one below). SEG rows plus the finish's two REDC products
+// (below 2^(2RB) each) stay below 2^64. SEG is a multiple of FIX_ROWS that
+// divides LP, or LP itself when all rows fit (27- and 28-bit shapes).
+template <int RB, int LP>
+struct FixedSeg {
+  static constexpr double TWO64 = 18446744073709551616.0;
+  static constexpr double ROW = (double)(1ull << RB) * ((double)(1ull << RB) + (double)(1ull << (64 - 2 * RB)) + 1.0);
+  static constexpr double BASE = (double)(1ull << RB) + (double)(1ull << (64 - RB)) + 2.0 * (double)(1ull << (2 * RB));
+  static constexpr int MAXR = (int)((TWO64 - BASE) / ROW) - 1;  // one row of margin for the rounding of doubles
+  static constexpr int pick(int s) {
+    return s <= 0 ? 0 : (LP % s == 0 && s % FixedRows<LP>::R == 0) ? s : pick(s - 1);
+  }
+  static constexpr int SEG = MAXR >= LP ? LP : pick(MAXR);
+  static_assert(SEG > 0 && LP % SEG == 0 && SEG % FixedRows<LP>::R == 0, "no fold interval fits the shape");
+  static_assert((double)SEG * ROW + BASE < TWO64, "fold interval overflows the accumulators");
+};
+
 // LDS of one k_fixed_mulmod workgroup (u32 words): the tile's positions
 // xs[LP][64], then the per-query exchange areas of the finish: REDC digits
 // md[2][64], chunk carries cw[TPI][64] (u64), carry lookahead flags
@@ -1136,16 +1156,18 @@ typedef __attribute__((address_space(4))) const u32 cu32;  // read-only, wave-un
 //     above for limbs that straddle the boundary;
 //  5  R - N on those limbs; a borrow lookahead over the chunks decides R >= N
 //     (or bit 32 L32 of R set), and each chunk stores R or R - N.
-// `mid` runs right after the first barrier: every wave of the workgroup has
-// then finished whatever it did before the finish (the served kernel converts
-// its next tile into the position rows there).
+// `hi` is what the chunk's accumulator folds moved above its top position
+// (in units of the chunk above's position 0; 0 when no fold ran): it joins the
+// chunk's carry in step 3. `mid` runs right after the first barrier: every
+// wave of the workgroup has then finished whatever it did before the finish
+// (the served kernel converts its next tile into the position rows there).
 struct NoMid {
   __device__ __forceinline__ void operator()() const {}
 };
 template <int RB, int LP, int TPI, int L32, int W, class Mid = NoMid>
 __device__ __forceinline__ void fixed_finish_w(const LazyDev* __restrict__ md, u64 (&acc)[LP / TPI], u32* lds,
                                                int lane, u32* __restrict__ ob, size_t so, bool store,
-                                               const Mid& mid = Mid()) {
+                                               u64 hi = 0, const Mid& mid = Mid()) {
   constexpr int K = LP / TPI, T = TPI;
   constexpr u32 MASK = (1u << RB) - 1u;
   typedef FixedLds<LP, TPI> S;
@@ -1192,8 +1214,8 @@ __device__ __forceinline__ void fixed_finish_w(const LazyDev* __restrict__ md, u
       X[p] = (u32)t & MASK;
       c = t >> RB;
     }
-    if constexpr (W < T - 1) cw[W * FIX_TILE + lane] = c;
-    else ctop = c + (u64)m2 * Nc[LP - 1];  // everything at positions >= LP (below 2^47: V < 2N 2^(2RB))
+    if constexpr (W < T - 1) cw[W * FIX_TILE + lane] = c + hi;  // below 2^37 (three shares of 2^(64-RB) at most)
+    else ctop = c + hi + (u64)m2 * Nc[LP - 1];  // everything at positions >= LP (below 2^47: V < 2N 2^(2RB))
   }
   __syncthreads();
   {
@@ -1376,24 +1398,40 @@ __device__ __forceinline__ void fixed_units(const LazyDev* __restrict__ md, cons
     // Rows in batches of FIX_ROWS: the batch's scalar loads (and its x
     // reads) are issued together, so one wait covers them all; a scalar
     // load can return out of order, so every use of one waits for all.
+    // Every SEG rows (29-bit shapes only) the accumulators are folded: each
+    // keeps its low RB bits plus the bits of the one below above RB, and the
+    // top one's high bits move to hi (the value is unchanged).
     constexpr int FIX_ROWS = FixedRows<LP>::R;
+    constexpr int SEG = FixedSeg<RB, LP>::SEG;
+    u64 hi = 0;
 #pragma unroll 1
-    for (int i0 = 0; i0 < LP; i0 += FIX_ROWS) {
-      u32 yv[FIX_ROWS][K], xv[FIX_ROWS];
+    for (int s0 = 0; s0 < LP; s0 += SEG) {
+#pragma unroll 1
+      for (int i0 = s0; i0 < s0 + SEG; i0 += FIX_ROWS) {
+        u32 yv[FIX_ROWS][K], xv[FIX_ROWS];
 #pragma unroll
-      for (int j = 0; j < FIX_ROWS; ++j) {
-        const cu32* yr = Y + (size_t)(i0 + j) * (TPI * CH);
+        for (int j = 0; j < FIX_ROWS; ++j) {
+          const cu32* yr = Y + (size_t)(i0 + j) * (TPI * CH);
 #pragma unroll
-        for (int p = 0; p < K; ++p) yv[j][p] = yr[p];
-        xv[j] = xs[(i0 + j) * FIX_TILE + lane];
+          for (int p = 0; p < K; ++p) yv[j][p] = yr[p];
+          xv[j] = xs[(i0 + j) * FIX_TILE + lane];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < FIX_ROWS; ++j) {
+#pragma unroll
+          for (int p = 0; p < K; ++p) acc[p] = (u64)xv[j] * yv[j][p] + acc[p];
+        }
+        __builtin_amdgcn_sched_barrier(0);
       }
-      __builtin_amdgcn_sched_barrier(0);
+      if constexpr (SEG < LP) {
+        if (s0 + SEG < LP) {
+          hi += acc[K - 1] >> RB;
 #pragma unroll
-      for (int j = 0; j < FIX_ROWS; ++j) {
-#pragma unroll
-        for (int p = 0; p < K; ++p) acc[p] = (u64)xv[j] * yv[j][p] + acc[p];
+          for (int p = K - 1; p >= 1; --p) acc[p] = (acc[p] & (u64)MASK) + (acc[p - 1] >> RB);
+          acc[0] &= (u64)MASK;
+        }
       }
-      __builtin_amdgcn_sched_barrier(0);
     }
     // reduction, exact positions, canonical 32-bit limbs, store; after the
     // finish's first barrier the position rows are free: the next unit's
@@ -1405,7 +1443,7 @@ __device__ __forceinline__ void fixed_units(const LazyDev* __restrict__ md, cons
     };
     const size_t so = (size_t)opaque(out_stride);
     fixed_finish_w<RB, LP, TPI, L32, W>(md, acc, lds, lane, mq.out[q] + (size_t)begin + base + lane, so,
-                                        base + lane < count, mid);
+                                        base + lane < count, hi, mid);
     __syncthreads();  // every wave has issued the unit's stores; slot[0] and the next positions are set
     // a query is complete when its last unit is: each unit counts itself and
     // raises its query's stamp to the device clock at its end, so the stamp

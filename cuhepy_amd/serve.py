@@ -264,11 +264,17 @@ class KnnServer:
         self.y_host = torch.empty((rows, self.limbs), dtype=torch.int32, pin_memory=True)
         self.y_np = self.y_host.numpy().view(np.uint32)
         self.y_free = list(range(rows))
-        if mode == "fifo":  # per-launch completion counters and stamps, reused once retired
+        if mode == "fifo":
+            # per-launch completion counters and stamps, reused once retired;
+            # each set has a pinned host copy of its stamps that the stream
+            # fills right after the launch, so a retire reads host memory and
+            # never waits for the device (a blocking read of device memory
+            # would stall the host behind the launches still in flight)
             self.clock_hz = ctx.wall_clock_hz()
             with torch.cuda.stream(self.stream):
                 self.stamp_sets = [(torch.zeros(MQ_MAX + 1, dtype=torch.int32, device=self.device),
-                                    torch.zeros(MQ_MAX, dtype=torch.int64, device=self.device))
+                                    torch.zeros(MQ_MAX, dtype=torch.int64, device=self.device),
+                                    torch.zeros(MQ_MAX, dtype=torch.int64, pin_memory=True))
                                    for _ in range(depth + 1)]
 
     # ---- requests ----
@@ -306,7 +312,7 @@ class KnnServer:
             t = self.t_base + self.base_event.elapsed_time(ev) / 1e3
             if stamps is not None:  # fifo batch: each query completed at its own stamp, the last one at the event
                 self.stamp_sets.append(stamps)
-                st = stamps[1][:len(finished)].cpu().tolist()
+                st = stamps[2][:len(finished)].tolist()
                 last = max(st)
                 for r, s_q in zip(finished, st):
                     r.done = t - (last - s_q) / self.clock_hz
@@ -365,6 +371,8 @@ class KnnServer:
                 stamps = self.stamp_sets.pop()
                 self.mod.mulmod_multi_stamped_dev(self.index, self.stride, begin, cnt, [r._op for r in batch],
                                                   [r._buf for r in batch], self.stride, stamps[0], stamps[1])
+                with self.torch.cuda.stream(self.stream):
+                    stamps[2].copy_(stamps[1], non_blocking=True)  # complete when the launch's event is
             else:  # one query: the plain launch, complete at its event
                 stamps = None
                 self.mod.mulmod_multi_dev(self.index, self.stride, begin, cnt, [batch[0]._op], [batch[0]._buf],
