@@ -780,11 +780,14 @@ def serve_leg(ctx, args, peak: float) -> dict:
     for _ in range(2):
         for name, server in (("batched", srv), ("plain", plain)):
             it0 = server.iterations
+            gc.collect()
+            gc.disable()
             t0 = time.perf_counter()
             for y in ys:
                 server.submit(y)
             server.drain()
             sat_runs[name].append(round(len(ys) / (time.perf_counter() - t0), 2))
+            gc.enable()
             if name == "batched":
                 launches = server.iterations - it0
     sat, sat_plain = max(sat_runs["batched"]), max(sat_runs["plain"])
@@ -803,22 +806,35 @@ def serve_leg(ctx, args, peak: float) -> dict:
         row = {"rate_qps": round(rate, 2)}
         pooled = {"batched": [], "plain": []}
         walls = {"batched": 0.0, "plain": 0.0}
+        replays = {"batched": [], "plain": []}
         order = (("batched", srv), ("plain", plain))
         for rep in range(2):
             for name, server in (order if (k + rep) % 2 == 0 else order[::-1]):
                 server.completed.clear()
-                t0 = time.perf_counter()
-                pooled[name] += server.run_trace(list(zip(offs, pool_ys)))
-                walls[name] += time.perf_counter() - t0
+                it0, lp0 = server.iterations, server.launched_pairs
+                gc.collect()  # no collector pause inside a replay (as the reference's time_call,
+                gc.disable()  # benchmarks/bfv_client_matrix.py:40-44): near saturation one pause
+                t0 = time.perf_counter()  # of a few ms becomes a backlog that takes seconds to drain
+                reqs = server.run_trace(list(zip(offs, pool_ys)))
+                wall = time.perf_counter() - t0
+                gc.enable()
+                pooled[name] += reqs
+                walls[name] += wall
+                rs = latency_stats(reqs)
+                nl = server.iterations - it0
+                replays[name].append({"p50_ms": round(rs["p50_ms"], 3), "p99_ms": round(rs["p99_ms"], 3),
+                                      "launches": nl,
+                                      "mean_batch": round((server.launched_pairs - lp0) / count / max(nl, 1), 2)})
         for name in ("batched", "plain"):
             st = latency_stats(pooled[name])
             row[name] = {"p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
                          "mean_ms": round(st["mean_ms"], 3), "queries": st["n"],
-                         "throughput_qps": round(len(pooled[name]) / walls[name], 2)}
+                         "throughput_qps": round(len(pooled[name]) / walls[name], 2),
+                         "replays": replays[name]}
         sweep.append(row)
         log(f"serve: {rate:7.1f} q/s: batched p50 {row['batched']['p50_ms']:.2f} p99 {row['batched']['p99_ms']:.2f} "
             f"ms | plain p50 {row['plain']['p50_ms']:.2f} p99 {row['plain']['p99_ms']:.2f} ms "
-            f"({row['plain']['queries']} queries each)")
+            f"({row['plain']['queries']} queries each); replays {replays}")
     return {
         "workload": f"{count} lookup-encrypted 512-bit vectors (key_len 1024, n^2 4096-bit) resident in HBM; "
                     "each query = one mulmod per index entry (encode_hamming_server)",
@@ -846,7 +862,8 @@ def serve_leg(ctx, args, peak: float) -> dict:
                    "(device clock stamp); a launch of one query is the plain launch",
         "plain": "one launch per query, whole index",
         "max_active": 64, "queries_per_launch_max": 32, "queries_per_rate": 2 * args.serve_queries,
-        "sweep_method": "one Poisson trace per rate, replayed twice per form in alternating order, latencies pooled",
+        "sweep_method": "one Poisson trace per rate, replayed twice per form in alternating order, latencies pooled; "
+                        "Python's collector off inside each replay",
         "latency_endpoint": "device-complete from arrival (plain: event after the query's launch; batched: the "
                             "stamp of the query's last tile, anchored to the event after its launch)",
         "poisson_sweep": sweep,

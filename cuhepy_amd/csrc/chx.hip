@@ -667,18 +667,62 @@ u32* queue_head(chx_ctx* ctx) {
   return h;
 }
 
-// The one-reduction pointwise multiply (k_pw_mulmod) runs at the lazy
-// configuration of each width with 19 positions per chunk.
+// The one-reduction pointwise multiply (k_pw_mulmod) runs at the positions of
+// the served multiply (serve_cfg: 29 bits at 2048- and 4096-bit moduli, the
+// modulus constants shared with it) with 18 or 19 positions per chunk.
 #define CHX_DISPATCH_PW(L32_, ...)                                                                 \
   do {                                                                                            \
     const int l__ = (L32_);                                                                       \
     if (l__ == 32) { constexpr int RB = 28, LP = 38, TPI = 2, L32 = 32; __VA_ARGS__; }             \
-    else if (l__ == 64) { constexpr int RB = 28, LP = 76, TPI = 4, L32 = 64; __VA_ARGS__; }        \
-    else if (l__ == 128) { constexpr int RB = 27, LP = 152, TPI = 8, L32 = 128; __VA_ARGS__; }     \
+    else if (l__ == 64) { constexpr int RB = 29, LP = 72, TPI = 4, L32 = 64; __VA_ARGS__; }        \
+    else if (l__ == 128) { constexpr int RB = 29, LP = 144, TPI = 8, L32 = 128; __VA_ARGS__; }     \
     else fail(CHX_EUNSUPPORTED, "no pointwise kernel for %d limbs", l__);                         \
   } while (0)
 
-int pw_cfg(int L32) { return L32 == 32 ? 1 : 0; }  // index in lazy_cfgs of the shape above
+static LazyCfg pw_shape(int L32) {
+  return L32 == 32 ? LazyCfg{28, 38, 2} : L32 == 64 ? LazyCfg{29, 72, 4} : LazyCfg{29, 144, 8};
+}
+// Shape of the served multiply (k_fixed_operand, k_fixed_mulmod; query
+// operands are prepared in its positions). At 2048- and 4096-bit moduli it
+// uses 29-bit positions: 72 and 144 of them instead of 76 and 152 at 28 and
+// 27 bits, 10% fewer multiply-adds per entry; the row loop folds its
+// accumulators every FixedSeg rows (one and two folds per entry).
+static LazyCfg serve_cfg(int L32) {
+  return L32 == 32 ? LazyCfg{28, 38, 1} : L32 == 64 ? LazyCfg{29, 72, 4} : LazyCfg{29, 144, 8};
+}
+
+#define CHX_DISPATCH_SERVE(L32_, ...)                                                            \
+  do {                                                                                         \
+    const int l__ = (L32_);                                                                    \
+    if (l__ == 32) { constexpr int RB = 28, LP = 38, TPI = 1, L32 = 32; __VA_ARGS__; }           \
+    else if (l__ == 64) { constexpr int RB = 29, LP = 72, TPI = 4, L32 = 64; __VA_ARGS__; }      \
+    else if (l__ == 128) { constexpr int RB = 29, LP = 144, TPI = 8, L32 = 128; __VA_ARGS__; }   \
+    else fail(CHX_EUNSUPPORTED, "no served multiply for %d limbs", l__);                         \
+  } while (0)
+
+// The modulus in the served shape's positions and B_i = 2^(RB (i + 2)) R' mod N
+// as positions (LP rows), uploaded once per modulus, on first use.
+static void serve_constants(chx_ctx* ctx, chx_mod* m) {
+  std::lock_guard<std::mutex> lock(m->mu);
+  if (m->d_fixb) return;
+  const LazyCfg c = serve_cfg(m->hc.L);
+  std::vector<uint32_t> h((size_t)c.lp * c.lp);
+  BN one(m->hc.L, 0u);
+  one[0] = 1;
+  for (int i = 0; i < c.lp; ++i) {
+    BN b = host::bn_shl_mod(one, c.rb * (i + 2 + c.lp), m->hc.N);
+    to_positions(b, c.rb, c.lp, h.data() + (size_t)i * c.lp);
+  }
+  LazyDev* d = make_lazy_dev(m->hc, c.rb, c.lp, ctx->stream);
+  try {
+    m->d_fixb = dev_upload(h.data(), h.size(), ctx->stream);
+  } catch (...) {
+    (void)hipFree(d);
+    throw;
+  }
+  m->d_serve = d;
+}
+
 
 // Fold table of k_pw_mulmod: rows r < nz hold Z = 2^(RB (h + r + 2)) mod N as
 // canonical positions; the two rows 2g, 2g + 1 of chunk w (positions
@@ -697,7 +741,7 @@ bool pw_constants(chx_ctx* ctx, chx_mod* m) {
     m->pw_h = -2;
     return false;
   }
-  const LazyCfg c = lazy_cfgs(L)[pw_cfg(L)];
+  const LazyCfg c = pw_shape(L);
   const int k = c.lp / c.tpi, ch = (k + 3) / 4 * 4;
   const int h0 = (host::bn_bits(m->hc.N) - 2) / c.rb;
   const int nz = (2 * c.lp - h0 + 3) / 4 * 4;
@@ -732,6 +776,7 @@ void launch_mulmod_mod(chx_ctx* ctx, const chx_mod* mc, const uint32_t* a, const
   }
   // window 2 of CHX_OP_MULMOD selects the two-product form below
   if (ctx->tune_window[CHX_OP_MULMOD][li] != 2 && pw_constants(ctx, m)) {
+    serve_constants(ctx, m);
     const size_t tiles = (count + FIX_TILE - 1) / FIX_TILE;
     u32* head = queue_head(ctx);
     record_start(ctx, "pw_mulmod");
@@ -744,7 +789,7 @@ void launch_mulmod_mod(chx_ctx* ctx, const chx_mod* mc, const uint32_t* a, const
                   "hipFuncSetAttribute");
       });
       hipLaunchKernelGGL(kern, dim3(fixed_blocks(ctx, kern, 64 * TPI, lds, tiles)), dim3(64 * TPI), lds,
-                         ctx->stream, m->lz.d[pw_cfg(L)], m->d_pwz, m->pw_h, m->pw_nz, a, b, out, (int)count,
+                         ctx->stream, m->d_serve, m->d_pwz, m->pw_h, m->pw_nz, a, b, out, (int)count,
                          (int)stride, head);
     });
     record_stop(ctx);
@@ -1282,24 +1327,6 @@ int chx_mulmod_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* a, const uint
   });
 }
 
-// Shape of the served multiply (k_fixed_operand, k_fixed_mulmod; query
-// operands are prepared in its positions). At 2048- and 4096-bit moduli it
-// uses 29-bit positions: 72 and 144 of them instead of 76 and 152 at 28 and
-// 27 bits, 10% fewer multiply-adds per entry; the row loop folds its
-// accumulators every FixedSeg rows (one and two folds per entry).
-static LazyCfg serve_cfg(int L32) {
-  return L32 == 32 ? LazyCfg{28, 38, 1} : L32 == 64 ? LazyCfg{29, 72, 4} : LazyCfg{29, 144, 8};
-}
-
-#define CHX_DISPATCH_SERVE(L32_, ...)                                                            \
-  do {                                                                                         \
-    const int l__ = (L32_);                                                                    \
-    if (l__ == 32) { constexpr int RB = 28, LP = 38, TPI = 1, L32 = 32; __VA_ARGS__; }           \
-    else if (l__ == 64) { constexpr int RB = 29, LP = 72, TPI = 4, L32 = 64; __VA_ARGS__; }      \
-    else if (l__ == 128) { constexpr int RB = 29, LP = 144, TPI = 8, L32 = 128; __VA_ARGS__; }   \
-    else fail(CHX_EUNSUPPORTED, "no served multiply for %d limbs", l__);                         \
-  } while (0)
-
 // Query operand of the served multiply: the table Y_i = y 2^(RB (i + 2)) mod N,
 // LP rows of TPI chunks of CH words (k_fixed_operand, k_fixed_mulmod).
 int chx_mulmod_operand_words(const chx_mod* m) {
@@ -1309,29 +1336,6 @@ int chx_mulmod_operand_words(const chx_mod* m) {
   const LazyCfg c = serve_cfg(L);
   const int k = c.lp / c.tpi;
   return c.lp * c.tpi * ((k + 3) / 4 * 4);
-}
-
-// The modulus in the served shape's positions and B_i = 2^(RB (i + 2)) R' mod N
-// as positions (LP rows), uploaded once per modulus, on first use.
-static void serve_constants(chx_ctx* ctx, chx_mod* m) {
-  std::lock_guard<std::mutex> lock(m->mu);
-  if (m->d_fixb) return;
-  const LazyCfg c = serve_cfg(m->hc.L);
-  std::vector<uint32_t> h((size_t)c.lp * c.lp);
-  BN one(m->hc.L, 0u);
-  one[0] = 1;
-  for (int i = 0; i < c.lp; ++i) {
-    BN b = host::bn_shl_mod(one, c.rb * (i + 2 + c.lp), m->hc.N);
-    to_positions(b, c.rb, c.lp, h.data() + (size_t)i * c.lp);
-  }
-  LazyDev* d = make_lazy_dev(m->hc, c.rb, c.lp, ctx->stream);
-  try {
-    m->d_fixb = dev_upload(h.data(), h.size(), ctx->stream);
-  } catch (...) {
-    (void)hipFree(d);
-    throw;
-  }
-  m->d_serve = d;
 }
 
 static void launch_operands(chx_ctx* ctx, chx_mod* m, const uint32_t* y, uint32_t* ops, size_t count) {

@@ -1061,6 +1061,18 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_fixed_operand(const LazyDev* 
 }
 
 constexpr int FIX_TILE = 64;  // index entries per workgroup tile
+
+// One fold of a chunk's K accumulators (value unchanged): each keeps its low
+// RB bits plus the bits of the one below above RB; the top one's high bits go
+// to hi, in units of the position above the chunk.
+template <int RB, int K>
+__device__ __forceinline__ void chunk_fold(u64 (&acc)[K], u64& hi) {
+  constexpr u64 MASK = (1ull << RB) - 1ull;
+  hi += acc[K - 1] >> RB;
+#pragma unroll
+  for (int p = K - 1; p >= 1; --p) acc[p] = (acc[p] & MASK) + (acc[p - 1] >> RB);
+  acc[0] &= MASK;
+}
 // table rows per scalar-load batch (FIX_ROWS x K words of SGPRs): 4, or the
 // largest of 2, 1 that divides LP
 template <int LP>
@@ -1315,12 +1327,13 @@ __device__ __forceinline__ void fixed_finish_w(const LazyDev* __restrict__ md, u
 
 template <int RB, int LP, int TPI, int L32, int W = 0>
 __device__ __forceinline__ void fixed_finish(int wave, const LazyDev* __restrict__ md, u64 (&acc)[LP / TPI],
-                                             u32* lds, int lane, u32* __restrict__ ob, size_t so, bool store) {
+                                             u32* lds, int lane, u32* __restrict__ ob, size_t so, bool store,
+                                             u64 hi = 0) {
   if constexpr (W < TPI) {
     if (wave == W)
-      fixed_finish_w<RB, LP, TPI, L32, W>(md, acc, lds, lane, ob, so, store);
+      fixed_finish_w<RB, LP, TPI, L32, W>(md, acc, lds, lane, ob, so, store, hi);
     else
-      fixed_finish<RB, LP, TPI, L32, W + 1>(wave, md, acc, lds, lane, ob, so, store);
+      fixed_finish<RB, LP, TPI, L32, W + 1>(wave, md, acc, lds, lane, ob, so, store, hi);
   }
 }
 
@@ -1425,12 +1438,7 @@ __device__ __forceinline__ void fixed_units(const LazyDev* __restrict__ md, cons
         __builtin_amdgcn_sched_barrier(0);
       }
       if constexpr (SEG < LP) {
-        if (s0 + SEG < LP) {
-          hi += acc[K - 1] >> RB;
-#pragma unroll
-          for (int p = K - 1; p >= 1; --p) acc[p] = (acc[p] & (u64)MASK) + (acc[p - 1] >> RB);
-          acc[0] &= (u64)MASK;
-        }
+        if (s0 + SEG < LP) chunk_fold<RB, K>(acc, hi);
       }
     }
     // reduction, exact positions, canonical 32-bit limbs, store; after the
@@ -1560,11 +1568,37 @@ __device__ __forceinline__ void pw_block(u64 (&acc)[K], u32 (&ring)[K], const u3
   }
 }
 
+// Fold intervals of k_pw_mulmod (29-bit shapes; the 27- and 28-bit ones never
+// fold). PB: product blocks of K rows between folds (a row adds a_i b_j <
+// 2^(2RB) to a position; a folded position holds less than 2^RB +
+// 2^(64-RB)). FS: fold-table rows between folds, a multiple of 4: a row adds a
+// T position (below 2^RB + 3 2^(64-2RB) + 1 after the chunk carries, which
+// include what the product folds moved above each chunk) times a canonical Z
+// position, and the finish's two REDC products follow the last segment.
+template <int RB, int LP, int TPI>
+struct PwSeg {
+  static constexpr int K = LP / TPI;
+  static constexpr double TWO64 = 18446744073709551616.0;
+  static constexpr double P2 = (double)(1ull << (2 * RB));
+  static constexpr int PROWS = (int)((TWO64 - (double)(1ull << RB) - (double)(1ull << (64 - RB))) / P2) - 1;
+  static constexpr int PB = PROWS >= TPI * K ? TPI : (PROWS / K > 0 ? PROWS / K : 1);
+  static constexpr double ROW = (double)(1ull << RB) * ((double)(1ull << RB) + 3.0 * (double)(1ull << (64 - 2 * RB)) + 1.0);
+  static constexpr double BASE = (double)(1ull << RB) + (double)(1ull << (64 - RB)) + 2.0 * P2;
+  static constexpr int FS = ((int)((TWO64 - BASE) / ROW) - 1) / 4 * 4;
+  static_assert(PB >= 1 && (double)(PB * K) * P2 + (double)(1ull << RB) + (double)(1ull << (64 - RB)) < TWO64,
+                "product fold interval overflows");
+  static_assert(FS >= 4 && (double)FS * ROW + BASE < TWO64, "fold-table fold interval overflows");
+};
+
 // Positions [c K, c K + K) of the product (chunk c; LOW: c < TPI) of the
-// operands in LDS (as_l, bs_l: column of this lane, rows of FIX_TILE words).
-template <int LP, int TPI, bool LOW>
-__device__ __forceinline__ void pw_chunk(u64 (&acc)[LP / TPI], const u32* as_l, const u32* bs_l, int c) {
+// operands in LDS (as_l, bs_l: column of this lane, rows of FIX_TILE words);
+// hi receives what the folds move above the chunk (0 without folds).
+template <int RB, int LP, int TPI, bool LOW>
+__device__ __forceinline__ void pw_chunk(u64 (&acc)[LP / TPI], u64& hi, const u32* as_l, const u32* bs_l, int c) {
   constexpr int K = LP / TPI;
+  constexpr int PB = PwSeg<RB, LP, TPI>::PB;
+  const int nblocks = LOW ? c + 1 : 2 * TPI - c;  // wave-uniform
+  hi = 0;
 #pragma unroll
   for (int p = 0; p < K; ++p) acc[p] = 0;
   u32 ring[K];
@@ -1579,6 +1613,9 @@ __device__ __forceinline__ void pw_chunk(u64 (&acc)[LP / TPI], const u32* as_l, 
 #pragma unroll 1
     for (int blk = 0; blk < c; ++blk) {
       pw_block<K, 0>(acc, ring, ap, bq);
+      if constexpr (PB < TPI) {
+        if ((blk + 1) % PB == 0) chunk_fold<RB, K>(acc, hi);  // another block follows
+      }
       ap += K * FIX_TILE;
       bq -= K * FIX_TILE;
     }
@@ -1589,11 +1626,15 @@ __device__ __forceinline__ void pw_chunk(u64 (&acc)[LP / TPI], const u32* as_l, 
     pw_block<K, 2>(acc, ring, ap, bq);
 #pragma unroll 1
     for (int blk = 1; blk < 2 * TPI - c; ++blk) {
+      if constexpr (PB < TPI) {
+        if (blk % PB == 0) chunk_fold<RB, K>(acc, hi);  // blk blocks done, one more follows
+      }
       ap += K * FIX_TILE;
       bq -= K * FIX_TILE;
       pw_block<K, 0>(acc, ring, ap, bq);
     }
   }
+  (void)nblocks;
 }
 
 template <int RB, int K>
@@ -1654,17 +1695,17 @@ __global__ __launch_bounds__(64 * TPI) __attribute__((amdgpu_waves_per_eu(4, 8))
     u32 xl[K], xh[K];
     u64 cl, chi;
     {
-      u64 acc[K];
-      pw_chunk<LP, TPI, true>(acc, as + lane, bs + lane, wave);
-      cl = pw_ripple<RB, K>(xl, acc);
+      u64 acc[K], hi;
+      pw_chunk<RB, LP, TPI, true>(acc, hi, as + lane, bs + lane, wave);
+      cl = pw_ripple<RB, K>(xl, acc) + hi;  // below 3 2^(64-RB)
       // the low chunk leaves its 64-bit accumulators here: K words stay live
       // across the high chunk instead of 2K
 #pragma unroll
       for (int p = 0; p < K; ++p) asm volatile("" : "+v"(xl[p]));
       asm volatile("" : "+v"(cl));
       __builtin_amdgcn_sched_barrier(0);
-      pw_chunk<LP, TPI, false>(acc, as + lane, bs + lane, TPI + wave);
-      chi = pw_ripple<RB, K>(xh, acc);  // the top chunk's carry is 0: a b < 2^(RB 2 LP)
+      pw_chunk<RB, LP, TPI, false>(acc, hi, as + lane, bs + lane, TPI + wave);
+      chi = pw_ripple<RB, K>(xh, acc) + hi;  // the top chunk's carry is 0: a b < 2^(RB 2 LP)
     }
     __syncthreads();  // as, bs read by every wave
     // 3: chunk carries
@@ -1683,7 +1724,7 @@ __global__ __launch_bounds__(64 * TPI) __attribute__((amdgpu_waves_per_eu(4, 8))
     bd[(2 * wave + 1) * FIX_TILE + lane] = xl[K - 1];
     __syncthreads();
     // 4: V positions [wave K, wave K + K)
-    u64 acc[K];
+    u64 acc[K], vhi = 0;
 #pragma unroll
     for (int p = 0; p < K; ++p) {
       const int t = wave * K + p - 2;  // T_t shifted up by two positions
@@ -1699,6 +1740,7 @@ __global__ __launch_bounds__(64 * TPI) __attribute__((amdgpu_waves_per_eu(4, 8))
       // scalar loads and its ts reads are issued together, so one wait covers
       // them all (a scalar load can return out of order).
       const cu32* Zw = (const cu32*)(Z + wave * 2 * CH);
+      constexpr int FS = PwSeg<RB, LP, TPI>::FS;
 #pragma unroll 1
       for (int r0 = 0; r0 < nz; r0 += 4) {  // nz is a multiple of 4
         u32 zv[4][K], xv[4];
@@ -1716,12 +1758,15 @@ __global__ __launch_bounds__(64 * TPI) __attribute__((amdgpu_waves_per_eu(4, 8))
           for (int p = 0; p < K; ++p) acc[p] = (u64)xv[j] * zv[j][p] + acc[p];
         }
         __builtin_amdgcn_sched_barrier(0);
+        if constexpr (FS < 2 * LP) {  // 29-bit shapes: fold every FS rows (wave-uniform)
+          if ((r0 + 4) % FS == 0 && r0 + 4 < nz) chunk_fold<RB, K>(acc, vhi);
+        }
       }
     }
     // 5: reduction, exact positions, canonical limbs, store (the finish's
     // areas lie above every ts row: no barrier before it)
     fixed_finish<RB, LP, TPI, L32>(wave, md, acc, lds + S::FIN, lane, out + base + lane, (size_t)opaque(stride),
-                                   base + lane < count);
+                                   base + lane < count, vhi);
     __syncthreads();  // the next tile's positions overwrite the finish's areas
   }
 }

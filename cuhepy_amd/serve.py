@@ -252,9 +252,7 @@ class KnnServer:
         ctx.sync()
         torch.cuda.synchronize(self.device)
         self.base_event = torch.cuda.Event(enable_timing=True)
-        self.base_event.record(self.stream)
-        self.base_event.synchronize()
-        self.t_base = clock()
+        self._anchor()
         # query ciphertexts are staged in pinned host rows, so the copy that
         # feeds a query's operand preparation is asynchronous (a pageable
         # source would make the host wait for the launches in flight); a row
@@ -276,6 +274,16 @@ class KnnServer:
                                     torch.zeros(MQ_MAX, dtype=torch.int64, device=self.device),
                                     torch.zeros(MQ_MAX, dtype=torch.int64, pin_memory=True))
                                    for _ in range(depth + 1)]
+
+    def _anchor(self) -> None:
+        """Pair a device timestamp with the host clock. Completion times are
+        device event times mapped to the host clock through this pair; the two
+        clocks drift apart by tens of parts per million, so the pair is renewed
+        whenever the server is idle before a trace (an error of a millisecond
+        after a minute would otherwise enter every latency)."""
+        self.base_event.record(self.stream)
+        self.base_event.synchronize()
+        self.t_base = self.clock()
 
     # ---- requests ----
     def submit(self, y: int, arrival: Optional[float] = None, keep: bool = False) -> Request:
@@ -405,6 +413,8 @@ class KnnServer:
     def run_trace(self, arrivals: list[tuple[float, int]], keep: Callable[[int], bool] = lambda i: False) -> list:
         """Replay (offset seconds, query ciphertext) arrivals against the host
         clock; returns the completed requests in arrival order."""
+        if self.sched.idle() and not self.inflight:
+            self._anchor()
         t0 = self.clock()
         reqs = []
         i = 0

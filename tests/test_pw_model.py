@@ -15,8 +15,41 @@ import pytest
 
 from test_fixed_model import finish_model
 
-# (RB, LP, waves, L32): the configuration of each width with 19 positions per chunk
-SHAPES = [(27, 152, 8, 128), (28, 76, 4, 64), (28, 38, 2, 32)]
+# (RB, LP, waves, L32): the shape of each width (pw_shape in chx.hip)
+SHAPES = [(29, 144, 8, 128), (29, 72, 4, 64), (28, 38, 2, 32)]
+
+
+def settled_bound(RB):
+    """Positions of T after the chunk carries: a carry (ripple carry plus what
+    the folds moved above the chunk, below 3 2^(64-RB)) enters at position 0
+    and its top part at position 1."""
+    return (1 << RB) + 3 * (1 << (64 - 2 * RB)) + 1
+
+
+def product_fold_blocks(RB, LP, T):
+    """PwSeg<RB, LP, TPI>::PB: product blocks of K rows between folds (a row adds
+    a_i b_j < 2^(2RB); a folded position holds less than 2^RB + 2^(64-RB))."""
+    K = LP // T
+    rows = ((1 << 64) - (1 << RB) - (1 << (64 - RB))) // (1 << (2 * RB)) - 1
+    return max(1, rows // K) if rows < T * K else T
+
+
+def fold_phase_rows(RB):
+    """PwSeg<...>::FS: fold-table rows between folds (a row adds a settled T
+    position times a canonical Z position; the finish's two REDC products
+    follow the last segment), a multiple of 4."""
+    row = settled_bound(RB) * (1 << RB)
+    base = (1 << RB) + (1 << (64 - RB)) + 2 * (1 << (2 * RB))
+    return (((1 << 64) - base) // row - 1) // 4 * 4
+
+
+def fold(acc, hi, RB):
+    M = (1 << RB) - 1
+    hi += acc[-1] >> RB
+    for p in range(len(acc) - 1, 0, -1):
+        acc[p] = (acc[p] & M) + (acc[p - 1] >> RB)
+    acc[0] &= M
+    return hi
 
 
 def fold_rows(N, RB, LP, T):
@@ -70,35 +103,53 @@ def pw_model(N, a, b, RB, LP, T, L32, h_shift=0, drop_carry=False):
                     acc[p] += a_s * v
                     assert acc[p] < LIM
 
+        nblocks = c + 1 if low else 2 * T - c
+        pb = product_fold_blocks(RB, LP, T)
+        hi = 0
+        done = 0
+
+        def after_block():
+            nonlocal done, hi
+            done += 1
+            if done % pb == 0 and done < nblocks:
+                hi = fold(acc, hi, RB)
+
         if low:
             for _ in range(c):
                 block(0)
+                after_block()
                 ai += K
                 bq -= K
             block(1)
+            after_block()
         else:
             block(2)
+            after_block()
             for _ in range(1, 2 * T - c):
                 ai += K
                 bq -= K
                 block(0)
-        for p in range(K):  # the chunk holds exactly the schoolbook sums
-            t = c * K + p
-            assert acc[p] == sum(A[i] * B[t - i] for i in range(max(0, t - LP + 1), min(t, LP - 1) + 1))
-        return acc
+                after_block()
+        # the chunk holds exactly the schoolbook sums (the folds keep the value)
+        want = sum(sum(A[i] * B[t - i] for i in range(max(0, t - LP + 1), min(t, LP - 1) + 1)) << (RB * (t - c * K))
+                   for t in range(c * K, c * K + K))
+        assert sum(v << (RB * p) for p, v in enumerate(acc)) + (hi << (RB * K)) == want
+        return acc, hi
 
-    def ripple(acc):
+    def ripple(acc_hi):
+        acc, hi = acc_hi
         x, c = [], 0
         for v in acc:
             t = v + c
             x.append(t & M)
             c = t >> RB
-        return x, c
+        return x, c + hi
 
     X, C = {}, {}
     for w in range(T):
         X[w], C[w] = ripple(chunk(w, True))
         X[T + w], C[T + w] = ripple(chunk(T + w, False))
+        assert C[w] < 3 << (64 - RB) and C[T + w] < 3 << (64 - RB)
     assert C[2 * T - 1] == 0  # a b < 2^(RB 2 LP)
     for c in range(1, 2 * T):  # carry-in at the bottom position (settled positions)
         if drop_carry and c == T:
@@ -108,26 +159,33 @@ def pw_model(N, a, b, RB, LP, T, L32, h_shift=0, drop_carry=False):
         X[c][1] += t >> RB
     Tpos = [X[c][p] for c in range(2 * T) for p in range(K)]
     assert sum(v << (RB * t) for t, v in enumerate(Tpos)) == a * b or drop_carry
-    assert all(v < (1 << RB) + (1 << (64 - 2 * RB)) + 1 for v in Tpos)
+    assert all(v < settled_bound(RB) for v in Tpos)
     # LDS rows: ts [0, nz), finish areas [2LP - 12T - 2, 2LP - 6T), bd [2LP - 6T, 2LP - 4T), cx [2LP - 4T, 2LP)
     assert nz <= 2 * LP - 12 * T - 2
     Zp = [pos(pow(2, RB * (h + r + 2), N), LP) for r in range(nz)]
-    acc = []
+    fs = fold_phase_rows(RB)
+    acc, his = [], []
     for w in range(T):
         ch = []
         for p in range(K):
             t = w * K + p - 2
-            v = Tpos[t] if 0 <= t < h else 0
-            for r in range(nz):
-                v += Tpos[h + r] * Zp[r][w * K + p]
-            assert v < LIM
-            ch.append(v)
+            ch.append(Tpos[t] if 0 <= t < h else 0)
+        hi = 0
+        for r0 in range(0, nz, 4):  # rows in batches of 4, folded every fs rows
+            for r in range(r0, r0 + 4):
+                for p in range(K):
+                    ch[p] += Tpos[h + r] * Zp[r][w * K + p]
+                    assert ch[p] < LIM
+            if (r0 + 4) % fs == 0 and r0 + 4 < nz:
+                hi = fold(ch, hi, RB)
         acc.append(ch)
-    V = sum(v << (RB * (w * K + p)) for w in range(T) for p, v in enumerate(acc[w]))
+        his.append(hi)
+    V = sum(v << (RB * (w * K + p)) for w in range(T) for p, v in enumerate(acc[w])) + \
+        sum(his[w] << (RB * (w * K + K)) for w in range(T))
     assert V % N == a * b * 2 ** (2 * RB) % N or drop_carry or h_shift
     if not h_shift:
         assert V < 2 * N * 2 ** (2 * RB)
-    return finish_model(N, acc, RB, LP, T, L32)
+    return finish_model(N, acc, RB, LP, T, L32, hi=his)
 
 
 def _moduli(rng, L32):
@@ -174,3 +232,26 @@ def test_pw_planted_mistakes_fail():
         for _ in range(8):
             a, b = rng.randrange(N), rng.randrange(N)
             assert pw_model(N, a, b, RB, LP, T, L32, h_shift=4) == a * b % N
+
+
+def test_pw_fold_intervals():
+    """The 29-bit shapes fold the product every 3 blocks of 18 rows and the
+    fold-table rows every 60 rows; the 28-bit shape never folds."""
+    assert product_fold_blocks(29, 144, 8) == 3 and product_fold_blocks(29, 72, 4) == 3
+    assert product_fold_blocks(28, 38, 2) == 2
+    assert fold_phase_rows(29) == 60 and fold_phase_rows(28) >= 2 * 38
+
+
+def test_pw_model_needs_the_product_folds():
+    """A missing fold in the product phase overflows at 29 bits."""
+    rng = random.Random(4)
+    RB, LP, T, L32 = SHAPES[0]
+    N = _moduli(rng, L32)[0]
+    x = (1 << (32 * L32)) - 1
+    orig = product_fold_blocks
+    globals()["product_fold_blocks"] = lambda RB_, LP_, T_: T_
+    try:
+        with pytest.raises(AssertionError):
+            pw_model(N, x, x, RB, LP, T, L32)
+    finally:
+        globals()["product_fold_blocks"] = orig
