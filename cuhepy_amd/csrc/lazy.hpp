@@ -1366,7 +1366,7 @@ __device__ __forceinline__ void fixed_units(const LazyDev* __restrict__ md, cons
   const int tiles = (count + FIX_TILE - 1) / FIX_TILE;
   // Work units in query-major order: the workgroups sweep the range for one
   // query before the next, so at any time they read one or two operand tables
-  // (97 KB each at 4096-bit moduli) instead of all of them. Units come from a
+  // (92 KB each at 4096-bit moduli) instead of all of them. Units come from a
   // queue, so a fast workgroup takes more units than a slow one (strided
   // units leave the launch waiting for its slowest workgroups, 14% slower
   // measured) and the queries complete one after another at the pace of the

@@ -110,8 +110,9 @@ int chx_mulmod_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* a, const uint
  * with x interleaved [limbs][stride] and every outs[q] interleaved
  * [limbs][out_stride], indexed by the global entry i. ops and outs are host
  * arrays of nq device pointers. Results are canonical (below n). The operand is
- * the query's table y * 2^(RB (i + 2)) mod n for every position i of the lazy
- * radix (97 KB at a 4096-bit n), so the multiply needs no reduction rows. */
+ * the query's table y * 2^(RB (i + 2)) mod n for every position i of the
+ * multiply's radix (144 positions of 29 bits at a 4096-bit n: 92 KB), so the
+ * multiply needs no reduction rows. */
 int chx_mulmod_operand_words(const chx_mod* m);
 int chx_mulmod_operands_dev(chx_ctx* ctx, const chx_mod* m, const uint32_t* y, uint32_t* ops, size_t count);
 /* the same from host limbs (count x limbs): stream-ordered, no synchronisation */
