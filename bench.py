@@ -95,7 +95,7 @@ def cpu_baseline(cp: dict, cs: list[int], want: list[int], cores: int) -> dict:
 
 def lazy_tag(key) -> str:
     """Template arguments of the exponentiation kernel the CRT halves use."""
-    return {32: "28, 38, 1, 32", 64: "28, 76, 4, 64", 128: "27, 152, 8, 128"}.get(key.half, "?")
+    return {32: "28, 38, 1, 32", 64: "29, 72, 4, 64", 128: "29, 144, 8, 128"}.get(key.half, "?")
 
 
 def committed_traffic(kernel: str):

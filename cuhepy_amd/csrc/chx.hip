@@ -456,8 +456,8 @@ const std::vector<LazyCfg>& lazy_cfgs(int L32) {
   // variants whose accumulators fit in VGPRs without spilling (checked with
   // -Rpass-analysis=kernel-resource-usage); index = dispatch slot
   static const std::vector<LazyCfg> c32 = {{28, 38, 1}, {28, 38, 2}};
-  static const std::vector<LazyCfg> c64 = {{28, 76, 4}, {28, 80, 8}};
-  static const std::vector<LazyCfg> c128 = {{27, 152, 8}, {27, 160, 16}};
+  static const std::vector<LazyCfg> c64 = {{29, 72, 4}, {29, 72, 8}};
+  static const std::vector<LazyCfg> c128 = {{29, 144, 8}, {29, 144, 16}};
   static const std::vector<LazyCfg> none = {};
   return L32 == 32 ? c32 : L32 == 64 ? c64 : L32 == 128 ? c128 : none;
 }
@@ -567,10 +567,10 @@ size_t lazy_lds_bytes() {
     const int l__ = (L32_), i__ = (IDX_);                                                       \
     if (l__ == 32 && i__ == 0) { constexpr int RB = 28, LP = 38, TPI = 1, L32 = 32; __VA_ARGS__; }      \
     else if (l__ == 32 && i__ == 1) { constexpr int RB = 28, LP = 38, TPI = 2, L32 = 32; __VA_ARGS__; } \
-    else if (l__ == 64 && i__ == 0) { constexpr int RB = 28, LP = 76, TPI = 4, L32 = 64; __VA_ARGS__; }  \
-    else if (l__ == 64 && i__ == 1) { constexpr int RB = 28, LP = 80, TPI = 8, L32 = 64; __VA_ARGS__; }  \
-    else if (l__ == 128 && i__ == 0) { constexpr int RB = 27, LP = 152, TPI = 8, L32 = 128; __VA_ARGS__; } \
-    else if (l__ == 128 && i__ == 1) { constexpr int RB = 27, LP = 160, TPI = 16, L32 = 128; __VA_ARGS__; } \
+    else if (l__ == 64 && i__ == 0) { constexpr int RB = 29, LP = 72, TPI = 4, L32 = 64; __VA_ARGS__; }  \
+    else if (l__ == 64 && i__ == 1) { constexpr int RB = 29, LP = 72, TPI = 8, L32 = 64; __VA_ARGS__; }  \
+    else if (l__ == 128 && i__ == 0) { constexpr int RB = 29, LP = 144, TPI = 8, L32 = 128; __VA_ARGS__; } \
+    else if (l__ == 128 && i__ == 1) { constexpr int RB = 29, LP = 144, TPI = 16, L32 = 128; __VA_ARGS__; } \
     else fail(CHX_EUNSUPPORTED, "no lazy kernel for %d limbs (configuration %d)", l__, i__);   \
   } while (0)
 
@@ -1633,11 +1633,11 @@ void ragged_enqueue(chx_ragged* rg) {
                        rg->d_scratch, rg->wave_words);
   };
   if (rg->variant == 2) {
-    mixed(Lazy<28, 80, 8, 64>(), Lazy<27, 160, 16, 128>(), k_ragged2<28, 80, 8, 27, 160, 16>);
+    mixed(Lazy<29, 72, 8, 64>(), Lazy<29, 144, 16, 128>(), k_ragged2<29, 72, 8, 29, 144, 16>);
     return;
   }
   if (rg->variant == 3) {
-    mixed(Lazy<28, 76, 4, 64>(), Lazy<27, 152, 8, 128>(), k_ragged2<28, 76, 4, 27, 152, 8>);
+    mixed(Lazy<29, 72, 4, 64>(), Lazy<29, 144, 8, 128>(), k_ragged2<29, 72, 4, 29, 144, 8>);
     return;
   }
   for (int q = 0; q < 2; ++q) {

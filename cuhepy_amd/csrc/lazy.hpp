@@ -1,16 +1,13 @@
 // Lazy-carry Montgomery exponentiation for gfx950 (the modexp hot loop).
 //
-// Numbers are held in LP positions of RB bits (RB = 28 or 27) inside 64-bit
-// accumulators. A limb product is < 2^(2 RB), and a position receives at most
-// 2 products per row of a general product (3 products' worth in a squaring
-// row, see row_sq) for LP rows, so 3 * LP * 2^(2 RB) < 2^64 and no carry has
-// to be propagated inside the product: every limb product is exactly one
-// in-place v_mad_u64_u32 (64-bit accumulator as the addend). Carries are
-// resolved once per product (settle). A wider radix (RB = 29: 10% fewer
-// positions at 2048 bits) overflows after 31 rows and needs carry folds inside
-// the product (`fold`, supported below); measured on the MI355X it was 14-20%
-// slower than RB = 28 (the folds and the lost occupancy cost more than the
-// saved MACs).
+// Numbers are held in LP positions of RB bits (RB = 28 or 29) inside 64-bit
+// accumulators. A limb product is < 2^(2 RB), so every limb product is exactly
+// one in-place v_mad_u64_u32 (64-bit accumulator as the addend) as long as no
+// accumulator can pass 2^64; carries are resolved once per product (settle).
+// The row step below splits each lane's bottom register every row, so a
+// register accumulates for at most K = LP / TPI rows (see FMAX): 29-bit
+// positions (72 at 2048-bit moduli, 144 at 4096-bit) need no carry folds at
+// 4 or 8 lanes per modulus.
 //
 // Montgomery product with R' = 2^(RB*LP) > 4N ("almost Montgomery", Walter
 // 2002): inputs and outputs stay below 2N, so no conditional subtraction is
@@ -51,23 +48,23 @@ struct Lazy {
   static_assert(LP % TPI == 0, "LP must be a multiple of TPI");
   static_assert(L32 % TPI == 0 && K32 % 4 == 0, "32-bit limbs per lane must be a multiple of 4");
   static_assert(RB * LP >= 32 * L32 + 2, "R' must exceed 4N");
-  // Carry folding. Between two folds a position receives at most 2 products
-  // (< 2^(2 RB) each) per row, on top of what a fold leaves (< 2^RB + 2^(64-RB))
-  // and one row carry (< 2^(64-RB)). FMAX rows keep that below 2^64. When all
-  // LP rows fit, no fold is needed (e.g. RB = 28 at 2048-bit moduli); otherwise the
-  // accumulators are folded every FR rows (a block of K rows, or half of one).
+  // Accumulator bound. A register holds one position for at most K rows: it
+  // becomes a lane's top position with the low RB bits passed down from the
+  // lane above (< 2^RB), takes one carry from the position below (< 2^(64-RB))
+  // and 2 products per row of a general product (at most 3 products' worth in
+  // a squaring row, see row_sq), and leaves as the lane's bottom position,
+  // whose bits above RB go to the lane's next register and whose low bits go
+  // one lane down. Products of settled positions (below 2^RB + 2^(64-2RB) + 1)
+  // are below P = 2^(2RB) + 2^(66-RB). FMAX (FMAX_SQ) is the most rows that
+  // keep a register below 2^64.
   static constexpr double TWO64 = 18446744073709551616.0;
-  static constexpr int FMAX =
-      (int)((TWO64 - 4.0 * (double)(1ull << (64 - RB))) / (2.0 * (double)(1ull << (2 * RB))));
-  static constexpr bool FOLD = LP > FMAX;
-  static constexpr int FR = !FOLD ? LP : (K <= FMAX ? K * (FMAX / K) : K / 2);
-  static_assert(FR <= FMAX || !FOLD, "fold interval exceeds the accumulator bound");
-  static_assert(!FOLD || K <= FMAX || (K % 2 == 0 && K / 2 <= FMAX), "block too long to fold at half");
-  // Symmetric squaring (row_sq) puts up to 3 products' worth into a position
-  // per row; it is used when LP such rows fit without folds.
-  static constexpr int FMAX_SQ =
-      (int)((TWO64 - 4.0 * (double)(1ull << (64 - RB))) / (3.0 * (double)(1ull << (2 * RB))));
-  static constexpr bool SYM = !FOLD && LP <= FMAX_SQ;
+  static constexpr double PMAX = (double)(1ull << (2 * RB)) + (double)(1ull << (66 - RB));
+  static constexpr double ROOM = TWO64 - 2.0 * (double)(1ull << (64 - RB));
+  static constexpr int FMAX = (int)(ROOM / (2.0 * PMAX));
+  static constexpr int FMAX_SQ = (int)(ROOM / (3.0 * PMAX));
+  static_assert(K <= FMAX, "more rows per register than a 64-bit accumulator holds");
+  // Symmetric squaring rows where K of them fit; plain rows otherwise.
+  static constexpr bool SYM = K <= FMAX_SQ;
   // Long product chains may run modulo N' = kN with k = -N^-1 mod 2^RB
   // instead of N: N' = -1 mod 2^RB, so -N'^-1 mod 2^RB = 1 and each row's
   // digit is the bottom position's low RB bits, with no multiply. Only the
@@ -97,25 +94,6 @@ struct Lazy {
     for (int p = 0; p < K; ++p) N[p] = npos[r * K + p];
   }
 
-  // One carry step on every position (not rippled): position p keeps its low
-  // RB bits plus the bits of position p-1 above RB; the top position of the
-  // top lane keeps all its bits. The value sum V[p] 2^(RB p) is unchanged.
-  // O = register of relative position 0 (the rotation of the row loop).
-  // In place, top position first, so each step reads the old value below it.
-  template <int O>
-  __device__ __forceinline__ void fold(u64 (&V)[K]) {
-    static_assert(K >= 2, "fold needs two positions per lane");
-    const u64 ctop = V[(K - 1 + O) % K] >> RB;
-    const u64 cin = make64(grp::shift_up((u32)ctop), grp::shift_up((u32)(ctop >> 32)));
-    const u64 topmask = grp::rank() == TPI - 1 ? ~0ull : (u64)MASK;
-#pragma unroll
-    for (int p = K - 1; p >= 1; --p) {
-      const u64 m = p == K - 1 ? topmask : (u64)MASK;
-      V[(p + O) % K] = (V[(p + O) % K] & m) + (V[(p - 1 + O) % K] >> RB);
-    }
-    V[O % K] = (V[O % K] & (u64)MASK) + cin;
-  }
-
   // stage this lane's positions as the streamed operand: row i = position i
   __device__ __forceinline__ void stage(const u32 (&x)[K]) {
     const int g = TPI == 1 ? grp::gid() : opaque(grp::gid()), r = TPI == 1 ? 0 : opaque(grp::rank());
@@ -136,9 +114,9 @@ struct Lazy {
   // its own positions, then passes its carry (< 2^(64-RB)) one lane up, where
   // it is absorbed by positions 0 and 1 only: every position ends below
   // D = 2^RB + 2^(64-2RB) + 1 instead of 2^RB. Products of such positions stay
-  // below 2^(2RB) (1 + 2^(66-3RB)), which the row bounds above absorb (they
-  // leave more than 10% of 2^64 unused). renorm() finishes the job where exact
-  // RB-bit positions are needed (to32, top32).
+  // below 2^(2RB) + 2^(66-RB) (PMAX, which the row bounds above use).
+  // renorm() finishes the job where exact RB-bit positions are needed (to32,
+  // top32).
   __device__ __forceinline__ void settle(u32 (&X)[K], const u64 (&V)[K]) {
     static_assert(TPI == 1 || K >= 2, "settle needs two positions per lane");
     u64 c = 0;
@@ -197,9 +175,6 @@ struct Lazy {
     for (int blk = 0; blk < TPI; ++blk) {
       const u32* arow = lds + (blk * K) * G + g;
       row<0, U>(V, b, arow);
-      if constexpr (FOLD && FR >= K) {
-        if ((blk + 1) % (FR / K) == 0) fold<0>(V);  // wave-uniform
-      }
     }
     settle(b, V);
   }
@@ -215,7 +190,6 @@ struct Lazy {
         V[q] = (u64)a * b[p] + V[q];
       }
       reduce<S, U>(V);
-      if constexpr (FOLD && FR < K && (S + 1) % FR == 0) fold<(S + 1) % K>(V);
       row<S + 1, U>(V, b, arow);
     }
   }

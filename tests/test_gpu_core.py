@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 
 OP_MODEXP, OP_MULMOD, OP_LOOKUP = 0, 1, 2
 # shape (RB, LP, waves) of k_pw_mulmod per width, and the host's fold-row rule
-PW_SHAPE = {32: (28, 38, 2), 64: (28, 76, 4), 128: (27, 152, 8)}
+PW_SHAPE = {32: (28, 38, 2), 64: (29, 72, 4), 128: (29, 144, 8)}
 from test_pw_model import fold_rows  # noqa: E402
 
 

@@ -17,9 +17,10 @@ import random
 
 import pytest
 
-# (RB, LP, L32) of the lazy configurations whose UNIT flag is set
-UNIT_SHAPES = [(28, 38, 32), (28, 76, 64), (28, 80, 64), (27, 160, 128)]
-# a built shape where it is not (R' has no room for N')
+# (RB, LP, L32) of the lazy configurations whose UNIT flag is set (every
+# built one: lazy_cfgs in chx.hip), and two earlier 28- and 27-bit shapes
+UNIT_SHAPES = [(28, 38, 32), (29, 72, 64), (29, 144, 128), (28, 76, 64), (27, 160, 128)]
+# an earlier shape where it is not (R' has no room for N')
 NON_UNIT = (27, 152, 128)
 
 
