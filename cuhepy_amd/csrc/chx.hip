@@ -153,10 +153,9 @@ struct chx_mod {
   ModDev* d = nullptr;
   LazySet lz;
   uint32_t* d_fixb = nullptr;  // served-multiply constants 2^(RB(i+2)) R' mod N (built on first use)
-  LazyDev* d_serve = nullptr;  // the modulus in the served multiply's positions (built with d_fixb)
   uint32_t* d_pwz = nullptr;   // fold table of the pointwise multiply (built on first use)
   int pw_h = -1, pw_nz = 0;    // its first folded position and row count; pw_h -2: not usable
-  std::mutex mu;               // guards d_fixb, d_serve and d_pwz: a modulus may serve several contexts of its device
+  std::mutex mu;               // guards d_fixb and d_pwz: a modulus may serve several contexts of its device
 };
 
 struct ExpSched {
@@ -667,9 +666,10 @@ u32* queue_head(chx_ctx* ctx) {
   return h;
 }
 
-// The one-reduction pointwise multiply (k_pw_mulmod) runs at the positions of
-// the served multiply (serve_cfg: 29 bits at 2048- and 4096-bit moduli, the
-// modulus constants shared with it) with 18 or 19 positions per chunk.
+// The one-reduction pointwise multiply (k_pw_mulmod) and the served multiply
+// run at the positions of the exponentiation engine (lazy configuration 0:
+// 29 bits at 2048- and 4096-bit moduli), whose modulus constants (lz.d[0])
+// they use, with 18 or 19 positions per chunk.
 #define CHX_DISPATCH_PW(L32_, ...)                                                                 \
   do {                                                                                            \
     const int l__ = (L32_);                                                                       \
@@ -683,10 +683,9 @@ static LazyCfg pw_shape(int L32) {
   return L32 == 32 ? LazyCfg{28, 38, 2} : L32 == 64 ? LazyCfg{29, 72, 4} : LazyCfg{29, 144, 8};
 }
 // Shape of the served multiply (k_fixed_operand, k_fixed_mulmod; query
-// operands are prepared in its positions). At 2048- and 4096-bit moduli it
-// uses 29-bit positions: 72 and 144 of them instead of 76 and 152 at 28 and
-// 27 bits, 10% fewer multiply-adds per entry; the row loop folds its
-// accumulators every FixedSeg rows (one and two folds per entry).
+// operands are prepared in its positions): 72 and 144 positions of 29 bits at
+// 2048- and 4096-bit moduli; its row loop folds its accumulators every
+// FixedSeg rows (one and two folds per entry).
 static LazyCfg serve_cfg(int L32) {
   return L32 == 32 ? LazyCfg{28, 38, 1} : L32 == 64 ? LazyCfg{29, 72, 4} : LazyCfg{29, 144, 8};
 }
@@ -700,12 +699,22 @@ static LazyCfg serve_cfg(int L32) {
     else fail(CHX_EUNSUPPORTED, "no served multiply for %d limbs", l__);                         \
   } while (0)
 
-// The modulus in the served shape's positions and B_i = 2^(RB (i + 2)) R' mod N
-// as positions (LP rows), uploaded once per modulus, on first use.
+// The served and pointwise multiplies use the modulus constants of lazy
+// configuration 0 (lz.d[0]); their positions must be the same.
+static const LazyDev* shared_constants(const chx_mod* m, const LazyCfg& c) {
+  const LazyCfg z = lazy_cfgs(m->hc.L)[0];
+  if (z.rb != c.rb || z.lp != c.lp || !m->lz.d[0]) fail(CHX_EINVAL, "internal: served positions differ from the engine's");
+  return m->lz.d[0];
+}
+
+// B_i = 2^(RB (i + 2)) R' mod N as positions (LP rows), uploaded once per
+// modulus, on first use.
 static void serve_constants(chx_ctx* ctx, chx_mod* m) {
+  if (lidx(m->hc.L) < 0) fail(CHX_EUNSUPPORTED, "no served multiply for %d limbs", m->hc.L);
   std::lock_guard<std::mutex> lock(m->mu);
   if (m->d_fixb) return;
   const LazyCfg c = serve_cfg(m->hc.L);
+  (void)shared_constants(m, c);
   std::vector<uint32_t> h((size_t)c.lp * c.lp);
   BN one(m->hc.L, 0u);
   one[0] = 1;
@@ -713,14 +722,7 @@ static void serve_constants(chx_ctx* ctx, chx_mod* m) {
     BN b = host::bn_shl_mod(one, c.rb * (i + 2 + c.lp), m->hc.N);
     to_positions(b, c.rb, c.lp, h.data() + (size_t)i * c.lp);
   }
-  LazyDev* d = make_lazy_dev(m->hc, c.rb, c.lp, ctx->stream);
-  try {
-    m->d_fixb = dev_upload(h.data(), h.size(), ctx->stream);
-  } catch (...) {
-    (void)hipFree(d);
-    throw;
-  }
-  m->d_serve = d;
+  m->d_fixb = dev_upload(h.data(), h.size(), ctx->stream);
 }
 
 
@@ -742,6 +744,7 @@ bool pw_constants(chx_ctx* ctx, chx_mod* m) {
     return false;
   }
   const LazyCfg c = pw_shape(L);
+  (void)shared_constants(m, c);
   const int k = c.lp / c.tpi, ch = (k + 3) / 4 * 4;
   const int h0 = (host::bn_bits(m->hc.N) - 2) / c.rb;
   const int nz = (2 * c.lp - h0 + 3) / 4 * 4;
@@ -776,7 +779,6 @@ void launch_mulmod_mod(chx_ctx* ctx, const chx_mod* mc, const uint32_t* a, const
   }
   // window 2 of CHX_OP_MULMOD selects the two-product form below
   if (ctx->tune_window[CHX_OP_MULMOD][li] != 2 && pw_constants(ctx, m)) {
-    serve_constants(ctx, m);
     const size_t tiles = (count + FIX_TILE - 1) / FIX_TILE;
     u32* head = queue_head(ctx);
     record_start(ctx, "pw_mulmod");
@@ -789,7 +791,7 @@ void launch_mulmod_mod(chx_ctx* ctx, const chx_mod* mc, const uint32_t* a, const
                   "hipFuncSetAttribute");
       });
       hipLaunchKernelGGL(kern, dim3(fixed_blocks(ctx, kern, 64 * TPI, lds, tiles)), dim3(64 * TPI), lds,
-                         ctx->stream, m->d_serve, m->d_pwz, m->pw_h, m->pw_nz, a, b, out, (int)count,
+                         ctx->stream, m->lz.d[0], m->d_pwz, m->pw_h, m->pw_nz, a, b, out, (int)count,
                          (int)stride, head);
     });
     record_stop(ctx);
@@ -1290,7 +1292,6 @@ int chx_mod_destroy(chx_mod* m) {
     hip_check(hipSetDevice(m->ctx->device), "hipSetDevice");
     if (m->d) (void)hipFree(m->d);
     if (m->d_fixb) (void)hipFree(m->d_fixb);
-    if (m->d_serve) (void)hipFree(m->d_serve);
     if (m->d_pwz) (void)hipFree(m->d_pwz);
     free_lazy_set(m->lz);
   });
@@ -1345,7 +1346,7 @@ static void launch_operands(chx_ctx* ctx, chx_mod* m, const uint32_t* y, uint32_
   if (count * (size_t)c.lp > (size_t)INT32_MAX / 64) fail(CHX_EINVAL, "too many operands (%zu)", count);
   CHX_DISPATCH_SERVE(L, {
     hipLaunchKernelGGL((k_fixed_operand<RB, LP, TPI, L32>), grid_for(count * LP, TPI), dim3(BLOCK_THREADS),
-                       (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, m->d_serve, m->d_fixb, y, ops,
+                       (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, m->lz.d[0], m->d_fixb, y, ops,
                        (int)count);
   });
   check_launch("k_fixed_operand");
@@ -1413,7 +1414,6 @@ static void multi_launch(chx_ctx* ctx, const chx_mod* m, const uint32_t* x, size
   } else {
     mq.head = queue_head(ctx);
   }
-  serve_constants(ctx, const_cast<chx_mod*>(m));
   record_start(ctx, "mulmod_multi");
   CHX_DISPATCH_SERVE(L, {
     auto kern = k_fixed_mulmod<RB, LP, TPI, L32>;
@@ -1424,7 +1424,7 @@ static void multi_launch(chx_ctx* ctx, const chx_mod* m, const uint32_t* x, size
                 "hipFuncSetAttribute");
     });
     hipLaunchKernelGGL(kern, dim3(fixed_blocks(ctx, kern, 64 * TPI, lds, tiles * nq)), dim3(64 * TPI), lds,
-                       ctx->stream, m->d_serve, x, (int)begin, (int)count, (int)stride, mq,
+                       ctx->stream, m->lz.d[0], x, (int)begin, (int)count, (int)stride, mq,
                        (int)out_stride);
   });
   record_stop(ctx);
