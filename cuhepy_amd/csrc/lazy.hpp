@@ -200,9 +200,9 @@ struct Lazy {
   __device__ __forceinline__ void reduce(u64 (&V)[K]) {
     u32 m;
     if constexpr (U)
-      m = grp::bcast0((u32)V[S]) & maskv;
+      m = grp::bcast0_and((u32)V[S], maskv);
     else
-      m = grp::bcast0((u32)V[S] * n0) & maskv;
+      m = grp::bcast0_and((u32)V[S] * n0, maskv);
 #pragma unroll
     for (int p = 0; p < K; ++p) {
       const int q = (p + S) % K;

@@ -67,6 +67,21 @@ template <int TPI> struct Grp {
     }
   }
 
+  // bcast0(v) & mask. For 8 and 16 lanes the AND rides on the quad broadcast
+  // (v_and_b32_dpp: every source lane of quad_perm [0,0,0,0] exists, so
+  // bound_ctrl and the old value never take effect) and the masked quads are
+  // then copied up the row: two VALU instructions instead of three.
+  __device__ __forceinline__ static u32 bcast0_and(u32 v, u32 mask) {
+    if constexpr (TPI == 8 || TPI == 16) {
+      int x = __builtin_amdgcn_update_dpp(0, (int)v, 0x00, 0xF, 0xF, true) & (int)mask;
+      x = __builtin_amdgcn_update_dpp(x, x, 0x114, 0xF, 0xA, false);  // row_shr:4, banks 1, 3
+      if constexpr (TPI == 16) x = __builtin_amdgcn_update_dpp(x, x, 0x118, 0xF, 0xC, false);  // row_shr:8
+      return (u32)x;
+    } else {
+      return bcast0(v) & mask;
+    }
+  }
+
   // lane r receives v from rank r+1; the top rank receives 0
   __device__ __forceinline__ static u32 shift_down(u32 v) {
     if constexpr (TPI == 1) {
