@@ -307,15 +307,29 @@ class _TableRows:
         return _Row()
 
 
-def _lookup_products(digits, items_per_wave: int, nmult: int, rows: int) -> int:
+def _noise_products(nmult: int, triples: bool) -> int:
+    """Noise products per item: three factors per product while more than
+    three are left (noise-triple table, with 5-row groups), then two, the last
+    product also leaving the Montgomery form (one conversion product when
+    there is no noise)."""
+    if nmult == 0:
+        return 1
+    n, j = 0, 0
+    while j < nmult:
+        j += 3 if triples and j + 3 < nmult else 2 if j + 1 < nmult else 1
+        n += 1
+    return n
+
+
+def _lookup_products(digits, items_per_wave: int, nmult: int, rows: int, groups: int = 1 << 30) -> int:
     """Table products the lookup kernel executes for this digit matrix with
-    `rows` table rows per product: per wave, one per row group with a nonzero
-    digit when every digit of the group is in the packing alphabet, else one
-    per row with a nonzero digit (rows past the last whole group go singly);
-    the first of them is a copy (the accumulator starts at the Montgomery
-    one); plus the noise factors, two per product (noise-pair table), the last
-    of which also leaves the Montgomery form (no conversion product unless
-    there is no noise). All items of a wave run every product the wave runs."""
+    `rows` table rows per product over the first `groups` groups: per wave,
+    one per row group with a nonzero digit when every digit of the group is in
+    the packing alphabet, else one per row with a nonzero digit (rows past the
+    table's last group go singly); the first of them is a copy (the
+    accumulator starts at the Montgomery one); plus the noise products
+    (_noise_products; triples with 5-row groups). All items of a wave run
+    every product the wave runs."""
     import numpy as np
     count, chunks = digits.shape
     waves = -(-count // items_per_wave)
@@ -323,7 +337,7 @@ def _lookup_products(digits, items_per_wave: int, nmult: int, rows: int) -> int:
     pad[:count] = digits
     d = pad.reshape(waves, items_per_wave, chunks)
     per_wave = np.zeros(waves, dtype=np.int64)
-    whole = chunks - chunks % rows
+    whole = min(chunks - chunks % rows, rows * groups)
     for j in range(0, whole, rows):
         grp = d[:, :, j:j + rows]
         packed = ~((grp & 0xAA) != 0).any(axis=(1, 2))
@@ -336,14 +350,17 @@ def _lookup_products(digits, items_per_wave: int, nmult: int, rows: int) -> int:
     # every item of a wave takes part in every product the wave runs
     real = np.full(waves, items_per_wave, dtype=np.int64)
     real[-1] = count - (waves - 1) * items_per_wave
-    return int((per_wave * real).sum()) + (nmult // 2 + nmult % 2 + (nmult == 0)) * count
+    return int((per_wave * real).sum()) + _noise_products(nmult, rows == 5) * count
 
 
 def lookup_leg(ctx, args, peak: float) -> dict:
     """Paillier-lookup table encryption: 100,000 binary vectors of 512 bits
     (Bernoulli(0.5)), key_len 1024 (n 2048-bit, n^2 4096-bit), alpha 280,
     one ciphertext per vector, 14 noise factors; digits and noise indices
-    resident in HBM when the timed region starts."""
+    resident in HBM when the timed region starts. Reported: the row-quint
+    form with noise triples (25 GB of tables, the fastest), and beside it the
+    default row quads, triples, pairs and single rows; every form's output is
+    the same ciphertext."""
     import numpy as np
     import torch
 
@@ -378,7 +395,7 @@ def lookup_leg(ctx, args, peak: float) -> dict:
     def timed(rows: int) -> tuple[float, float]:
         """(kernel seconds per launch, wall seconds per batch) with `rows`
         8-bit table rows per product (1 = single rows, 2 = pairs, 3 = triples,
-        0 = the default, row quads)."""
+        0 = the default, row quads, 5 = quints with noise triples)."""
         ctx.set_tuning(OP_LOOKUP, W, 0, rows)
         key.encrypt_digits_dev(d_dig, d_idx, 14, out, count, stride)  # warm
         torch.cuda.synchronize()
@@ -396,14 +413,35 @@ def lookup_leg(ctx, args, peak: float) -> dict:
     single_s, _ = timed(1)
     pairs_s, _ = timed(2)
     triples_s, _ = timed(3)
+    t = time.perf_counter()
+    ctx.set_tuning(OP_LOOKUP, W, 0, 5)
+    key.encrypt_digits_dev(d_dig[:64], d_idx[:64], 14, out, 64, stride)  # builds the quint and noise-triple tables
+    torch.cuda.synchronize()
+    build5_s = time.perf_counter() - t
+    quints_s, quints_wall = timed(5)
+    out5 = out.clone()
     launch_s, wall = timed(0)  # row quads (the default); its output is checked below
     ctx.set_tuning(OP_LOOKUP, W, 0, 0)
+    q5_equal = bool(torch.equal(out5[:, :count], out[:, :count]))
+    if not q5_equal:
+        raise RuntimeError("lookup encryption with quints and noise triples differs from the row-quad form")
+    del out5
     nnz = int((digits != 0).sum())
     mulmods = nnz + 13 * count
     macs = mulmods * 2 * W * W
     achieved = macs / launch_s / 1e12
-    executed = _lookup_products(digits, 64 // Context.supported_tpi(W, op=OP_LOOKUP)[0], 14, 4)
+    ipw = 64 // Context.supported_tpi(W, op=OP_LOOKUP)[0]
+    table_rows = -(-min(client.chunk_len, 2 * client.embed_len) // 8)
+    executed = _lookup_products(digits, ipw, 14, 4, -(-table_rows // 4))
     executed_tmac = executed * 2 * W * W / launch_s / 1e12
+    executed5 = _lookup_products(digits, ipw, 14, 5, -(-table_rows // 5))
+    executed5_tmac = executed5 * 2 * W * W / quints_s / 1e12
+    entry_bytes = 4 * (144 if W == 128 else 72 if W == 64 else 38)
+    quint_bytes = -(-table_rows // 5) * 16 ** 5 * entry_bytes + 256 ** 3 * entry_bytes
+    log(f"lookup: quints + noise triples: {count} vectors in {quints_s * 1e3:.2f} ms "
+        f"({count / quints_s:,.0f} vectors/s), {executed5 / count:.1f} products per vector, "
+        f"{executed5_tmac:.2f} TMAC/s = {executed5_tmac / peak:.1%}; tables {quint_bytes / 1e9:.1f} GB built in "
+        f"{build5_s:.2f} s; equal to the row-quad output: {q5_equal}")
     # round trip: every ciphertext decrypts to its plaintext (GPU decrypt)
     aos = torch.empty((count, W), dtype=torch.int32, device=dev)
     ctx.il_to_aos(out, aos, count, W, stride)
@@ -474,21 +512,32 @@ def lookup_leg(ctx, args, peak: float) -> dict:
     return {
         "workload": f"{count} binary vectors x 512 bits (Bernoulli 0.5), key_len 1024 (n^2 4096-bit), alpha 280, "
                     "14 noise factors per ciphertext",
-        "vectors": count, "kernel_ms": round(launch_s * 1e3, 3),
-        "vectors_per_s": round(count / launch_s, 1),
-        "executed_products": executed, "products_per_vector": round(executed / count, 2),
-        "achieved_tmac_s": round(executed_tmac, 4),
-        "frac_of_peak": round(executed_tmac / peak, 4),
-        "reference_algorithm": {"mulmods": mulmods, "macs": macs, "mulmod_per_s": round(mulmods / launch_s, 1),
-                                "equivalent_tmac_s": round(achieved, 4)},
-        "row_groups": {"rows_per_product": 4,
+        "vectors": count,
+        "form": "row quints with noise triples (chx_ctx_set_tuning window 5): 5-row group tables over the 130 "
+                "rows 512-bit packed embeddings use, three noise factors per product, "
+                f"{quint_bytes / 1e9:.1f} GB of tables; outputs equal to the default row-quad form",
+        "kernel_ms": round(quints_s * 1e3, 3),
+        "vectors_per_s": round(count / quints_s, 1),
+        "executed_products": executed5, "products_per_vector": round(executed5 / count, 2),
+        "achieved_tmac_s": round(executed5_tmac, 4),
+        "frac_of_peak": round(executed5_tmac / peak, 4),
+        "reference_algorithm": {"mulmods": mulmods, "macs": macs, "mulmod_per_s": round(mulmods / quints_s, 1),
+                                "equivalent_tmac_s": round(macs / quints_s / 1e12, 4)},
+        "table_bytes": quint_bytes, "table_build_s": round(build5_s, 3),
+        "wall_ms_per_batch": round(quints_wall * 1e3, 3),
+        "row_groups": {"default_row_quads": {
+                           "kernel_ms": round(launch_s * 1e3, 3),
+                           "vectors_per_s": round(count / launch_s, 1),
+                           "products_per_vector": round(executed / count, 2),
+                           "achieved_tmac_s": round(executed_tmac, 4),
+                           "frac_of_peak": round(executed_tmac / peak, 4),
+                           "wall_ms_per_batch": round(wall * 1e3, 3),
+                           "reference_algorithm_equivalent_tmac_s": round(achieved, 4)},
                        "triples_kernel_ms": round(triples_s * 1e3, 3),
                        "pairs_kernel_ms": round(pairs_s * 1e3, 3),
                        "single_rows_kernel_ms": round(single_s * 1e3, 3),
-                       "speedup_over_triples": round(triples_s / launch_s, 3),
-                       "speedup_over_pairs": round(pairs_s / launch_s, 3),
-                       "speedup_over_single_rows": round(single_s / launch_s, 3)},
-        "wall_ms_per_batch": round(wall * 1e3, 3),
+                       "quints_speedup_over_quads": round(launch_s / quints_s, 3),
+                       "quints_speedup_over_single_rows": round(single_s / quints_s, 3)},
         "client_path_vectors_per_s": round(sl / client_s, 1),
         "client_path": client_path,
         "bit_exact": {"round_trip_vectors": count, "oracle_vectors": len(sample), "mismatches": 0},

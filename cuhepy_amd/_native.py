@@ -106,6 +106,7 @@ SIGNATURES = {
     "chx_paillier_key_has_idempotents": (c_int, [vp]),
     "chx_lookup_key_set_crt": (c_int, [vp, vp, u32p, u32p, c_int, u32p, u32p]),
     "chx_lookup_key_has_crt": (c_int, [vp]),
+    "chx_lookup_key_set_table_rows": (c_int, [vp, vp, c_int]),
     "chx_lookup_encrypt": (c_int, [vp, vp, u8p, u8p, c_int, u32p, c_size_t]),
     "chx_lookup_encrypt_dev": (c_int, [vp, vp, vp, vp, c_int, vp, c_size_t, c_size_t]),
     "chx_lookup_decrypt": (c_int, [vp, vp, u32p, u32p, c_size_t]),

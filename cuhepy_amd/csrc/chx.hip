@@ -190,12 +190,17 @@ struct chx_lkey {
   uint32_t* d_gar = nullptr;   // g_a_inv * R mod n
   uint32_t* d_gtab = nullptr;  // Montgomery-form g table, AoS
   uint32_t* d_ntab = nullptr;  // Montgomery-form noise table, AoS
-  uint32_t* d_ptab = nullptr;  // row-pair table (k_lazy_group_table<2>), built on first use
-  bool ptab_failed = false;    // no memory for it: single rows instead
-  uint32_t* d_ttab = nullptr;  // row-triple table (k_lazy_group_table<3>), built on first use
-  bool ttab_failed = false;    // no memory for it: row pairs instead
-  uint32_t* d_qtab = nullptr;  // row-quad table (k_lazy_group_table<4>), built on first use
-  bool qtab_failed = false;    // no memory for it: triples instead
+  // row-group tables (k_lazy_group_table<R>), group[R] for R = 2..5 rows per
+  // entry, each built on first use over its first `groups` groups; failed: no
+  // memory for it, the next smaller group serves
+  struct GroupTab {
+    uint32_t* d = nullptr;
+    int groups = 0;
+    bool failed = false;
+  } group[6];
+  int table_rows = 0;  // rows the group tables cover, 0 = all (chx_lookup_key_set_table_rows)
+  uint32_t* d_n3tab = nullptr;  // noise-triple table (k_lazy_noise_triples), noise^3 entries, with 5-row groups
+  bool n3tab_failed = false;
   uint32_t* d_nptab = nullptr;  // noise-pair table (k_lazy_noise_pairs), noise^2 entries
   uint32_t* d_ntab_plain = nullptr;   // the noise table as plain residues (k_lazy_plain)
   uint32_t* d_nptab_plain = nullptr;  // the noise-pair table as plain residues
@@ -997,6 +1002,7 @@ const ExpSched& get_sched_l(chx_lkey* k, int window) {
 }
 
 uint32_t* lazy_group_table(chx_ctx* ctx, const LazySet& ls, const uint32_t* gtab, size_t groups, int R);
+uint32_t* lazy_noise_triples(chx_ctx* ctx, const LazySet& ls, const uint32_t* nptab, const uint32_t* ntab, int nn);
 
 // any_packed (optional) is asked before a row-group table would be built:
 // false means that no item has all its digits in the packing alphabet, so no
@@ -1008,43 +1014,52 @@ void launch_lookup_encrypt(chx_lkey* k, const uint8_t* digits, const uint8_t* ni
   const int L = k->W;
   const int ci = pick_lazy(ctx, L, count, CHX_OP_LOOKUP);
   // rows per table product: the context's lookup window (0 = quads). The
-  // group tables (16^R entries per R rows: 19.9 MB, 212 MB and 2.55 GB for
-  // 256 rows at a 4096-bit n^2) are built on first use; without the memory
-  // for one, the next smaller group serves.
-  int rows = ctx->tune_window[CHX_OP_LOOKUP][lidx(L)];
-  if (rows == 0) rows = 4;
+  // group tables (16^R entries per R rows: 18.9 MB, 200 MB, 2.42 GB and 30.8
+  // GB for 256 rows at a 4096-bit n^2; the 128 rows a 512-bit packed
+  // embedding uses take half, chx_lookup_key_set_table_rows) are built on first use; without the memory for
+  // one, the next smaller group serves. Window 5 also builds the noise-triple
+  // table (16.8 M entries, 9.7 GB at a 4096-bit n^2 and 256 noise entries).
+  const int window = ctx->tune_window[CHX_OP_LOOKUP][lidx(L)];
+  int rows = window == 0 ? 4 : window;
   int packed = -1;  // any_packed's answer, asked once
-  auto ensure = [&](uint32_t*& tab, bool& failed, int R) {
-    if (tab || failed || k->chunks < R) return;
+  auto ensure = [&](int R) {
+    chx_lkey::GroupTab& t = k->group[R];
+    if (t.d || t.failed || k->chunks < R) return;
     if (any_packed && packed < 0) packed = any_packed() ? 1 : 0;
     if (packed == 0) return;
+    size_t groups = (size_t)(k->chunks / R);
+    if (k->table_rows > 0) groups = std::min(groups, (size_t)((k->table_rows + R - 1) / R));
     try {
-      tab = lazy_group_table(ctx, k->lz_n2, k->d_gtab, k->chunks / R, R);
+      t.d = lazy_group_table(ctx, k->lz_n2, k->d_gtab, groups, R);
+      t.groups = (int)groups;
     } catch (const ChxError& e) {
       if (e.code != CHX_ENOMEM) throw;
       (void)hipGetLastError();  // the failed allocation is not this launch's error
-      failed = true;
+      t.failed = true;
     }
   };
-  if (rows == 4) {
-    ensure(k->d_qtab, k->qtab_failed, 4);
-    if (!k->d_qtab) rows = 3;
+  for (; rows >= 2; --rows) {
+    ensure(rows);
+    if (k->group[rows].d) break;
   }
-  if (rows == 3) {
-    ensure(k->d_ttab, k->ttab_failed, 3);
-    if (!k->d_ttab) rows = 2;
+  const uint32_t* ptab = rows >= 2 ? k->group[rows].d : nullptr;
+  const int pgroups = rows >= 2 ? k->group[rows].groups : 0;
+  if (window == 5 && k->noise > 0 && nmult > 3 && !k->d_n3tab && !k->n3tab_failed) {
+    try {
+      k->d_n3tab = lazy_noise_triples(ctx, k->lz_n2, k->d_nptab, k->d_ntab, k->noise);
+    } catch (const ChxError& e) {
+      if (e.code != CHX_ENOMEM) throw;
+      (void)hipGetLastError();
+      k->n3tab_failed = true;
+    }
   }
-  if (rows == 2) {
-    ensure(k->d_ptab, k->ptab_failed, 2);
-    if (!k->d_ptab) rows = 1;
-  }
-  const uint32_t* ptab = rows == 2 ? k->d_ptab : rows == 3 ? k->d_ttab : rows == 4 ? k->d_qtab : nullptr;
+  const uint32_t* n3tab = window == 5 ? k->d_n3tab : nullptr;
   record_start(ctx, "lookup_encrypt");
   CHX_DISPATCH_LAZY(L, ci, {
     hipLaunchKernelGGL((k_lazy_lookup_encrypt<RB, LP, TPI, L32>), grid_for(count, TPI), dim3(BLOCK_THREADS),
                        (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, k->lz_n2.d[ci], k->d_gtab, ptab, rows,
-                       k->d_ntab, k->d_nptab, k->d_ntab_plain, k->d_nptab_plain, digits, k->chunks, nidx, nmult,
-                       k->noise, c, (int)count, (int)stride);
+                       pgroups, k->d_ntab, k->d_nptab, n3tab, k->d_ntab_plain, k->d_nptab_plain, digits, k->chunks,
+                       nidx, nmult, k->noise, c, (int)count, (int)stride);
   });
   record_stop(ctx);
   check_launch("k_lazy_lookup_encrypt");
@@ -1098,6 +1113,31 @@ uint32_t* lazy_noise_pairs(chx_ctx* ctx, const LazySet& ls, const uint32_t* ntab
   return dst;
 }
 
+// noise-triple table from the device noise-pair and noise tables (nn^2 and
+// nn entries, positions of configuration 0); returns the buffer (nn^3 entries)
+uint32_t* lazy_noise_triples(chx_ctx* ctx, const LazySet& ls, const uint32_t* nptab, const uint32_t* ntab, int nn) {
+  const int L = ls.L32;
+  const LazyCfg cfg = lazy_cfgs(L)[0];
+  const int ci = 0;
+  const size_t entries = (size_t)nn * nn * nn;
+  if (!nptab || !ntab) fail(CHX_EINVAL, "noise triples need the noise and noise-pair tables");
+  if (entries > (size_t)INT32_MAX / 2) fail(CHX_EINVAL, "too many noise-triple entries");
+  uint32_t* dst = nullptr;
+  hipError_t e = hipMalloc(&dst, entries * cfg.lp * sizeof(uint32_t));
+  if (e != hipSuccess) fail(CHX_ENOMEM, "hipMalloc: %s", hipGetErrorString(e));
+  CHX_DISPATCH_LAZY(L, ci, {
+    hipLaunchKernelGGL((k_lazy_noise_triples<RB, LP, TPI, L32>), grid_for(entries, TPI), dim3(BLOCK_THREADS),
+                       (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, ls.d[ci], nptab, ntab, nn, dst);
+  });
+  hipError_t le = hipGetLastError();
+  if (le != hipSuccess) {
+    (void)hipFree(dst);
+    fail(CHX_EDEVICE, "k_lazy_noise_triples: %s", hipGetErrorString(le));
+  }
+  hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  return dst;
+}
+
 // plain residues (MontMul(e, 1)) of a device table in lazy Montgomery form
 // (positions of configuration 0); returns the buffer (entries x LP)
 uint32_t* lazy_plain_copy(chx_ctx* ctx, const LazySet& ls, const uint32_t* in, size_t entries) {
@@ -1120,14 +1160,14 @@ uint32_t* lazy_plain_copy(chx_ctx* ctx, const LazySet& ls, const uint32_t* in, s
   return dst;
 }
 
-// row-group table of `groups` groups of R = 2 to 4 rows from the device g
+// row-group table of `groups` groups of R = 2 to 5 rows from the device g
 // table (both in the positions of configuration 0); returns the buffer
 // (groups x 16^R entries)
 uint32_t* lazy_group_table(chx_ctx* ctx, const LazySet& ls, const uint32_t* gtab, size_t groups, int R) {
   const int L = ls.L32;
   const LazyCfg cfg = lazy_cfgs(L)[0];
   const int ci = 0;
-  if (R < 2 || R > 4) fail(CHX_EINVAL, "row groups of %d rows are not built", R);
+  if (R < 2 || R > 5) fail(CHX_EINVAL, "row groups of %d rows are not built", R);
   const size_t entries = groups * ((size_t)1 << (4 * R));
   if (entries > (size_t)INT32_MAX / 2) fail(CHX_EINVAL, "too many table rows");
   uint32_t* dst = nullptr;
@@ -1136,6 +1176,9 @@ uint32_t* lazy_group_table(chx_ctx* ctx, const LazySet& ls, const uint32_t* gtab
   CHX_DISPATCH_LAZY(L, ci, {
     if (R == 2)
       hipLaunchKernelGGL((k_lazy_group_table<RB, LP, TPI, L32, 2>), grid_for(entries, TPI), dim3(BLOCK_THREADS),
+                         (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, ls.d[ci], gtab, dst, (int)entries);
+    else if (R == 5)
+      hipLaunchKernelGGL((k_lazy_group_table<RB, LP, TPI, L32, 5>), grid_for(entries, TPI), dim3(BLOCK_THREADS),
                          (lazy_lds_bytes<RB, LP, TPI, L32>()), ctx->stream, ls.d[ci], gtab, dst, (int)entries);
     else if (R == 4)
       hipLaunchKernelGGL((k_lazy_group_table<RB, LP, TPI, L32, 4>), grid_for(entries, TPI), dim3(BLOCK_THREADS),
@@ -1234,7 +1277,7 @@ int chx_ctx_set_tuning(chx_ctx* ctx, int op, int limbs, int tpi, int window) {
       if (!ok) fail(CHX_EUNSUPPORTED, "tpi=%d is not built for op %d at %d limbs", tpi, op, limbs);
     }
     if (window < 0 || window > 7) fail(CHX_EINVAL, "window %d out of range [1, 7]", window);
-    if (op == CHX_OP_LOOKUP && window > 4) fail(CHX_EINVAL, "lookup rows per product %d out of range [1, 4]", window);
+    if (op == CHX_OP_LOOKUP && window > 5) fail(CHX_EINVAL, "lookup rows per product %d out of range [1, 5]", window);
     if (op == CHX_OP_MULMOD && window > 2) fail(CHX_EINVAL, "mulmod form %d out of range [1, 2]", window);
     ctx->tune_tpi[op][li] = tpi;
     ctx->tune_window[op][li] = window;
@@ -2232,9 +2275,10 @@ int chx_lookup_key_destroy(chx_lkey* k) {
   int rc = guarded([&] {
     hip_check(hipSetDevice(k->ctx->device), "hipSetDevice");
     for (void* p : {(void*)k->d_n2, (void*)k->d_n, (void*)k->d_gar, (void*)k->d_gtab, (void*)k->d_ntab,
-                    (void*)k->d_ptab, (void*)k->d_ttab, (void*)k->d_qtab, (void*)k->d_nptab,
-                    (void*)k->d_ntab_plain, (void*)k->d_nptab_plain})
+                    (void*)k->d_n3tab, (void*)k->d_nptab, (void*)k->d_ntab_plain, (void*)k->d_nptab_plain})
       if (p) (void)hipFree(p);
+    for (auto& t : k->group)
+      if (t.d) (void)hipFree(t.d);
     if (k->d_crt) (void)hipFree(k->d_crt);
     free_lazy_set(k->lz_n2);
     free_lazy_set(k->lz_p2);
@@ -2290,6 +2334,23 @@ int chx_lookup_key_set_crt(chx_ctx* ctx, chx_lkey* k, const uint32_t* p, const u
 }
 
 int chx_lookup_key_has_crt(const chx_lkey* k) { return k && k->crt ? 1 : 0; }
+
+int chx_lookup_key_set_table_rows(chx_ctx* ctx, chx_lkey* k, int rows) {
+  return with_ctx(ctx, [&] {
+    if (!k) fail(CHX_EINVAL, "null argument");
+    if (k->ctx != ctx) fail(CHX_EINVAL, "key belongs to another context");
+    if (rows < 0) fail(CHX_EINVAL, "rows %d out of range", rows);
+    if (rows == k->table_rows) return;
+    // tables built for the old coverage go; the next encryption that asks
+    // for one builds it for the new one
+    hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    for (auto& t : k->group) {
+      if (t.d) (void)hipFree(t.d);
+      t = chx_lkey::GroupTab();
+    }
+    k->table_rows = rows;
+  });
+}
 
 int chx_lookup_width(const chx_lkey* k) { return k ? k->W : 0; }
 

@@ -606,7 +606,7 @@ template <int RB, int LP, int TPI, int L32, int R>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_group_table(const LazyDev* __restrict__ md,
                                                                     const u32* __restrict__ gtab,
                                                                     u32* __restrict__ out, int entries) {
-  static_assert(R >= 2 && R <= 4, "2 to 4 rows per entry");
+  static_assert(R >= 2 && R <= 5, "2 to 5 rows per entry");
   extern __shared__ uint4 smem4[];
   typedef Lazy<RB, LP, TPI, L32> E;
   constexpr int K = E::K, SH = 4 * R;
@@ -666,6 +666,37 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_noise_pairs(const LazyDe
   }
 }
 
+// Noise-triple table: entry (a * nn + b) * nn + c = noise[a] * noise[b] *
+// noise[c] mod n^2, from the noise-pair table (entry a * nn + b) and the noise
+// table, one product per entry, so three noise factors of an encryption take
+// one product. AoS positions in lazy Montgomery form, one group per entry.
+template <int RB, int LP, int TPI, int L32>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_noise_triples(const LazyDev* __restrict__ md,
+                                                                      const u32* __restrict__ nptab,
+                                                                      const u32* __restrict__ ntab, int nn,
+                                                                      u32* __restrict__ out) {
+  extern __shared__ uint4 smem4[];
+  typedef Lazy<RB, LP, TPI, L32> E;
+  constexpr int K = E::K;
+  u32* area = reinterpret_cast<u32*>(smem4) + (threadIdx.x / 64) * E::AREA;
+  E M;
+  M.init(area, md->N, md->n0);
+  const int entries = nn * nn * nn;
+  const int item = group_item<TPI>();
+  const int it = item < entries ? item : entries - 1;
+  const int r = Grp<TPI>::rank();
+  const u32* a = nptab + (size_t)(it / nn) * LP + r * K;
+  const u32* b = ntab + (size_t)(it % nn) * LP + r * K;
+  u32 x[K], y[K];
+#pragma unroll
+  for (int p = 0; p < K; ++p) x[p] = a[p], y[p] = b[p];
+  M.mul_into(x, y);
+  if (item < entries) {
+#pragma unroll
+    for (int p = 0; p < K; ++p) out[(size_t)item * LP + r * K + p] = x[p];
+  }
+}
+
 // Plain residues of a table in lazy Montgomery form: out[e] = MontMul(in[e],
 // 1) = the entry's value v (at most N) instead of v R'. AoS positions, one
 // group per entry.
@@ -699,12 +730,14 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lazy_plain(const LazyDev* __r
 // the wave has digit 0 are skipped (g_table[i][0] = 1); other items with
 // digit 0 multiply by their row's j = 0 entry (the Montgomery one), so the
 // wave stays convergent and the product is unchanged. With a row-group table
-// of R = prow rows per entry (ptab, k_lazy_group_table), rows R j .. R j + R - 1
-// take one product from it when every item of the wave has all R digits in
-// the packing alphabet; otherwise the wave multiplies the rows one at a time
-// (so do rows past the last whole group). Either way the result is the same
-// residue. With a noise-pair table (nptab, k_lazy_noise_pairs) the noise
-// factors go two per product. The first product of a wave with an active row
+// of R = prow rows per entry (ptab, k_lazy_group_table, the first pgroups
+// groups), rows R j .. R j + R - 1 take one product from it when every item
+// of the wave has all R digits in the packing alphabet; otherwise the wave
+// multiplies the rows one at a time (so do rows past the last group of the
+// table). Either way the result is the same residue. With a noise-pair table
+// (nptab, k_lazy_noise_pairs) the noise factors go two per product, with a
+// noise-triple table (n3tab, k_lazy_noise_triples) three, as long as more
+// than three are left. The first product of a wave with an active row
 // is a copy (the accumulator starts at the Montgomery one). With plain-residue
 // copies of the noise tables (ntab_plain, nptab_plain: v instead of v R'),
 // the last noise product also leaves the Montgomery form, so no separate
@@ -713,8 +746,10 @@ template <int RB, int LP, int TPI, int L32>
 __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_lazy_lookup_encrypt(const LazyDev* __restrict__ md,
                                                                        const u32* __restrict__ gtab,
                                                                        const u32* __restrict__ ptab, int prow,
+                                                                       int pgroups,
                                                                        const u32* __restrict__ ntab,
                                                                        const u32* __restrict__ nptab,
+                                                                       const u32* __restrict__ n3tab,
                                                                        const u32* __restrict__ ntab_plain,
                                                                        const u32* __restrict__ nptab_plain,
                                                                        const unsigned char* __restrict__ digits,
@@ -746,7 +781,7 @@ __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(3
     const u32* e = gtab + ((size_t)i * 256 + d0) * LP;
     bool act;
     int step = 1;
-    if (ptab != nullptr && i % prow == 0 && i + prow <= chunks) {
+    if (ptab != nullptr && i % prow == 0 && i + prow <= chunks && i / prow < pgroups) {
       int dor = d0, idx = squeeze4(d0);
       for (int t = 1; t < prow; ++t) {
         const int dt = dg[i + t];
@@ -783,11 +818,13 @@ __global__ __launch_bounds__(BLOCK_THREADS) __attribute__((amdgpu_waves_per_eu(3
   for (int j = 0; j < nmult;) {
     // index mod table size, as lcu:933-934
     const int a = ni[j] % nnoise;
-    const bool pair = nptab != nullptr && j + 1 < nmult;
-    const int step = pair ? 2 : 1;
+    const bool triple = n3tab != nullptr && j + 3 < nmult;  // never the last product
+    const bool pair = !triple && nptab != nullptr && j + 1 < nmult;
+    const int step = triple ? 3 : pair ? 2 : 1;
     const bool last = plain && j + step >= nmult;
-    const u32* e = pair ? (last ? nptab_plain : nptab) + ((size_t)a * nnoise + ni[j + 1] % nnoise) * LP
-                        : (last ? ntab_plain : ntab) + (size_t)a * LP;
+    const u32* e = triple ? n3tab + (((size_t)a * nnoise + ni[j + 1] % nnoise) * nnoise + ni[j + 2] % nnoise) * LP
+                   : pair ? (last ? nptab_plain : nptab) + ((size_t)a * nnoise + ni[j + 1] % nnoise) * LP
+                          : (last ? ntab_plain : ntab) + (size_t)a * LP;
     j += step;
     u32 y[K];
     e += r * K;

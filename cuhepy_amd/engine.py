@@ -477,6 +477,13 @@ class LookupKey:
     def crt(self) -> bool:
         return bool(N.lib.chx_lookup_key_has_crt(self.handle))
 
+    def set_table_rows(self, rows: int) -> None:
+        """Build the row-group tables over the first `rows` table rows only
+        (0 = every row): plaintexts with zero digits above them (packed
+        embeddings of chunk_len bits use ceil(chunk_len / 8) rows) need no
+        more. Larger digits stay exact, one product per row."""
+        N.check(N.lib.chx_lookup_key_set_table_rows(self.ctx.handle, self.handle, int(rows)))
+
     def encrypt_digits(self, digits: np.ndarray, noise_idx: np.ndarray) -> np.ndarray:
         D = np.ascontiguousarray(digits, dtype=np.uint8)
         NI = np.ascontiguousarray(noise_idx, dtype=np.uint8)

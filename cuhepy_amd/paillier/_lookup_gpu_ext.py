@@ -189,6 +189,12 @@ class PaillierGPULookupClient:
             g_tab, noise = self._ensure_tables()
             k = self._keys
             self._dev = _with_crt(LookupKey(self._ctx, k["n"], k["a"], k["g_a_inv"], g_tab, noise), k["phi"])
+            # packed embeddings leave the digits above a chunk's bit string zero
+            # (a chunk of s <= chunk_len bits is an integer below 2^s,
+            # hamming/paillier.py:86-94): the row-group tables need cover no more
+            rows = -(-min(self._chunk_len, 2 * self._embed_len) // kg.MSG_BITS)
+            if rows < len(g_tab):
+                self._dev.set_table_rows(rows)
         return self._dev
 
     @property

@@ -69,10 +69,13 @@ void* chx_ctx_get_stream(chx_ctx* ctx);
 int chx_ctx_sync(chx_ctx* ctx);
 /* lanes per bignum (tpi) and window width for `op` at `limbs`; 0 = automatic.
  * For CHX_OP_LOOKUP the window is the number of 8-bit table rows one product
- * covers: 1 = single rows, 2 = row pairs, 3 = row triples, 4 = row quads
- * (16^R-entry tables per R rows, built on first use; the next smaller group
- * when the memory for one cannot be allocated); groups are used for waves
- * whose digits are all in the bit-packing alphabet; 0 = 4. For CHX_OP_MULMOD
+ * covers: 1 = single rows, 2 = row pairs, 3 = row triples, 4 = row quads,
+ * 5 = row quints with three noise factors per product (16^R-entry tables per
+ * R rows, built on first use; the next smaller group when the memory for one
+ * cannot be allocated; quints take 604 MB per 5 rows and 9.7 GB of noise
+ * triples at a 4096-bit n^2, see chx_lookup_key_set_table_rows); groups are
+ * used for waves whose digits are all in the bit-packing alphabet; 0 = 4.
+ * For CHX_OP_MULMOD
  * the window selects the form of the pointwise multiply of batches of 8,192
  * or more: 0 or 1 = one product and one fold (k_pw_mulmod), 2 = two
  * Montgomery products (k_lazy_mulmod); a forced tpi selects the canonical
@@ -259,8 +262,8 @@ int chx_paillier_decode_hamming(chx_ctx* ctx, const chx_pkey* k, const uint32_t*
  * on the device and kept resident in HBM (ensure_device_tables_, lcu:2625-2652).
  * The key also builds, on the device, a noise-pair table with plain-residue
  * copies (noise^2 entries; 40 MB each at a 4096-bit n^2); the row-group
- * tables (pairs, triples, quads) are built by the first encryption that uses
- * them (see chx_ctx_set_tuning).
+ * tables (pairs, triples, quads, quints) are built by the first encryption that
+ * uses them (see chx_ctx_set_tuning), quints with a noise-triple table.
  * chunks = 0 (tables NULL) creates a decrypt-only key. */
 int chx_lookup_key_create(chx_ctx* ctx, const uint32_t* n, int n_limbs, const uint32_t* a, int a_limbs,
                           const uint32_t* g_a_inv, const uint32_t* g_table, int chunks,
@@ -281,6 +284,15 @@ int chx_lookup_key_set_crt(chx_ctx* ctx, chx_lkey* k, const uint32_t* p, const u
                            const uint32_t* ep, const uint32_t* eq);
 /* 1 when chx_lookup_key_set_crt has succeeded on k, else 0 */
 int chx_lookup_key_has_crt(const chx_lkey* k);
+/* Rows the row-group tables cover (0, the default: every row of the g
+ * table). No counterpart in the reference, whose tables have one product per
+ * row (lcu:550-588): a memory hint for callers whose plaintexts have zero
+ * digits above a row, e.g. packed embeddings of chunk_len bits use
+ * ceil(chunk_len / 8) rows (hamming/paillier.py:26-32), so a 4096-bit n^2 key
+ * with 512-bit embeddings needs half of every group table. Digits above the
+ * covered rows stay correct: those rows take one product each. Group tables
+ * built for another coverage are freed and rebuilt on next use. */
+int chx_lookup_key_set_table_rows(chx_ctx* ctx, chx_lkey* k, int rows);
 /* c[i] = prod_j g_table[j][digits[i][j]] * prod_t noise[noise_idx[i][t]] mod n^2
  * -- encrypt_kernel (lcu:839-948), encrypt_cts_kernel (lcu:951-1011).
  * digits: count x chunks bytes; noise_idx: count x nmult bytes. */

@@ -3,9 +3,11 @@
 * The lookup client at C3's configuration (embed_len 512, key_len 1024, so
   n^2 of 4096 bits, alpha 280, 256 message chunks of which packed 512-bit
   embeddings use rows 0-127 with digits in the packing alphabet): 256
-  Bernoulli(0.5) embeddings through the row-quad, row-triple, row-pair and
-  single-row tables, each ciphertext equal to lookup.py:213-252 (the
-  oracle) for the same tables and noise indices.
+  Bernoulli(0.5) embeddings through the row-quint (with noise triples),
+  row-quad, row-triple, row-pair and single-row tables, each ciphertext
+  equal to lookup.py:213-252 (the oracle) for the same tables and noise
+  indices; quints at every noise count's remainder, and packed digits above
+  the rows the client's group tables cover.
 * Multi-chunk embeddings (embed_len > key_len, hamming/paillier.py:26-32):
   chunk_num 2 and 3 for both clients; every chunk's ciphertext equal to the
   oracle's encryption of the oracle's packing with the same randomness, and
@@ -75,11 +77,30 @@ def test_lookup_client_c3_shape_every_row_group(ctx):
     m_limbs = ints_to_limbs(ms, 128)
     W = 128
     try:
-        for rows in (0, 4, 3, 2, 1):  # 0: the default (quads); 4, 3, 2, 1 forced
+        for rows in (0, 5, 4, 3, 2, 1):  # 0: the default (quads); 5 (with noise triples), 4, 3, 2, 1 forced
             ctx.set_tuning(OP_LOOKUP, W, 0, rows)
             got = limbs_to_ints(client.encrypt_limbs(m_limbs, noise_idx=idx))
             bad = [i for i, (a, b) in enumerate(zip(got, want)) if a != b]
             assert not bad, f"rows {rows}: {len(bad)} mismatches, first at {bad[:4]}"
+        # quints: noise counts where the triples leave 1, 2 or 3 factors, or none are taken
+        ctx.set_tuning(OP_LOOKUP, W, 0, 5)
+        n2 = keys["pk"]["n_squared"]
+        for nm in (13, 12, 4, 3, 2, 1, 0):
+            sub = idx[:64, :nm]
+            got = limbs_to_ints(client.encrypt_limbs(m_limbs[:64], noise_idx=sub))
+            # the oracle takes 14 factors (lookup.py:213-252): divide out the omitted ones
+            exp = []
+            for c, ix in zip(want[:64], idx[:64]):
+                for j in ix[nm:]:
+                    c = c * pow(nt[int(j)], -1, n2) % n2
+                exp.append(c)
+            assert got == exp, f"quints with {nm} noise factors"
+        # digits in the packing alphabet above the rows the client's tables
+        # cover (rows 127-254: a packed value shifted up 127 bytes) go singly
+        hi = [m << (8 * 127) for m in ms[:64]]
+        assert max(hi) < n
+        got = limbs_to_ints(client.encrypt_limbs(ints_to_limbs(hi, 128), noise_idx=idx[:64]))
+        assert got == [O.lookup_encrypt(m, keys["pk"], gt, nt, mc, ix.tolist()) for m, ix in zip(hi, idx[:64])]
     finally:
         ctx.set_tuning(OP_LOOKUP, W, 0, 0)
     # the reference surface (bits in, injected noise through the client's rng)
