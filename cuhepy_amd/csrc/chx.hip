@@ -144,7 +144,7 @@ struct chx_ctx {
 // of its width (the configurations differ in positions per number).
 struct LazySet {
   int L32 = 0;
-  LazyDev* d[2] = {nullptr, nullptr};
+  LazyDev* d[3] = {nullptr, nullptr, nullptr};
 };
 
 struct chx_mod {
@@ -458,9 +458,13 @@ struct LazyCfg {
 
 const std::vector<LazyCfg>& lazy_cfgs(int L32) {
   // variants whose accumulators fit in VGPRs without spilling (checked with
-  // -Rpass-analysis=kernel-resource-usage); index = dispatch slot
-  static const std::vector<LazyCfg> c32 = {{28, 38, 1}, {28, 38, 2}};
-  static const std::vector<LazyCfg> c64 = {{29, 72, 4}, {29, 72, 8}};
+  // -Rpass-analysis=kernel-resource-usage); index = dispatch slot. Each list
+  // runs from the throughput shape to the latency shape for small batches:
+  // (28, 40, 8) and (29, 80, 16) give 5 positions per lane (a few more
+  // positions than the width needs, so that 8 or 16 lanes divide them), and
+  // 4x or 2x the lanes per modulus of the shape before them.
+  static const std::vector<LazyCfg> c32 = {{28, 38, 1}, {28, 38, 2}, {28, 40, 8}};
+  static const std::vector<LazyCfg> c64 = {{29, 72, 4}, {29, 72, 8}, {29, 80, 16}};
   static const std::vector<LazyCfg> c128 = {{29, 144, 8}, {29, 144, 16}};
   static const std::vector<LazyCfg> none = {};
   return L32 == 32 ? c32 : L32 == 64 ? c64 : L32 == 128 ? c128 : none;
@@ -517,7 +521,7 @@ LazySet make_lazy_set(const host::MontConsts& c, hipStream_t st) {
   ls.L32 = c.L;
   const auto& cfgs = lazy_cfgs(c.L);
   try {
-    for (size_t i = 0; i < cfgs.size() && i < 2; ++i) ls.d[i] = make_lazy_dev(c, cfgs[i].rb, cfgs[i].lp, st);
+    for (size_t i = 0; i < cfgs.size() && i < 3; ++i) ls.d[i] = make_lazy_dev(c, cfgs[i].rb, cfgs[i].lp, st);
   } catch (...) {
     for (auto& p : ls.d)
       if (p) (void)hipFree(p), p = nullptr;
@@ -571,8 +575,10 @@ size_t lazy_lds_bytes() {
     const int l__ = (L32_), i__ = (IDX_);                                                       \
     if (l__ == 32 && i__ == 0) { constexpr int RB = 28, LP = 38, TPI = 1, L32 = 32; __VA_ARGS__; }      \
     else if (l__ == 32 && i__ == 1) { constexpr int RB = 28, LP = 38, TPI = 2, L32 = 32; __VA_ARGS__; } \
+    else if (l__ == 32 && i__ == 2) { constexpr int RB = 28, LP = 40, TPI = 8, L32 = 32; __VA_ARGS__; } \
     else if (l__ == 64 && i__ == 0) { constexpr int RB = 29, LP = 72, TPI = 4, L32 = 64; __VA_ARGS__; }  \
     else if (l__ == 64 && i__ == 1) { constexpr int RB = 29, LP = 72, TPI = 8, L32 = 64; __VA_ARGS__; }  \
+    else if (l__ == 64 && i__ == 2) { constexpr int RB = 29, LP = 80, TPI = 16, L32 = 64; __VA_ARGS__; } \
     else if (l__ == 128 && i__ == 0) { constexpr int RB = 29, LP = 144, TPI = 8, L32 = 128; __VA_ARGS__; } \
     else if (l__ == 128 && i__ == 1) { constexpr int RB = 29, LP = 144, TPI = 16, L32 = 128; __VA_ARGS__; } \
     else fail(CHX_EUNSUPPORTED, "no lazy kernel for %d limbs (configuration %d)", l__, i__);   \

@@ -29,8 +29,8 @@ import random
 import pytest
 
 # (RB, LP, TPI, L32) of the lazy configurations (lazy_cfgs in chx.hip)
-BUILT = [(28, 38, 1, 32), (28, 38, 2, 32), (29, 72, 4, 64), (29, 72, 8, 64),
-         (29, 144, 8, 128), (29, 144, 16, 128)]
+BUILT = [(28, 38, 1, 32), (28, 38, 2, 32), (28, 40, 8, 32), (29, 72, 4, 64), (29, 72, 8, 64),
+         (29, 80, 16, 64), (29, 144, 8, 128), (29, 144, 16, 128)]
 # the shapes these replaced (27- and 28-bit positions at 2048- and 4096-bit
 # moduli), still valid configurations of the engine
 EARLIER = [(28, 76, 4, 64), (28, 80, 8, 64), (27, 152, 8, 128), (27, 160, 16, 128)]
