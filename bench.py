@@ -577,9 +577,8 @@ def _reencrypt_block(ctx, args, key, c_il, d_dig, d_idx, m, count: int, stride: 
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) / args.lookup_steps
     per = {}
-    for nm in names:
-        ms, nl = ctx.kernel_time(nm)
-        per[nm] = ms / max(nl, 1)
+    for nm in names:  # per step (the decryption's finish may be two launches)
+        per[nm] = ctx.kernel_time(nm)[0] / args.lookup_steps
     ctx.set_timing(False)
     kernel_ms = sum(per.values())
     kb.encrypt_digits_dev(d_dig, d_idx, 14, ref, count, stride)

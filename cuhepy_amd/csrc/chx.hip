@@ -211,6 +211,8 @@ struct chx_lkey {
   bool crt = false;
   LazySet lz_p2, lz_q2;
   uint32_t* d_crt = nullptr;  // [e_p R | e_q R] mod n^2, W limbs each
+  BN gai;                          // g_a_inv mod n
+  LookupCrtDev* d_lcrt = nullptr;  // textbook CRT route (k_lookup_crt_fast), with the factors
 };
 
 namespace {
@@ -872,9 +874,12 @@ void launch_decrypt(chx_ctx* ctx, const LazySet& lz, const ModDev* nm, const uin
 // reduce it on entry), then k_lookup_crt_finish recombines u exactly with the
 // idempotents eR = [e_p R | e_q R] mod n^2 and finishes as k_decrypt_finish,
 // so the result equals launch_decrypt's for every c in [0, n^2).
+// fast (optional, a lookup key's textbook CRT constants): k_lookup_crt_fast
+// for every item, then the exact finish only for the items it lists.
 void launch_exact_crt(chx_ctx* ctx, const LazySet& p2, const LazySet& q2, const ExpSched& s, const ModDev* n2m,
                       const uint32_t* eR, const ModDev* nm, const uint32_t* cR, int L, const uint32_t* c,
-                      uint32_t* out, size_t count, size_t stride, const uint32_t* mask, uint32_t* pop) {
+                      uint32_t* out, size_t count, size_t stride, const uint32_t* mask, uint32_t* pop,
+                      const LookupCrtDev* fast = nullptr) {
   const int H = L / 2;
   const size_t half_words = (size_t)H * stride;
   ctx->slot[S_MID0].ensure(2 * half_words * sizeof(uint32_t));
@@ -886,10 +891,26 @@ void launch_exact_crt(chx_ctx* ctx, const LazySet& p2, const LazySet& q2, const 
   uint32_t* outs[2] = {u, u + half_words};
   run_lazy(ctx, H, sets, scheds, ins, outs, 2, count, stride, his);
   const int tpi = pick_tpi(ctx, CHX_OP_MULMOD, L, count);
-  record_start(ctx, "decrypt_finish");
+  uint32_t* list = nullptr;
+  if (fast) {
+    ctx->slot[S_AUX].ensure((count + 1) * sizeof(uint32_t));
+    list = ctx->slot[S_AUX].as<uint32_t>();
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, list, 0u, 1);  // a fill kernel, not a memset
+    check_launch("k_fill_u32");
+    // the finish splits each lane group into a p- and a q-half: at least 2 lanes
+    const int ftpi = std::max(pick_tpi(ctx, CHX_OP_MULMOD, H, count), 4);
+    record_start(ctx, "decrypt_finish");
+    CHX_DISPATCH_H(H, ftpi, {
+      hipLaunchKernelGGL((k_lookup_crt_fast<H, TPI>), grid_for(count, TPI), dim3(BLOCK_THREADS), lds_bytes(H, TPI),
+                         ctx->stream, fast, u, out, (int)count, (int)stride, list, mask, pop);
+    });
+    record_stop(ctx);
+    check_launch("k_lookup_crt_fast");
+  }
+  record_start(ctx, "decrypt_finish");  // with `fast`: the listed items only
   CHX_DISPATCH(L, tpi, {
     hipLaunchKernelGGL((k_lookup_crt_finish<L, TPI>), grid_for(count, TPI), dim3(BLOCK_THREADS), lds_bytes(L, TPI),
-                       ctx->stream, n2m, eR, nm, cR, u, out, (int)count, (int)stride, mask, pop);
+                       ctx->stream, n2m, eR, nm, cR, u, out, (int)count, (int)stride, mask, pop, list);
   });
   record_stop(ctx);
   check_launch("k_lookup_crt_finish");
@@ -2258,6 +2279,7 @@ int chx_lookup_key_create(chx_ctx* ctx, const uint32_t* n, int n_limbs, const ui
       fill_moddev(md, k->n);
       k->d_n = dev_upload(&md, 1, ctx->stream);
       k->lz_n2 = make_lazy_set(k->n2, ctx->stream);
+      k->gai = host::bn_mod(host::bn_from(g_a_inv, n_limbs, W), nW);
       BN gar = host::to_mont_host(host::bn_from(g_a_inv, n_limbs, W), k->n);
       k->d_gar = dev_upload(gar.data(), W, ctx->stream);
       if (tables) {
@@ -2286,6 +2308,7 @@ int chx_lookup_key_destroy(chx_lkey* k) {
     for (auto& t : k->group)
       if (t.d) (void)hipFree(t.d);
     if (k->d_crt) (void)hipFree(k->d_crt);
+    if (k->d_lcrt) (void)hipFree(k->d_lcrt);
     free_lazy_set(k->lz_n2);
     free_lazy_set(k->lz_p2);
     free_lazy_set(k->lz_q2);
@@ -2294,6 +2317,47 @@ int chx_lookup_key_destroy(chx_lkey* k) {
   });
   delete k;
   return rc;
+}
+
+// Constants of the textbook CRT route of a lookup key (k_lookup_crt_fast),
+// uploaded; nullptr (the route stays off, every item takes the exact
+// recombination) when p or q does not fit the quarter width or has no
+// inverse modulo the other.
+LookupCrtDev* make_lookup_crt(const BN& nW, const BN& pb, const BN& qb, const BN& gai, int H, hipStream_t st) {
+  const int Hh = H / 2;
+  for (int i = Hh; i < H; ++i)
+    if (pb[i] || qb[i]) return nullptr;
+  BN ph = host::bn_from(pb.data(), Hh, Hh), qh = host::bn_from(qb.data(), Hh, Hh);
+  host::MontConsts cph = host::mont_consts(ph), cqh = host::mont_consts(qh);
+  BN nH = host::bn_from(nW.data(), H, H);
+  host::MontConsts cn = host::mont_consts(nH);
+  // inverses by Fermat (p, q prime), each checked
+  BN two(Hh, 0u);
+  two[0] = 2;
+  BN pm2 = ph, qm2 = qh;
+  host::bn_sub(pm2, two);
+  host::bn_sub(qm2, two);
+  BN qinv = host::bn_powmod(host::bn_mod(qh, ph), pm2, cph);  // q^-1 mod p
+  BN pinv = host::bn_powmod(host::bn_mod(ph, qh), qm2, cqh);  // p^-1 mod q
+  BN one(Hh, 0u);
+  one[0] = 1;
+  if (host::bn_cmp(host::bn_mulmod(qinv, host::bn_mod(qh, ph), ph), one) != 0 ||
+      host::bn_cmp(host::bn_mulmod(pinv, host::bn_mod(ph, qh), qh), one) != 0)
+    return nullptr;
+  LookupCrtDev d;
+  memset(&d, 0, sizeof d);
+  fill_moddev(d.nh, cn);
+  fill_moddev(d.ph, cph);
+  fill_moddev(d.qh, cqh);
+  const BN hp = host::bn_mulmod(qinv, host::bn_mod(gai, ph), ph);
+  const BN hq = host::bn_mulmod(pinv, host::bn_mod(gai, qh), qh);
+  fill_val(d.hpR, host::to_mont_host(hp, cph));
+  fill_val(d.hqR, host::to_mont_host(hq, cqh));
+  fill_val(d.qinvR, host::to_mont_host(qinv, cph));
+  fill_val(d.qRn, host::to_mont_host(host::bn_from(qb.data(), H, H), cn));
+  fill_val(d.pR3, host::to_mont_host(cph.R2, cph));  // R^2 R = R^3 mod p
+  fill_val(d.qR3, host::to_mont_host(cqh.R2, cqh));
+  return dev_upload(&d, 1, st);
 }
 
 int chx_lookup_key_set_crt(chx_ctx* ctx, chx_lkey* k, const uint32_t* p, const uint32_t* q, int half_limbs,
@@ -2328,6 +2392,7 @@ int chx_lookup_key_set_crt(chx_ctx* ctx, chx_lkey* k, const uint32_t* p, const u
     try {  // a failure below leaves the key without CRT factors
       k->lz_p2 = make_lazy_set(host::mont_consts(p2), ctx->stream);
       k->lz_q2 = make_lazy_set(host::mont_consts(q2), ctx->stream);
+      k->d_lcrt = make_lookup_crt(k->n.N, pb, qb, k->gai, H, ctx->stream);
     } catch (...) {
       free_lazy_set(k->lz_p2);
       free_lazy_set(k->lz_q2);
@@ -2414,7 +2479,7 @@ static void lookup_decrypt_impl(chx_lkey* k, const uint32_t* dc, uint32_t* dm, s
   if (k->crt) {
     const ExpSched& s = get_sched_l(k, pick_window(ctx, CHX_OP_MODEXP, L / 2, host::bn_bits(k->e_a)));
     launch_exact_crt(ctx, k->lz_p2, k->lz_q2, s, k->d_n2, k->d_crt, k->d_n, k->d_gar, L, dc, dm, count, stride, mask,
-                     pop);
+                     pop, k->d_lcrt);
     return;
   }
   const ExpSched& s = get_sched_l(k, pick_window(ctx, CHX_OP_MODEXP, L, host::bn_bits(k->e_a)));

@@ -48,6 +48,20 @@ struct PaillierDev {
   ValDev qRn;     // q * R_H mod n     (q*h = MontMul_n(h, qRn))
 };
 
+// Constants of the textbook CRT route of a lookup key's decryption (half
+// width H = W/2; p and q at H/2 limbs, R_h = 2^(16 H)): the layout of the
+// PaillierDev CRT part, with h_p = q^-1 g_a_inv mod p and h_q = p^-1 g_a_inv
+// mod q, and R_h^3 mod p, mod q for the test x_p = 1 mod p.
+struct LookupCrtDev {
+  ModDev nh, ph, qh;
+  ValDev hpR;     // q^-1 g_a_inv R_h mod p
+  ValDev hqR;     // p^-1 g_a_inv R_h mod q
+  ValDev qinvR;   // q^-1 R_h mod p
+  ValDev qRn;     // q R_H mod n
+  ValDev pR3;     // R_h^3 mod p
+  ValDev qR3;     // R_h^3 mod q
+};
+
 template <int L>
 __device__ __forceinline__ size_t limb_index(int limb, int item, int stride) {
   return (size_t)limb * (size_t)stride + (size_t)item;
@@ -318,6 +332,8 @@ __device__ __forceinline__ void crt_recombine(Mont<L, TPI>& M2, const u32* __res
 // x = x_p e_p + x_q e_q mod n^2 is c^a mod n^2 for every c in [0, n^2), unit
 // or not; eR = [e_p R | e_q R] mod n^2 (L limbs each). Then as
 // k_decrypt_finish: m = L(x) * g_a_inv mod n, optionally the odd-bit count.
+// list != nullptr: only the list[0] items list[1..] (k_lookup_crt_fast's
+// flagged ones), one per group; blocks past the list exit at once.
 template <int L, int TPI>
 __global__ __launch_bounds__(BLOCK_THREADS) void k_lookup_crt_finish(const ModDev* __restrict__ n2m,
                                                                      const u32* __restrict__ eR,
@@ -326,11 +342,20 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lookup_crt_finish(const ModDe
                                                                      const u32* __restrict__ U, u32* __restrict__ out,
                                                                      int count, int stride,
                                                                      const u32* __restrict__ mask,
-                                                                     u32* __restrict__ popcnt) {
+                                                                     u32* __restrict__ popcnt,
+                                                                     const u32* __restrict__ list = nullptr) {
   extern __shared__ uint4 smem[];
   constexpr int K = L / TPI;
-  const int item = group_item<TPI>();
-  const int it = item < count ? item : count - 1;
+  int item = group_item<TPI>();
+  int it;
+  if (list != nullptr) {
+    const int nl = (int)list[0];
+    if ((int)(blockIdx.x * (BLOCK_THREADS / TPI)) >= nl) return;  // the whole block is past the list
+    it = (int)list[1 + (item < nl ? item : nl - 1)];
+    item = item < nl ? it : count;  // groups past the list compute a duplicate and skip their stores
+  } else {
+    it = item < count ? item : count - 1;
+  }
   Mont<L, TPI> M2;
   mont_init<L, TPI>(M2, n2m, smem);
   u32 x[K];
@@ -342,6 +367,97 @@ __global__ __launch_bounds__(BLOCK_THREADS) void k_lookup_crt_finish(const ModDe
   if (popcnt != nullptr) {
     const u32 c = group_popcount<L, TPI>(x, mask);
     if (item < count && Grp<TPI>::rank() == 0) popcnt[item] = c;
+  }
+}
+
+// Lookup decryption by the textbook CRT route (the form of
+// paillier/lookup.py:257-278 for valid ciphertexts). U as k_lookup_crt_finish.
+// When x = c^a mod n^2 is 1 mod n (every ciphertext the scheme makes: g^a and
+// each noise^a are 1 mod n), x_p = 1 + p (qL mod p) with L = L(x), so L_p =
+// (x_p - 1)/p is an exact quotient and m = L g_a_inv mod n has m_p = L_p h_p
+// mod p (h_p = q^-1 g_a_inv), m_q likewise; Garner then gives m below n, all
+// at half or quarter width (each group's halves of TPI/2 lanes work on p and
+// q, as k_crt_finish). Test: x_p R = lo R + hi R^2 = R (mod p) with x_p = lo +
+// hi R, two quarter-width products per half. An item that fails it for p or
+// q (c invalid or a non-unit, where L(x) depends on all of x) appends its
+// index to list[1..] and counts itself in list[0] (zeroed by the caller); its
+// outputs here are placeholders, which k_lookup_crt_finish over the list
+// replaces with the exact result.
+template <int H, int TPI>
+__global__ __launch_bounds__(BLOCK_THREADS) void k_lookup_crt_fast(const LookupCrtDev* __restrict__ key,
+                                                                   const u32* __restrict__ U, u32* __restrict__ out,
+                                                                   int count, int stride, u32* __restrict__ list,
+                                                                   const u32* __restrict__ mask,
+                                                                   u32* __restrict__ popcnt) {
+  extern __shared__ uint4 smem[];
+  constexpr int HH = H / 2, TH = TPI / 2, K = H / TPI;
+  static_assert(TPI >= 2 && HH / TH == K, "two half-width halves per group");
+  const int item = group_item<TPI>();
+  const int it = item < count ? item : count - 1;
+  const int half = Grp<TPI>::rank() / TH;  // 0: p, 1: q
+  const ModDev* md = half == 0 ? &key->ph : &key->qh;
+  Mont<HH, TH> M;
+  mont_init<HH, TH>(M, md, smem);
+  const u32* Uh = U + (size_t)half * H * stride;
+  u32 u[K], hi[K];
+  load_dist<HH, TH>(u, Uh, it, stride);
+  load_dist<HH, TH>(hi, Uh + (size_t)HH * stride, it, stride);
+  // x_p R mod p = MontMul(lo, R^2) + MontMul(hi, R^3), against R mod p
+  u32 t1[K], t2[K], c[K];
+  load_const<HH, TH>(c, md->R2);
+  M.mul(t1, u, c);
+  load_const<HH, TH>(c, half == 0 ? key->pR3.v : key->qR3.v);
+  M.mul(t2, hi, c);
+  add_mod<K, TH>(t1, t2, M.N);
+  load_const<HH, TH>(c, md->R1);
+  u32 diff = 0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) diff |= t1[k] ^ c[k];
+  // the group's both halves must pass
+  const bool ok = Grp<TPI>::group_bits(__ballot(diff != 0)) == 0ull;
+  if (!ok && item < count && Grp<TPI>::rank() == 0) list[1 + atomicAdd(list, 1u)] = (u32)item;
+  (void)sub_small<K, TH>(u, 1u);
+  u32 lq[K];
+  M.exact_div(lq, u);
+  u32 m[K];
+  load_const<HH, TH>(c, half == 0 ? key->hpR.v : key->hqR.v);
+  M.mul(m, lq, c);  // m_p (low half) or m_q (high half), canonical
+  // Garner, computed by both halves modulo p
+  u32 mp[K], mq[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const u32 o = (u32)__shfl_xor((int)m[k], TH);
+    mp[k] = half == 0 ? m[k] : o;
+    mq[k] = half == 0 ? o : m[k];
+  }
+  Mont<HH, TH> P;
+  mont_init<HH, TH>(P, &key->ph, smem);
+  u32 qi[K], h1[K], h2[K];
+  load_const<HH, TH>(qi, key->qinvR.v);
+  P.mul(h1, mp, qi);  // m_p * q^-1 mod p
+  P.mul(h2, mq, qi);  // m_q * q^-1 mod p   (m_q < R_h, so the product is reduced)
+  sub_mod<K, TH>(h1, h2, P.N);
+  // to the full-width distribution: the high half's lanes hold limbs >= H/2, zero
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    h1[k] = half == 0 ? h1[k] : 0u;
+    mq[k] = half == 0 ? mq[k] : 0u;
+  }
+  Mont<H, TPI> Mn;
+  mont_init<H, TPI>(Mn, &key->nh, smem);
+  u32 qr[K], res[K];
+  load_const<H, TPI>(qr, key->qRn.v);
+  Mn.mul(res, h1, qr);  // q*h  (< n, exact)
+  (void)add_dist<K, TPI>(res, mq);
+  if (out != nullptr && item < count) {
+    // the plaintext at W = 2H limbs, the upper H zero
+    store_dist<H, TPI>(out, res, item, stride);
+    const u32 z[K] = {};
+    store_dist<H, TPI>(out + (size_t)H * stride, z, item, stride);
+  }
+  if (popcnt != nullptr) {
+    const u32 pc = group_popcount<H, TPI>(res, mask);
+    if (item < count && Grp<TPI>::rank() == 0) popcnt[item] = pc;
   }
 }
 

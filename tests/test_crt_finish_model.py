@@ -123,3 +123,54 @@ def test_paillier_exact_crt_model_matches_oracle(key_len):
     assert any(power(x, phi, planted=True) != pow(x, phi, n2) for x in xs)
     m, r = rng.randrange(n), O.random_r(n, rng)
     assert (1 + m * n) * power(r, n) % n2 == O.encrypt(m, keys["pk"], r)
+
+
+def lookup_fast_model(c, keys, quarter_bits, drop_gai=False):
+    """k_lookup_crt_fast: x_p = c^a mod p^2 passes when x_p R = lo R + hi R^2
+    = R (mod p) for x_p = lo + hi R (R = 2^quarter_bits), likewise for q; then
+    L_p = (x_p - 1)/p (the Hensel quotient of the low quarter), m_p = L_p h_p
+    mod p with h_p = q^-1 g_a_inv, m_q likewise, and Garner. Returns (passed,
+    m)."""
+    n, p, q, a = keys["pk"]["n"], keys["p"], keys["q"], keys["sk"]["a"]
+    gai = 1 if drop_gai else keys["sk"]["g_a_inv"]
+    R = 1 << quarter_bits
+    assert p < R and q < R
+
+    def half(f, g):
+        x = pow(c, a, f * f)
+        lo, hi = x % R, x // R
+        ok = (lo * R + hi * R * R) % f == R % f
+        lq = ((x - 1) % R) * pow(f, -1, R) % R
+        return ok, lq * (pow(g, -1, f) * gai % f) % f
+
+    okp, mp = half(p, q)
+    okq, mq = half(q, p)
+    h = (mp - mq) * pow(q, -1, p) % p
+    return okp and okq, mq + q * h
+
+
+@pytest.mark.parametrize("key_len", [512, 1024])
+def test_lookup_fast_model(key_len):
+    """The textbook CRT route of lookup decryption passes every valid
+    ciphertext with the oracle's plaintext (lookup.py:257-278), and fails its
+    test exactly where x = c^a mod n^2 is not 1 mod n (random c in [0, n^2),
+    non-units, 0), which the exact recombination then recomputes; a planted
+    mistake (h_p without g_a_inv) gives wrong plaintexts."""
+    keys = O.lookup_key_gen(key_len, 280, 900 + key_len, with_tables=False)
+    n, p, q, a = keys["pk"]["n"], keys["p"], keys["q"], keys["sk"]["a"]
+    n2, g, g_n = n * n, keys["pk"]["g"], keys["pk"]["g_n"]
+    qb = 8 * width_for_bits(n2.bit_length())  # quarter width in bits: p, q at H/2 limbs
+    rng = random.Random(key_len + 7)
+    ms = [0, 1, n - 1] + [rng.randrange(n) for _ in range(6)]
+    valid = [pow(g, m, n2) * pow(g_n, O.lookup_random_r(n, rng), n2) % n2 for m in ms]
+    for m, c in zip(ms, valid):
+        ok, got = lookup_fast_model(c, keys, qb)
+        assert ok and got == m == O.lookup_decrypt(c, keys)
+    odd = [0, 1, p * p, 3 * q * q, 7 * p, n2 - 1] + [rng.randrange(n2) for _ in range(6)]
+    for c in odd:
+        ok, got = lookup_fast_model(c, keys, qb)
+        assert ok == (pow(c, a, n2) % n == 1)
+        if ok:
+            assert got == O.lookup_decrypt(c, keys)
+    assert any(not lookup_fast_model(c, keys, qb)[0] for c in odd)
+    assert [lookup_fast_model(c, keys, qb, drop_gai=True)[1] for c in valid[3:]] != ms[3:]

@@ -442,6 +442,15 @@ def test_lookup_crt_decrypt_matches_full_width(ctx, key_len):
         ctx.set_tuning(OP_MODEXP, W // 2, exp_tpis[i % len(exp_tpis)], 0)
         ctx.set_tuning(OP_MULMOD, W, fin_tpis[i % len(fin_tpis)], 0)
         assert limbs_to_ints(crt.decrypt(cs)) == want, f"variant {i}"
+    # the textbook route's finish (k_lookup_crt_fast) at every lane count of
+    # the half width, the invalid and non-unit items through the exact one
+    ctx.set_tuning(OP_MODEXP, W // 2, 0, 0)
+    ctx.set_tuning(OP_MULMOD, W, 0, 0)
+    for t in Context.supported_tpi(W // 2, op=OP_MULMOD):
+        ctx.set_tuning(OP_MULMOD, W // 2, t, 0)
+        assert limbs_to_ints(crt.decrypt(cs)) == want, f"fast finish tpi {t}"
+        assert limbs_to_ints(crt.decrypt(cs[:len(ms)])) == ms, f"valid only, fast finish tpi {t}"
+    ctx.set_tuning(OP_MULMOD, W // 2, 0, 0)
     mask = sum(1 << b for b in range(1, 32 * W, 2))
     counts = crt.decode_hamming(cs, mask)
     assert [int(x) for x in counts] == [bin(m & mask).count("1") for m in want]
