@@ -277,8 +277,10 @@ int chx_lookup_width(const chx_lkey* k);
  * _reencrypt compute c^a mod p^2 and mod q^2 and recombine c^a mod n^2 before
  * the floor division L, so every c in [0, n^2) gives the result of
  * PaillierLookup.decrypt (lookup.py:257-278), which exponentiates at full
- * width; the reference has no CRT form. The secret key's phi yields p and q
- * (p + q = n - phi + 1). Needs a 2048- or 4096-bit width; refused with
+ * width; the reference has no CRT form. Items whose c^a is 1 mod n (every
+ * ciphertext the scheme makes) take the textbook CRT finish at half and
+ * quarter width instead of the recombination; the result is the same. The
+ * secret key's phi yields p and q (p + q = n - phi + 1). Needs a 2048- or 4096-bit width; refused with
  * CHX_EUNSUPPORTED otherwise, and the key keeps the full-width form. */
 int chx_lookup_key_set_crt(chx_ctx* ctx, chx_lkey* k, const uint32_t* p, const uint32_t* q, int half_limbs,
                            const uint32_t* ep, const uint32_t* eq);
